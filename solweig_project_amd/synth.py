@@ -1,0 +1,265 @@
+"""Seeded synthetic city tiles and meteorological forcing.
+
+The generator produces the host-resident inputs of one SOLWEIG full solve:
+DSM, DEM, CDSM (trees above ground), TDSM rule, walls, wall aspect, land
+cover and a 25-column UMEP-layout forcing table. It replaces the GeoTIFF /
+ERA5 preprocessing chain of the reference (``preprocessor.py``,
+``create_inputs.py``) which needs network data.
+
+Walls follow ``findwalls`` (``solweig_gpu/walls_aspect.py:27-58``): the
+4-neighbour maximum minus the cell, zeroed below the wall limit and on the
+border. The wall aspect follows the rotating-filter method of
+``filter1Goodwin_as_aspect_v3`` (``walls_aspect.py:99-166``) with the
+gradient fallback of ``get_ders`` (``walls_aspect.py:78-97``), evaluated
+with whole-plane correlations instead of a per-wall-cell loop.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy import ndimage
+
+F32 = np.float32
+
+#: Land-cover codes of the SOLWEIG class table (``landcoverclasses_2016a.txt``).
+LC_PAVED = 1
+LC_BUILDING = 2
+LC_GRASS = 5
+LC_WATER = 7
+
+
+@dataclass
+class Site:
+    """One tile of host-resident inputs (all planes row-major float32)."""
+
+    dsm: np.ndarray          # building DSM incl. ground (m)
+    dem: np.ndarray          # ground elevation (m)
+    cdsm: np.ndarray         # tree canopy height above ground (m); "temp1"
+    walls: np.ndarray        # wall height (m)
+    aspect: np.ndarray       # wall aspect (deg, 0 = north, clockwise)
+    landcover: np.ndarray    # land-cover class codes (float32 like a raster)
+    pixel_size: float = 1.0  # metres per pixel
+    latitude: float = 30.0
+    longitude: float = 0.0
+    utc_offset: float = 0.0
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def shape(self):
+        return self.dsm.shape
+
+    @property
+    def scale(self) -> float:
+        """Pixels per metre, ``1 / geotransform[1]`` in the reference."""
+        return 1.0 / self.pixel_size
+
+
+def findwalls(dsm: np.ndarray, walllimit: float = 3.0) -> np.ndarray:
+    """Wall heights from a DSM (``walls_aspect.py:27-58``).
+
+    ``max(N, S, E, W neighbours) - cell``; values below ``walllimit`` are
+    zeroed and the one-pixel border is zero. Computed in float64 like the
+    reference (``np.zeros`` planes), returned as float32 (the raster dtype).
+    """
+    a = dsm.astype(np.float64)
+    walls = np.zeros_like(a)
+    nb = np.maximum.reduce([a[:-2, 1:-1], a[2:, 1:-1], a[1:-1, :-2], a[1:-1, 2:]])
+    walls[1:-1, 1:-1] = nb
+    walls = walls - a
+    walls[walls < walllimit] = 0.0
+    walls[:, 0] = 0.0
+    walls[:, -1] = 0.0
+    walls[0, :] = 0.0
+    walls[-1, :] = 0.0
+    return walls.astype(F32)
+
+
+def _get_ders(dsm: np.ndarray, scale: float):
+    """Slope aspect in radians (``walls_aspect.py:78-97``)."""
+    dx = 1.0 / scale
+    fy, fx = np.gradient(dsm.astype(np.float64), dx, dx)
+    asp = np.arctan2(fx, fy)
+    asp = -asp
+    asp[asp < 0] += 2 * np.pi
+    return asp
+
+
+def wall_aspect(walls: np.ndarray, scale: float, dsm: np.ndarray) -> np.ndarray:
+    """Wall aspect in degrees by the rotating line-filter method.
+
+    For every rotation h in 0..179 a 9-pixel line filter (rotated with
+    bilinear interpolation, rounded) is correlated with the binary wall map;
+    the rotation with the strictly largest count wins (first maximum kept),
+    and the side of the filter with the larger DSM sum decides between
+    ``270 - h`` and ``270 - h - 180``. Cells left at 0 fall back to the DSM
+    gradient aspect. Mirrors ``walls_aspect.py:99-166``.
+    """
+    rows, cols = dsm.shape
+    filtersize = int(np.floor((scale + 1e-10) * 9))
+    if filtersize <= 2:
+        filtersize = 3
+    elif filtersize != 9 and filtersize % 2 == 0:
+        filtersize += 1
+    n = filtersize - 1
+    fceil = int(np.ceil(filtersize / 2.0))
+    ffloor = int(np.floor(filtersize / 2.0))
+
+    filt = np.zeros((filtersize, filtersize))
+    build = np.zeros((filtersize, filtersize))
+    filt[:, fceil - 1] = 1
+    build[fceil - 1, :ffloor] = 1
+    build[fceil - 1, fceil:] = 2
+
+    wallb = (walls > 0).astype(np.float64)
+    a64 = dsm.astype(np.float64)
+    y = np.zeros((rows, cols))
+    z = np.zeros((rows, cols))
+    x = np.zeros((rows, cols))
+    # interior window of the reference loop: i in [fceil-1, rows-fceil-1)
+    interior = np.zeros((rows, cols), dtype=bool)
+    interior[fceil - 1: max(fceil - 1, rows - fceil - 1), fceil - 1: max(fceil - 1, cols - fceil - 1)] = True
+    active = interior & (wallb == 1)
+    for h in range(180):
+        f1 = np.round(ndimage.rotate(filt, h, order=1, reshape=False, mode="nearest"))
+        fb = np.round(ndimage.rotate(build, h, order=0, reshape=False, mode="nearest"))
+        index = 270 - h
+        if h in (150, 30):
+            fb[:, n] = 0
+        if index == 225:
+            f1[0, 0] = f1[n, n] = 1
+        if index == 135:
+            f1[0, n] = f1[n, 0] = 1
+        s = ndimage.correlate(wallb, f1, mode="constant", cval=0.0)
+        s1 = ndimage.correlate(a64, (fb == 1).astype(np.float64), mode="constant", cval=0.0)
+        s2 = ndimage.correlate(a64, (fb == 2).astype(np.float64), mode="constant", cval=0.0)
+        upd = active & (z < s)
+        z[upd] = s[upd]
+        x[upd] = np.where(s1[upd] > s2[upd], 1, 2)
+        y[upd] = index
+    y[x == 1] -= 180
+    y[y < 0] += 360
+    asp = _get_ders(dsm, scale)
+    y += ((wallb == 1) & (y == 0)) * (asp / (math.pi / 180.0))
+    return y.astype(F32)
+
+
+def _place_buildings(rng, rows, cols, cover, dsm):
+    occupied = np.zeros((rows, cols), dtype=bool)
+    lo, hi = 8, 40
+    target = cover * rows * cols
+    guard = 0
+    while occupied.sum() < target and guard < 100000:
+        guard += 1
+        h = int(rng.integers(lo, hi + 1))
+        w = int(rng.integers(lo, hi + 1))
+        h = min(h, rows)
+        w = min(w, cols)
+        r0 = int(rng.integers(0, rows - h + 1))
+        c0 = int(rng.integers(0, cols - w + 1))
+        height = F32(rng.uniform(5.0, 40.0))
+        sl = (slice(r0, r0 + h), slice(c0, c0 + w))
+        dsm[sl] = np.maximum(dsm[sl], height)
+        occupied[sl] = True
+    return occupied
+
+
+def _place_trees(rng, rows, cols, cover, building):
+    cdsm = np.zeros((rows, cols), dtype=F32)
+    target = cover * rows * cols
+    ground = np.flatnonzero(~building.ravel())
+    if ground.size == 0:
+        return cdsm
+    guard = 0
+    while (cdsm > 0).sum() < target and guard < 100000:
+        guard += 1
+        k = int(ground[rng.integers(0, ground.size)])
+        ci, cj = divmod(k, cols)
+        r = int(rng.integers(2, 7))
+        h = float(rng.uniform(3.0, 15.0))
+        sigma = r / 2.0
+        i0, i1 = max(0, ci - r), min(rows, ci + r + 1)
+        j0, j1 = max(0, cj - r), min(cols, cj + r + 1)
+        ii, jj = np.mgrid[i0:i1, j0:j1]
+        d2 = (ii - ci) ** 2 + (jj - cj) ** 2
+        crown = (h * np.exp(-d2 / (2.0 * sigma * sigma))).astype(F32)
+        crown[d2 > r * r] = 0
+        crown[building[i0:i1, j0:j1]] = 0
+        cdsm[i0:i1, j0:j1] = np.maximum(cdsm[i0:i1, j0:j1], crown)
+    return cdsm
+
+
+def make_site(rows: int, cols: int, seed: int = 20261203, *, pixel_size: float = 1.0,
+              building_cover: float = 0.30, canopy_cover: float = 0.05,
+              with_aspect: bool = True, latitude: float = 30.0,
+              longitude: float = 0.0, utc_offset: float = 0.0) -> Site:
+    """Seeded city tile: flat ground, rectangular buildings, Gaussian crowns.
+
+    Buildings: axis-aligned rectangles with sides U{8..40} px and heights
+    U(5, 40) m placed (maximum on overlap) until ``building_cover`` of the
+    cells are roofs. Trees: crowns ``h exp(-d^2 / (2 sigma^2))`` with
+    h ~ U(3, 15) m, r ~ U{2..6} px, sigma = r/2, cut at d > r, on non-roof
+    cells, combined by maximum until ``canopy_cover`` of the cells carry
+    canopy. Land cover: roofs are class 2, other cells random 8x8 blocks of
+    paved (1), grass (5) and water (7).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = np.zeros((rows, cols), dtype=F32)
+    dsm = np.zeros((rows, cols), dtype=F32)
+    building = _place_buildings(rng, rows, cols, building_cover, dsm)
+    rng_t = np.random.Generator(np.random.PCG64(seed + 1000))
+    cdsm = _place_trees(rng_t, rows, cols, canopy_cover, building)
+    rng_l = np.random.Generator(np.random.PCG64(seed + 2000))
+    br, bc = (rows + 7) // 8, (cols + 7) // 8
+    blocks = rng_l.choice(np.array([LC_PAVED, LC_GRASS, LC_WATER], dtype=F32), size=(br, bc))
+    lc = np.kron(blocks, np.ones((8, 8), dtype=F32))[:rows, :cols].astype(F32)
+    lc[building] = LC_BUILDING
+    walls = findwalls(dsm, 3.0)
+    scale = 1.0 / pixel_size
+    if with_aspect:
+        aspect = wall_aspect(walls, scale, dsm)
+    else:
+        aspect = np.zeros_like(walls)
+    return Site(dsm=dsm, dem=dem, cdsm=cdsm, walls=walls, aspect=aspect,
+                landcover=np.ascontiguousarray(lc), pixel_size=pixel_size,
+                latitude=latitude, longitude=longitude, utc_offset=utc_offset,
+                seed=seed)
+
+
+def make_met(altitude_deg: np.ndarray, *, year: int = 2021, doy: int = 180,
+             hours=None, seed: int = 20261204, pressure_kpa: float = 101.3) -> np.ndarray:
+    """25-column UMEP forcing table for one synthetic day.
+
+    Columns used by the solver: 0 year, 1 DOY, 2 hour, 3 minute, 9 wind
+    speed, 10 RH, 11 Ta, 12 pressure (kPa), 14 global radiation, 21 diffuse
+    and 22 direct radiation (-999: recomputed from global), 23 wind
+    direction, 24 urban heat island increment. Ta = 29 + 7 sin(2 pi (h-9)/24)
+    (22-36 C), RH = 50 - 20 sin(...) (30-70 %), wind ~ U(0.5, 4) m/s,
+    clear-sky global radiation max(0, 0.75 * 1367 * sin(alt)).
+    """
+    if hours is None:
+        hours = np.arange(24)
+    hours = np.asarray(hours, dtype=np.float64)
+    n = hours.size
+    rng = np.random.Generator(np.random.PCG64(seed))
+    met = np.full((n, 25), -999.0)
+    met[:, 0] = year
+    met[:, 1] = doy
+    met[:, 2] = hours
+    met[:, 3] = 0.0
+    ph = 2.0 * np.pi * (hours - 9.0) / 24.0
+    met[:, 11] = 29.0 + 7.0 * np.sin(ph)
+    met[:, 10] = 50.0 - 20.0 * np.sin(ph)
+    met[:, 9] = rng.uniform(0.5, 4.0, n)
+    met[:, 12] = pressure_kpa
+    alt = np.asarray(altitude_deg, dtype=np.float64).reshape(-1)
+    met[:, 14] = np.maximum(0.0, 0.75 * 1367.0 * np.sin(np.radians(alt)))
+    met[:, 21] = -999.0
+    met[:, 22] = -999.0
+    met[:, 23] = rng.uniform(0.0, 360.0, n)
+    met[:, 24] = 0.0
+    for c in (4, 5, 6, 7, 8, 13, 15, 16, 17, 18, 19, 20):
+        met[:, c] = -999.0
+    return met
