@@ -1,0 +1,237 @@
+"""CPU oracle driver -- TEST INFRASTRUCTURE ONLY (never imported by the product).
+
+Runs the C oracle (``liboracle.so``, built from ``solweig_oracle.c``) over a
+full solve in the reference's order: scene compose, 153-patch SVF sweep
+(one plane-swept march per patch, then the annulus fold), then the
+time loop of ``run_utci_window`` (``solweig_gpu/utci_process.py:477-1090``):
+sun march + wall tail, ground view factors, flux assembly, Tmrt, UTCI.
+
+The per-step scalars come from the host chain ``solweig_project_amd.met``
+(pinned separately against the reference's recorded scalars) and the march
+tables from ``solweig_project_amd.skyvault`` (pinned against the
+reference's masks), so this driver checks the per-pixel physics.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from solweig_project_amd import met as M
+from solweig_project_amd import skyvault as SV
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+NP = 153
+F32 = np.float32
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB))
+        _declare(_lib)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _declare(L):
+    vp, ci, cl, cf = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float
+    L.or_march_svf.argtypes = [vp, vp, vp, ci, ci, vp, vp, vp, ci, vp, vp, vp, vp]
+    L.or_svf_fold.argtypes = [vp, vp, vp, cl, vp, vp, vp, vp, vp, vp, vp]
+    L.or_march_sun.argtypes = [vp, vp, vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, ci, cf, cf, vp, vp]
+    L.or_gvf.argtypes = [vp, vp, vp, vp, vp, vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp, cf, cf, cf, cf, vp, vp]
+    L.or_utci.argtypes = [vp, vp, vp, vp, cl, vp]
+    L.or_flux.argtypes = [ctypes.POINTER(OrStep), ctypes.POINTER(OrPatch), cl] + [vp] * 21
+    L.or_static.argtypes = [vp, vp, cl, vp, vp, vp]
+    L.or_walk_sources.argtypes = [vp, vp, vp, vp, vp, vp, vp, cl, cf, cf, cf, cf, vp, vp, vp]
+
+
+class OrStep(ctypes.Structure):
+    _fields_ = [("is_day", ctypes.c_int)] + [(k, ctypes.c_float) for k in (
+        "ta", "ta_mat", "rh", "va", "radI", "radD", "radG", "sinalt", "cosalt_k", "tan_zen",
+        "one_m_psi", "tak4")] + [("tw_first", ctypes.c_int), ("tw_update", ctypes.c_int)] + [
+        (k, ctypes.c_float) for k in ("tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
+                                      "shaded_l", "veg_l", "absK", "absL", "Fside", "Fup", "Fcyl",
+                                      "albedo_b", "ewall", "sbc")]
+
+
+class OrPatch(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("lv", "lumchi", "yi_k", "yi_l", "lsky_down",
+                                               "lsky_side", "l_sun", "alt", "ster", "cos_alt",
+                                               "sin_alt", "angl")]
+
+
+@dataclass
+class Scene:
+    a: np.ndarray
+    vegdsm: np.ndarray
+    vegdsm2: np.ndarray
+    amax: float
+    buildings: np.ndarray
+    scale: float
+
+
+def compose(dsm, dem, cdsm, scale) -> Scene:
+    """Scene compose of the driver (``utci_process.py:667-679``)."""
+    a = np.ascontiguousarray(dsm, F32)
+    t1 = np.array(cdsm, F32, copy=True)
+    t1[t1 < 0] = 0
+    t2 = np.ascontiguousarray(dem, F32)
+    vegdem = t1 + t2
+    vegdem2 = t1 * F32(0.25) + t2
+    bush = np.logical_not(vegdem2 * vegdem) * vegdem
+    if bush.max() > 0:
+        raise ValueError("bush layer present: outside the supported (bush-free) domain")
+    vegdsm = t1 + a
+    vegdsm[vegdsm == a] = 0
+    vegdsm2 = t1 * F32(0.25) + a
+    vegdsm2[vegdsm2 == a] = 0
+    amax = float(max(a.max(), vegdem.max()))
+    bu = a - t2
+    bu = np.where(bu < 2.0, F32(1.0), F32(0.0)).astype(F32)
+    return Scene(a, vegdsm.astype(F32), vegdsm2.astype(F32), amax, bu, float(scale))
+
+
+def march_svf_patch(sc: Scene, table: SV.StepTable):
+    rows, cols = sc.a.shape
+    n = rows * cols
+    sh = np.empty(n, F32); vs = np.empty(n, F32); vb = np.empty(n, F32); fw = np.empty(n, F32)
+    lib().or_march_svf(_p(sc.a), _p(sc.vegdsm), _p(sc.vegdsm2), rows, cols,
+                       _p(table.dx), _p(table.dy), _p(table.dz), table.count,
+                       _p(sh), _p(vs), _p(vb), _p(fw))
+    return sh.reshape(rows, cols), vs.reshape(rows, cols), vb.reshape(rows, cols)
+
+
+def svf(sc: Scene):
+    """153-patch sweep + fold: (fields (16, rows, cols), sh, veg, vbsh cubes (153, rows, cols))."""
+    rows, cols = sc.a.shape
+    n = rows * cols
+    vault = SV.build_skyvault()
+    offs, dx, dy, dz = SV.svf_tables(vault, sc.scale, np.float32(sc.amax), rows, cols)
+    shm = np.empty((NP, n), F32); vm = np.empty((NP, n), F32); bm = np.empty((NP, n), F32)
+    for p in range(NP):
+        s, e = offs[p], offs[p + 1]
+        t = SV.StepTable(np.ascontiguousarray(dx[s:e]), np.ascontiguousarray(dy[s:e]),
+                         np.ascontiguousarray(dz[s:e]), np.zeros(e - s, F32))
+        a, b, c = march_svf_patch(sc, t)
+        shm[p] = a.ravel(); vm[p] = b.ravel(); bm[p] = c.ravel()
+    out = np.empty((16, n), F32)
+    lib().or_svf_fold(_p(shm), _p(vm), _p(bm), n, _p(vault.w_iso), _p(vault.w_aniso),
+                      _p(vault.ring), _p(vault.n_annuli), _p(vault.dir_mask),
+                      _p(np.ascontiguousarray(sc.vegdsm2)), _p(out))
+    return out.reshape(16, rows, cols), shm, vm, bm
+
+
+#: order of the 15 fold outputs
+FOLD_NAMES = ["svf", "svfE", "svfS", "svfW", "svfN", "svfveg", "svfEveg", "svfSveg", "svfWveg",
+              "svfNveg", "svfaveg", "svfEaveg", "svfSaveg", "svfWaveg", "svfNaveg", "svftotal"]
+
+
+def _patch_struct(sp: M.StepParams, keep):
+    g = sp.geom
+    arrs = dict(lv=sp.lv, lumchi=sp.lumchi, yi_k=sp.yi_k, yi_l=sp.yi_l, lsky_down=sp.lsky_down,
+                lsky_side=sp.lsky_side, l_sun=sp.l_sun_side.astype(np.int32), alt=g.alt, ster=g.ster,
+                cos_alt=g.cos_alt, sin_alt=g.sin_alt, angl=g.angl_inc)
+    arrs = {k: np.ascontiguousarray(v) for k, v in arrs.items()}
+    keep.append(arrs)
+    return OrPatch(**{k: v.ctypes.data for k, v in arrs.items()})
+
+
+def _step_struct(sp: M.StepParams):
+    return OrStep(is_day=int(sp.is_day), ta=sp.ta, ta_mat=sp.ta_mat, rh=sp.rh, va=sp.va,
+                  radI=sp.radI, radD=sp.radD, radG=sp.radG, sinalt=sp.sinalt, cosalt_k=sp.cosalt_k,
+                  tan_zen=sp.tan_zen, one_m_psi=sp.one_m_psi, tak4=sp.tak4,
+                  tw_first=int(sp.tw_first), tw_update=int(sp.tw_update), tw_w=sp.tw_w,
+                  tw_omw=sp.tw_omw, sunlit_k=sp.sunlit_k, shaded_k=sp.shaded_k,
+                  sunlit_l=sp.sunlit_l, shaded_l=sp.shaded_l, veg_l=sp.veg_l,
+                  absK=M.ABSK, absL=M.ABSL, Fside=M.FSIDE, Fup=M.FUP, Fcyl=M.FCYL,
+                  albedo_b=M.ALBEDO_B, ewall=M.EWALL, sbc=M.SBC_PY)
+
+
+OUT_NAMES = ("tmrt", "utci", "kdown", "kup", "ldown", "lup", "shadow")
+
+
+def day(sc: Scene, fields, shm, vm, bm, walls, aspect_deg, landcover, met, sun: M.SunTable,
+        latitude: float, steps=None, record=None):
+    """Run the time loop; returns dict of (T, rows, cols) outputs."""
+    rows, cols = sc.a.shape
+    n = rows * cols
+    L = lib()
+    lc_clean = M.clean_landcover(landcover)
+    lcc = M.landcover_constants(lc_clean is not None)
+    cls = (lc_clean.astype(np.int64) if lc_clean is not None else np.zeros((rows, cols), np.int64)).ravel()
+    emis = lcc.emis[cls].astype(F32)
+    alb = lcc.alb[cls].astype(F32)
+    svf_ = np.ascontiguousarray(fields[0].ravel())
+    svfveg = np.ascontiguousarray(fields[5].ravel())
+    svfbuveg = np.empty(n, F32); tan_asvf = np.empty(n, F32); tan_sa = np.empty(n, F32)
+    L.or_static(_p(svf_), _p(svfveg), n, _p(svfbuveg), _p(tan_asvf), _p(tan_sa))
+    walls = np.ascontiguousarray(walls, F32).ravel()
+    aspect_rad = ((np.ascontiguousarray(aspect_deg, F32) * F32(np.pi)) / F32(180.0)).astype(F32).ravel()
+    bu = sc.buildings.ravel()
+    wt = SV.walk_tables(sc.scale)
+    nsec = wt.dx.shape[1]
+    loop = M.DayLoop(met, sun, latitude=latitude, landcover=lcc, scale=sc.scale, amax=sc.amax,
+                     rows=rows, cols=cols)
+    T = met.shape[0] if steps is None else steps
+    outs = {k: np.empty((T, rows, cols), F32) for k in OUT_NAMES}
+    tgmap = np.zeros(6 * n, F32)
+    sunout = np.empty(8 * n, F32); work = np.empty(25 * n, F32)
+    gvf = np.empty(15 * n, F32)
+    shadow = np.empty(n, F32); lupsrc = np.empty(n, F32); sunwall = np.empty(n, F32)
+    zeros = np.zeros(n, F32)
+    for i in range(T):
+        sp = loop.step(i)
+        keep = []
+        pst = _patch_struct(sp, keep)
+        sst = _step_struct(sp)
+        tg = sp.tg_class[cls].astype(F32) if sp.is_day else zeros
+        if sp.is_day:
+            tab = sp.table
+            L.or_march_sun(_p(sc.a), _p(sc.vegdsm), _p(sc.vegdsm2), rows, cols, _p(tab.dx), _p(tab.dy),
+                           _p(tab.dz), _p(tab.dzprev), tab.count, _p(walls), _p(aspect_rad),
+                           sp.facesh_branch, sp.azilow, sp.azihigh, _p(sunout), _p(work))
+            vegsh_sun = sunout[:n]; sh_sun = sunout[n:2 * n]; wallsun = sunout[4 * n:5 * n]
+            L.or_walk_sources(_p(sh_sun), _p(vegsh_sun), _p(wallsun), _p(walls), _p(bu), _p(emis), _p(tg),
+                              n, sp.one_m_psi, sp.ta, sp.tak4, M.SBC_PY, _p(shadow), _p(lupsrc), _p(sunwall))
+            L.or_gvf(_p(bu), _p(shadow), _p(lupsrc), _p(alb), _p(sunwall), _p(walls), _p(aspect_rad),
+                     rows, cols, _p(wt.dx), _p(wt.dy), nsec, _p(wt.branch), _p(wt.azilow), _p(wt.azihigh),
+                     _p(wt.dir_mask), wt.first, wt.second, sp.lwall, M.ALBEDO_B, _p(gvf), _p(work))
+            if record is not None:
+                record.append(dict(i=i, sun=sunout.reshape(8, rows, cols).copy(),
+                                   gvf=gvf.reshape(15, rows, cols).copy()))
+        else:
+            sh_sun = vegsh_sun = zeros
+        o = {k: np.empty(n, F32) for k in OUT_NAMES}
+        L.or_flux(ctypes.byref(sst), ctypes.byref(pst), n, _p(svfbuveg), _p(tan_asvf), _p(tan_sa),
+                  _p(bu), _p(emis), _p(alb), _p(tg), _p(shm), _p(vm), _p(bm),
+                  _p(np.ascontiguousarray(sh_sun)), _p(np.ascontiguousarray(vegsh_sun)), _p(gvf),
+                  _p(tgmap), _p(o["tmrt"]), _p(o["utci"]), _p(o["kdown"]), _p(o["kup"]),
+                  _p(o["ldown"]), _p(o["lup"]), _p(o["shadow"]))
+        for k in OUT_NAMES:
+            outs[k][i] = o[k].reshape(rows, cols)
+    return outs
+
+
+def utci(ta, rh, tmrt, va):
+    ta = np.ascontiguousarray(ta, F32); rh = np.ascontiguousarray(rh, F32)
+    tmrt = np.ascontiguousarray(tmrt, F32); va = np.ascontiguousarray(va, F32)
+    out = np.empty_like(ta)
+    lib().or_utci(_p(ta), _p(rh), _p(tmrt), _p(va), ta.size, _p(out))
+    return out
