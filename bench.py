@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""SOLWEIG full-solve benchmark on one MI355X.
+
+One "step" is one full solve of the 1000 x 1000 x 24 h synthetic tile
+(scene compose, 153-patch SVF sweep, 24 hourly steps of sun march, ground
+view factors, flux assembly, Tmrt and UTCI) with the input rasters already
+resident in HBM and the outputs (16 SVF fields, per-step Tmrt, UTCI and
+shadow) left in HBM. The host scalar chain of the day (clearness index,
+Perez luminance, tables) runs inside the timed region, overlapped with the
+SVF sweep. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rows", type=int, default=1000)
+    ap.add_argument("--cols", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=200, help="side of the CPU-oracle sample tile")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sun_for_site(lat, lon, utc, year, doy):
+    from solweig_project_amd import sunpos
+    return sunpos.day_sun_table(lat, lon, utc, year, doy)
+
+
+def cpu_baseline(rows: int, seed: int):
+    """CPU oracle (C restatement of the reference algorithm, OpenMP) on a bounded sample."""
+    import numpy as np
+    from oracle import driver as D
+    from solweig_project_amd import synth
+    site = synth.make_site(rows, rows, seed=seed)
+    sun = sun_for_site(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=seed + 1)
+    D.lib()
+    t0 = time.perf_counter()
+    sc = D.compose(site.dsm, site.dem, site.cdsm, site.scale)
+    fields, shm, vm, bm = D.svf(sc)
+    D.day(sc, fields, shm, vm, bm, site.walls, site.aspect, site.landcover, met, sun, site.latitude)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=round(rows * rows * met.shape[0] / dt / 1e6, 5), unit="Mpixel-steps/s", cores=cores,
+                kind="port", sample=f"C oracle full solve of a {rows}x{rows}x24h tile (same generator), "
+                                    f"{dt:.1f} s wall")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
+              "design-only (tiles with read-only halos, no collectives)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from solweig_project_amd import skyvault as SV
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import (DeviceScene, HipSolver, KernelTimer, TileInputs, _NoTimer,
+                                            upload_inputs)
+
+    rows, cols = args.rows, args.cols
+    log(f"generating {rows}x{cols} tile (seed {args.seed})")
+    site = synth.make_site(rows, cols, seed=args.seed)
+    sun = sun_for_site(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=args.seed + 1)
+    T = met.shape[0]
+    tile = TileInputs.from_site(site)
+    dev = torch.device("cuda:0")
+    solver = HipSolver(dev)
+    inputs = upload_inputs(tile, dev)
+    ds = DeviceScene(tile, dev, inputs)
+    out = torch.empty((T, 3, rows, cols), dtype=torch.float32, device=dev)
+    log("warmup")
+    for _ in range(args.warmup):
+        solver.solve_device(ds, met, sun, site.latitude, out=out)
+    torch.cuda.synchronize()
+    ds.check_flags()
+    timer = KernelTimer()
+    solver.timer = timer
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.solve_device(ds, met, sun, site.latitude, out=out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    solver.timer = _NoTimer()
+    ms = (t1 - t0) / args.steps * 1e3
+    value = rows * cols * T / (ms / 1e3) / 1e6
+    ksum = timer.summary()
+    log("kernels: " + ", ".join(f"{k}: {n} x {t / n:.3f} ms" for k, (n, t) in ksum.items()))
+
+    # host -> host rate (input upload + download of all outputs), reported beside value
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    ds_h = DeviceScene(tile, dev, upload_inputs(tile, dev))
+    f_dev, o_dev = solver.solve_device(ds_h, met, sun, site.latitude, out=out)
+    f_host = f_dev.cpu().numpy()
+    o_host = o_dev.cpu().numpy()
+    h1 = time.perf_counter()
+    host_value = rows * cols * T / (h1 - h0) / 1e6
+    del f_host, o_host
+
+    # roofline of the dominant kernel (algorithmic bytes / measured launch time)
+    amax, hmax = ds.amax_hmax()
+    offs, dx, dy, dz = SV.svf_tables(solver.vault, tile.scale, np.float32(amax), rows, cols)
+    n = rows * cols
+    pixel_steps = int(offs[-1]) * n
+    alg = {
+        # 3 float32 heights per executed march step (reference semantics) + heights in + masks out
+        "svf_march": 12.0 * pixel_steps + (16 + 80) * n,
+        # 18 x 22 walk gathers of 6 float planes + per-pixel compulsory traffic
+        "flux_day": (24.0 * 18 * 22 + 220) * n,
+        "flux_night": 220.0 * n,
+        "svf_fold": (80 + 64) * n,
+        "compose": (24 + 16 + 4 + 4 + 4 + 1) * n,
+    }
+    dom = max(ksum, key=lambda k: ksum[k][1])
+    launches, tot = ksum[dom]
+    avg_ms = tot / launches
+    roof = None
+    if dom in alg:
+        ach = alg[dom] / (avg_ms / 1e3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": int(alg[dom])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU oracle baseline")
+        cpu = cpu_baseline(args.cpu_rows, args.seed)
+    line = {
+        "metric": "Mpixel-steps/s (Tmrt+UTCI, 1000²×24h)",
+        "value": round(value, 3),
+        "unit": "Mpixel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded city tile generator; inputs resident in HBM, outputs left in HBM)",
+        "config": {"workload": f"full SOLWEIG solve {rows}x{cols}x{T}h (SVF sweep + sun march + GVF + "
+                               f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
+                   "time_steps": T, "pixel_size_m": site.pixel_size},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "value_host_to_host": round(host_value, 3),
+        "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

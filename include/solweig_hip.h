@@ -1,0 +1,174 @@
+/*
+ * solweig_hip.h -- C ABI of libsolweig_hip.so (MI355X / gfx950).
+ *
+ * The SOLWEIG full-solve hot path behind a plain C boundary: sky-view-factor
+ * sweep (153-patch ray march + annulus fold), sun march with the wall tail,
+ * ground-view-factor walk and flux assembly into Tmrt with a fused UTCI
+ * epilogue. All buffers are device pointers owned by the caller; the
+ * library allocates nothing on the solve path. Every entry point takes the
+ * HIP stream to enqueue on (NULL = legacy default stream), returns 0 on
+ * success or a nonzero code, and records a message retrievable with
+ * sw_last_error() (thread-local). Entry points enqueue work and do not
+ * synchronise; call sw_sync() or synchronise the stream before reading.
+ *
+ * Conventions kept from the reference's native C ABI
+ * (native/cuda/src/sw_api.cu:24-29, 64-85, 209-344): int return codes plus
+ * a last-error string, "pointers first, scalars last", version and
+ * build-flag queries.
+ */
+#ifndef SOLWEIG_HIP_H
+#define SOLWEIG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_ABI_VERSION 1
+#define SW_NPATCH 153          /* sky patches (patch_option 2) */
+#define SW_MASK_WORDS 5        /* ceil(153 / 32) */
+#define SW_MASK_PLANES 4       /* sh, vegsh, vbsh==1, vbsh==2 */
+#define SW_NFIELDS 16          /* 15 SVF fields + SVFtotal */
+#define SW_WALK_AZ 18          /* ground-view-factor search azimuths */
+#define SW_WALK_MAX 64         /* max walk steps per azimuth */
+#define SW_NCLASS 8            /* land-cover class codes 0..7 */
+
+/* Error codes */
+#define SW_OK 0
+#define SW_E_ARG 1             /* invalid argument (shape, pointer, range) */
+#define SW_E_HIP 2             /* HIP runtime error */
+#define SW_E_UNSUPPORTED 3     /* input outside the supported domain */
+
+/* Mask cube layout: masks[(plane * SW_MASK_WORDS + word) * n + pixel], bit
+ * (patch % 32) of word (patch / 32). Planes: 0 sh (1 = sky not blocked by
+ * buildings), 1 vegsh (1 = not blocked by vegetation), 2 vbsh == 1,
+ * 3 vbsh == 2 (vbsh == 0 when neither bit is set). */
+
+/* Per-time-step constants (host-computed scalar chain), one per step in
+ * device memory so a captured day replays without host work. */
+typedef struct sw_step {
+    int32_t is_day;
+    int32_t sun_off;          /* offset of this step's sun march table */
+    int32_t sun_count;        /* steps in the sun march table (day) */
+    int32_t facesh_branch;    /* 0, 1, 2: face-shading window form */
+    float azilow, azihigh;    /* face-shading window (radians, float32) */
+    float ta, ta_mat, rh, va, tak4;
+    float radI, radD, radG, sinalt, cosalt_k, tan_zen, one_m_psi;
+    int32_t tw_first, tw_update;
+    float tw_w, tw_omw;
+    float sunlit_k, shaded_k, sunlit_l, shaded_l, veg_l;
+    float tgwall, lwall;
+    float tg_class[SW_NCLASS];
+    /* per patch */
+    float lv[SW_NPATCH], lumchi[SW_NPATCH], yi_k[SW_NPATCH], yi_l[SW_NPATCH];
+    float lsky_down[SW_NPATCH], lsky_side[SW_NPATCH];
+    float alt[SW_NPATCH], ster[SW_NPATCH], cos_alt[SW_NPATCH], sin_alt[SW_NPATCH], angl[SW_NPATCH];
+    float sun_lt[SW_NPATCH];  /* x < sun_lt  <=> atan(x)*180/pi < alt  (patch sunlit) */
+    float sun_gt[SW_NPATCH];  /* x > sun_gt  <=> atan(x)*180/pi > alt  (patch shaded) */
+    int32_t l_sun[SW_NPATCH]; /* longwave: sun-facing patch window active */
+} sw_step_t;
+
+/* Step-invariant model constants and walk tables. */
+typedef struct sw_model {
+    float absK, absL, Fside, Fup, Fcyl, albedo_b, ewall, sbc;
+    float first, second;      /* walk snapshot / length (rounded, scaled) */
+    int32_t nsec;             /* walk steps per azimuth (<= SW_WALK_MAX) */
+    int32_t pad0;
+    float alb_class[SW_NCLASS], emis_class[SW_NCLASS];
+    int32_t walk_dx[SW_WALK_AZ * SW_WALK_MAX], walk_dy[SW_WALK_AZ * SW_WALK_MAX];
+    int32_t walk_branch[SW_WALK_AZ], walk_dir[SW_WALK_AZ];
+    float walk_lo[SW_WALK_AZ], walk_hi[SW_WALK_AZ];
+} sw_model_t;
+
+/* ---- runtime ---------------------------------------------------------- */
+int sw_abi_version(void);
+const char *sw_last_error(void);
+const char *sw_build_flags(void);
+int sw_device_count(int *count);
+int sw_sync(void *stream);
+
+/* ---- SVF sweep ----------------------------------------------------------
+ * Replaces shadow.svf_calculator (solweig_gpu/shadow.py:409-625) and the
+ * per-patch shadow() march (shadow.py:166-322).
+ * hpack: float4 per pixel {dsm, vegdsm, vegdsm2, 0} (sw_pack_heights).
+ * Patch tables concatenated: offs[154], dx/dy/dz[offs[153]].
+ * hmax: max over all three planes (enables the exact early exits).
+ * masks: SW_MASK_PLANES*SW_MASK_WORDS*n uint32. */
+int sw_pack_heights(const float *a, const float *vegdsm, const float *vegdsm2, void *hpack,
+                    int rows, int cols, void *stream);
+/* Scene compose on the device (utci_process.py:667-679 and the land-cover
+ * cleanup :738-744): hpack {dsm, vegdsm, vegdsm2}, vegdsm2, buildings
+ * (ground = 1), class codes, aspect in radians, and tile statistics
+ * stats[4] = {key(amax), key(hmax), flags, 0} where key() is the
+ * order-preserving uint32 image of a float. flags: 1 non-finite height,
+ * 4 bush layer present, 8 non-integer land-cover code (inputs outside the
+ * supported domain: the caller must refuse the solve). landcover may be
+ * NULL (no land cover). */
+int sw_compose(const float *dsm, const float *dem, const float *cdsm, const float *aspect_deg,
+               const float *landcover, void *hpack, float *vegdsm2, float *buildings, uint8_t *cls,
+               float *aspect_rad, uint32_t *stats, int rows, int cols, void *stream);
+/* Patch tables carry every geometrically valid step; the kernel applies
+ * the reference's amax stop test with amax from stats. */
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *dx, const int32_t *dy,
+                 const float *dz, uint32_t *masks, const uint32_t *stats, int rows, int cols,
+                 void *stream);
+/* Fold the masks into the 16 fields [svf, E, S, W, N, veg, vegE.., aveg, avegE.., total]
+ * (shadow.py:545-613), in the reference's per-annulus summation order. */
+int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso,
+                const float *w_aniso, const int32_t *ring, const int32_t *nann,
+                const int32_t *dirm, float *fields, int rows, int cols, void *stream);
+/* Expand the masks to the reference's float cubes (rows, cols, 153):
+ * shmat, vegshmat, vbshvegshmat (shadow.py:531-554). Any output may be NULL. */
+int sw_unpack_masks(const uint32_t *masks, float *shmat, float *vegshmat, float *vbshmat,
+                    int rows, int cols, void *stream);
+
+/* Single-patch marches with float outputs, the drop-in form of the
+ * reference native ABI (sw_api.cu:209-228). */
+int sw_run_march_svf_shadow(const float *a, const float *vegdem, const float *vegdem2,
+                            const int32_t *dxs, const int32_t *dys, const float *dzs,
+                            float *sh, float *vegsh, float *vbsh, int rows, int cols, int count,
+                            void *stream);
+int sw_run_march_wallheight23(const float *a, const float *vegdem, const float *vegdem2,
+                              const int32_t *dxs, const int32_t *dys, const float *dzs,
+                              const float *dzprevs, float *sh, float *vegsh, float *vbsh,
+                              int rows, int cols, int count, void *stream);
+
+/* ---- time loop ------------------------------------------------------------
+ * Static per-pixel fields of the day loop (utci_process.py:867-897):
+ * svfbuveg, tan(asvf), tan(svfalfa), and walkface: bit j set where the
+ * face-shading term of ground-view azimuth j is 0 (sunonsurface_2018a,
+ * solweig.py:236-252). */
+int sw_tile_static(const float *fields, const float *walls, const float *aspect_rad,
+                   const sw_model_t *model, float *svfbuveg, float *tan_asvf, float *tan_svfalfa,
+                   uint32_t *walkface, int rows, int cols, void *stream);
+
+/* Sun march + wall tail + ground-view-factor walk sources for one day step
+ * (shadowingfunction_wallheight_23, solweig.py:1079-1224; sunonsurface
+ * prologue, solweig.py:121-134). src: 2 x uint32 per pixel {Lup bits, flags}.
+ * sun_tab: int4 {dx, dy, dz bits, dzprev bits} per march step. */
+int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
+                const uint8_t *cls, const float *buildings, const sw_step_t *steps,
+                const sw_model_t *model, const int32_t *sun_tab, const uint32_t *stats,
+                uint32_t *src, int step, int rows, int cols, void *stream);
+
+/* Flux assembly -> Tmrt -> UTCI for one step (Solweig_2022a_calc,
+ * solweig.py:2101-2311; utci_calculator, calculate_utci.py:259-330).
+ * state: 6 planes (Tgmap1, E, S, W, N, TgOut1), read and updated.
+ * out: planes selected by out_mask bits: 1 tmrt, 2 utci, 4 shadow,
+ * 8 kdown, 16 kup, 32 ldown, 64 lup, written at out + k * n for the k-th
+ * selected plane in that bit order. */
+int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg,
+                 const float *tan_asvf, const float *tan_svfalfa, const uint32_t *walkface,
+                 const float *buildings, const uint8_t *cls, const sw_step_t *steps,
+                 const sw_model_t *model, float *state, float *out, int out_mask, int step,
+                 int rows, int cols, void *stream);
+
+/* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
+int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,
+            int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOLWEIG_HIP_H */

@@ -1,0 +1,140 @@
+"""ctypes binding of libsolweig_hip.so (the C ABI in include/solweig_hip.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point. The
+library must have been built in-tree (``python -m solweig_project_amd.build``
+or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libsolweig_hip.so"
+
+NPATCH = 153
+MASK_WORDS = 5
+MASK_PLANES = 4
+NFIELDS = 16
+WALK_AZ = 18
+WALK_MAX = 64
+NCLASS = 8
+
+_c_i32 = ctypes.c_int32
+_c_f = ctypes.c_float
+
+
+class StepStruct(ctypes.Structure):
+    """Mirror of ``sw_step_t``."""
+
+    _fields_ = [
+        ("is_day", _c_i32), ("sun_off", _c_i32), ("sun_count", _c_i32), ("facesh_branch", _c_i32),
+        ("azilow", _c_f), ("azihigh", _c_f),
+        ("ta", _c_f), ("ta_mat", _c_f), ("rh", _c_f), ("va", _c_f), ("tak4", _c_f),
+        ("radI", _c_f), ("radD", _c_f), ("radG", _c_f), ("sinalt", _c_f), ("cosalt_k", _c_f),
+        ("tan_zen", _c_f), ("one_m_psi", _c_f),
+        ("tw_first", _c_i32), ("tw_update", _c_i32), ("tw_w", _c_f), ("tw_omw", _c_f),
+        ("sunlit_k", _c_f), ("shaded_k", _c_f), ("sunlit_l", _c_f), ("shaded_l", _c_f), ("veg_l", _c_f),
+        ("tgwall", _c_f), ("lwall", _c_f),
+        ("tg_class", _c_f * NCLASS),
+        ("lv", _c_f * NPATCH), ("lumchi", _c_f * NPATCH), ("yi_k", _c_f * NPATCH), ("yi_l", _c_f * NPATCH),
+        ("lsky_down", _c_f * NPATCH), ("lsky_side", _c_f * NPATCH),
+        ("alt", _c_f * NPATCH), ("ster", _c_f * NPATCH), ("cos_alt", _c_f * NPATCH),
+        ("sin_alt", _c_f * NPATCH), ("angl", _c_f * NPATCH),
+        ("sun_lt", _c_f * NPATCH), ("sun_gt", _c_f * NPATCH),
+        ("l_sun", _c_i32 * NPATCH),
+    ]
+
+
+class ModelStruct(ctypes.Structure):
+    """Mirror of ``sw_model_t``."""
+
+    _fields_ = [
+        ("absK", _c_f), ("absL", _c_f), ("Fside", _c_f), ("Fup", _c_f), ("Fcyl", _c_f),
+        ("albedo_b", _c_f), ("ewall", _c_f), ("sbc", _c_f), ("first", _c_f), ("second", _c_f),
+        ("nsec", _c_i32), ("pad0", _c_i32),
+        ("alb_class", _c_f * NCLASS), ("emis_class", _c_f * NCLASS),
+        ("walk_dx", _c_i32 * (WALK_AZ * WALK_MAX)), ("walk_dy", _c_i32 * (WALK_AZ * WALK_MAX)),
+        ("walk_branch", _c_i32 * WALK_AZ), ("walk_dir", _c_i32 * WALK_AZ),
+        ("walk_lo", _c_f * WALK_AZ), ("walk_hi", _c_f * WALK_AZ),
+    ]
+
+
+#: exported symbols and their (restype, argtypes)
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+SIGNATURES = {
+    "sw_abi_version": (_I, []),
+    "sw_last_error": (ctypes.c_char_p, []),
+    "sw_build_flags": (ctypes.c_char_p, []),
+    "sw_device_count": (_I, [ctypes.POINTER(_I)]),
+    "sw_sync": (_I, [_VP]),
+    "sw_pack_heights": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "sw_compose": (_I, [_VP] * 11 + [_I, _I, _VP]),
+    "sw_svf_march": (_I, [_VP] * 7 + [_I, _I, _VP]),
+    "sw_svf_fold": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "sw_unpack_masks": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "sw_run_march_svf_shadow": (_I, [_VP] * 9 + [_I, _I, _I, _VP]),
+    "sw_run_march_wallheight23": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
+    "sw_tile_static": (_I, [_VP] * 8 + [_I, _I, _VP]),
+    "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
+    "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _VP]),
+    "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
+}
+
+
+class HipError(RuntimeError):
+    """A libsolweig_hip call returned a nonzero code."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed (code {code}): {msg}")
+        self.fn = fn
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: Path | str | None = None):
+    """Load the HIP library (cached). Raises if it is missing or incomplete."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: the SOLWEIG HIP library must be built for gfx950 "
+            "(run __graft_entry__.build() or python -m solweig_project_amd.build); "
+            "there is no CPU fallback")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError = symbol missing -> loud failure
+        fn.restype = res
+        fn.argtypes = args
+    if lib.sw_abi_version() != 1:
+        raise RuntimeError(f"unexpected ABI version {lib.sw_abi_version()}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def call(fn_name: str, *args):
+    lib = load()
+    rc = getattr(lib, fn_name)(*args)
+    if rc != 0:
+        raise HipError(fn_name, rc, (lib.sw_last_error() or b"").decode(errors="replace"))
+    return rc
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a torch tensor / numpy array."""
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def struct_bytes(s: ctypes.Structure) -> np.ndarray:
+    return np.frombuffer(bytes(s), dtype=np.uint8)

@@ -1,0 +1,1054 @@
+// libsolweig_hip -- SOLWEIG full-solve kernels for MI355X (gfx950, wave64).
+//
+// Kernels (one thread per output pixel, waves laid along image rows so each
+// march step of a wave reads one contiguous, coalesced row segment):
+//   k_svf_march   153-patch sky march -> packed mask cube      (shadow.py:166-322)
+//   k_svf_fold    exact annulus fold -> 15 SVF fields + total   (shadow.py:545-613)
+//   k_sun_step    sun march + wall tail -> GVF walk sources     (solweig.py:1079-1224, 121-134)
+//   k_flux_step   GVF walk + Kdown/Kup/Kside + Ldown/Lside -> Tmrt -> UTCI
+//                 (solweig.py:75-402, 451-832, 1439-1648, 2101-2299;
+//                  calculate_utci.py:259-330)
+//
+// Float discipline: built with -ffp-contract=off and without fast-math, so
+// each arithmetic operation rounds once, as the reference's elementwise
+// torch ops do. Transcendentals are evaluated in double and rounded to
+// float. Heights must be finite and non-negative (checked by the host),
+// which makes max/min NaN semantics moot and the early exits exact.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/solweig_hip.h"
+
+#define SW_BUILD_FLAGS "hipcc -O3 --offload-arch=gfx950 -ffp-contract=off -fno-fast-math -packed-fp32-ops(off)"
+
+static thread_local char g_err[512];
+
+static int set_err(int code, const char *msg) {
+    snprintf(g_err, sizeof(g_err), "%s", msg);
+    return code;
+}
+
+static int hip_check(hipError_t e, const char *where) {
+    if (e != hipSuccess) {
+        snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+        return SW_E_HIP;
+    }
+    return SW_OK;
+}
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float d_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float d_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float d_tan(float x) { return (float)tan((double)x); }
+__device__ __forceinline__ float d_atan(float x) { return (float)atan((double)x); }
+__device__ __forceinline__ float d_acos(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float d_asin(float x) { return (float)asin((double)x); }
+__device__ __forceinline__ float d_exp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float d_log(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float d_pow(float x, double e) { return (float)pow((double)x, e); }
+
+// Map a launch-order block index to a tile so that consecutive tiles land on
+// the same XCD (blocks are dealt round-robin over the 8 XCDs, each with its
+// own L2): tile = (b % 8) * (nb / 8) + b / 8. The grid is launched as a
+// multiple of 8 blocks (grid_xcd), which makes the map a bijection on
+// [0, nb); tiles past the image are idle.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int per = nb >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ bool wave_any(bool v) {
+    return __ballot(v) != 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// heights
+// ---------------------------------------------------------------------------
+__global__ void k_pack_heights(const float *__restrict__ a, const float *__restrict__ v1,
+                               const float *__restrict__ v2, float4 *__restrict__ hp, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hp[i] = make_float4(a[i], v1[i], v2[i], 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// scene compose on the device (utci_process.py:667-679, 738-744) and the
+// tile statistics: order-preserving float keys reduced with atomicMax.
+// stats[0] = key(amax), stats[1] = key(hmax), stats[2] = flags.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+    return (k & 0x80000000u) ? __uint_as_float(k & 0x7FFFFFFFu) : __uint_as_float(~k);
+}
+
+__global__ void k_stats_init(uint32_t *stats) {
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_compose(const float *__restrict__ dsm, const float *__restrict__ dem, const float *__restrict__ cdsm,
+          const float *__restrict__ aspect_deg, const float *__restrict__ lc, float4 *__restrict__ hp,
+          float *__restrict__ vegdsm2_out, float *__restrict__ bu, uint8_t *__restrict__ cls,
+          float *__restrict__ aspect_rad, uint32_t *__restrict__ stats, long n) {
+    __shared__ uint32_t s_amax, s_hmax, s_flags;
+    if (threadIdx.x == 0) { s_amax = 0u; s_hmax = 0u; s_flags = 0u; }
+    __syncthreads();
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float a = dsm[i], d = dem[i];
+        float t1 = cdsm[i];
+        if (t1 < 0.f) t1 = 0.f;
+        const float vegdem = t1 + d;
+        const float vegdem2 = t1 * 0.25f + d;
+        const float bush = (vegdem2 * vegdem == 0.f) ? vegdem : 0.f;
+        float v1 = t1 + a;
+        if (v1 == a) v1 = 0.f;
+        float v2 = t1 * 0.25f + a;
+        if (v2 == a) v2 = 0.f;
+        hp[i] = make_float4(a, v1, v2, 0.f);
+        vegdsm2_out[i] = v2;
+        bu[i] = (a - d < 2.f) ? 1.f : 0.f;
+        aspect_rad[i] = (aspect_deg[i] * 3.14159265358979323846f) / 180.0f;
+        uint32_t flags = 0u;
+        uint8_t k = 0;
+        if (lc) {
+            float g = lc[i];
+            if (g < 1.f || g > 7.f) g = 6.f;
+            if (g == 3.f || g == 4.f) g = 5.f;
+            if (g != (float)(int)g) flags |= 8u;  // non-integer land-cover code
+            k = (uint8_t)(int)g;
+        }
+        cls[i] = k;
+        if (!(isfinite(a) && isfinite(d) && isfinite(t1))) flags |= 1u;
+        if (bush > 0.f) flags |= 4u;
+        atomicMax(&s_amax, fkey(fmaxf(a, vegdem)));
+        atomicMax(&s_hmax, fkey(fmaxf(a, fmaxf(v1, v2))));
+        if (flags) atomicOr(&s_flags, flags);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(&stats[0], s_amax);
+        atomicMax(&stats[1], s_hmax);
+        if (s_flags) atomicOr(&stats[2], s_flags);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one SVF patch march for pixel (r, c); returns sh | vegsh<<1 | vbsh<<2
+// (vbsh in 0..2). Semantics of shadow(): out-of-bounds sources read an
+// exact 0, the first step carries the vegetation condition and the
+// target-local trunk gate and clears the accumulator.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t march_svf_pixel(const float4 *__restrict__ hp, int rows, int cols,
+                                                    int r, int c, float at, float veg2t, bool valid,
+                                                    const int32_t *__restrict__ dx,
+                                                    const int32_t *__restrict__ dy,
+                                                    const float *__restrict__ dz, int count,
+                                                    float amax, float hmax) {
+    float f = at;
+    int sh = 0, vs = 0, vb = 0;
+    bool active = valid;
+    const bool exit_ok = (at >= 0.f);
+    for (int s = 0; s < count; ++s) {
+        if (!wave_any(active)) break;
+        if (active) {
+            const int sr = r + dx[s], sc = c + dy[s];
+            const float dzs = dz[s];
+            const bool inb = ((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols);
+            float tmp = 0.f, tv = 0.f, tv2 = 0.f;
+            if (inb) {
+                const float4 h = hp[(long)sr * cols + sc];
+                tv = h.y - dzs;
+                tv2 = h.z - dzs;
+                tmp = h.x - dzs;
+            }
+            f = fmaxf(f, tmp);
+            sh = (f > at) ? 1 : 0;
+            const int v2 = (tv > at ? 1 : 0) - (tv2 > at ? 1 : 0);
+            vs = vs > v2 ? vs : v2;
+            if (vs * sh > 0) vs = 0;
+            vb = vs + vb;
+            if (s == 0) {
+                float fv = tv - tmp;
+                if (fv <= 0.f) fv = 1000.f;
+                if (fv < dzs) vs = 1;
+                vs = vs * ((veg2t > at) ? 1 : 0);
+                vb = 0;
+            }
+            // exact early exits: fixed point, or no source can rise above
+            // the target any more (all later sources out of bounds or lower).
+            if (sh == 1 && vs == 0) active = false;
+            else if (s >= 1 && exit_ok && (!inb || (hmax - dzs) <= at)) active = false;
+        }
+        // stop test of the reference loop: the next step runs only while
+        // amax >= dz of this step (the geometric part is cut on the host)
+        if (!(amax >= dz[s])) break;
+    }
+    const int out_sh = 1 - sh;
+    int b = vb > 0 ? 1 : vb;
+    b = b - vs;
+    const int vsf = vs > 0 ? 1 : vs;
+    const int out_vs = 1 - vsf;
+    const int out_vb = 1 - b;
+    return (uint32_t)out_sh | ((uint32_t)out_vs << 1) | ((uint32_t)out_vb << 2);
+}
+
+// grid: x = pixel tiles of 256 (XCD-remapped), y = 32-patch word group.
+__global__ void __launch_bounds__(256)
+k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs,
+            const int32_t *__restrict__ dx, const int32_t *__restrict__ dy,
+            const float *__restrict__ dz, uint32_t *__restrict__ masks,
+            const uint32_t *__restrict__ stats, int rows, int cols) {
+    const long n = (long)rows * cols;
+    const float amax = funkey(stats[0]);
+    const float hmax = funkey(stats[1]);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const long t = (long)tile * 256 + threadIdx.x;
+    const bool valid = (t < n);
+    const int r = valid ? (int)(t / cols) : 0;
+    const int c = valid ? (int)(t - (long)r * cols) : 0;
+    float at = 0.f, veg2t = 0.f;
+    if (valid) {
+        const float4 h = hp[t];
+        at = h.x;
+        veg2t = h.z;
+    }
+    const int w = blockIdx.y;
+    const int p0 = w * 32;
+    const int p1 = min(p0 + 32, SW_NPATCH);
+    uint32_t bsh = 0, bvs = 0, bv1 = 0, bv2 = 0;
+    for (int p = p0; p < p1; ++p) {
+        const int o = offs[p];
+        const int cnt = offs[p + 1] - o;
+        const uint32_t m = march_svf_pixel(hp, rows, cols, r, c, at, veg2t, valid, dx + o, dy + o, dz + o,
+                                           cnt, amax, hmax);
+        const uint32_t bit = 1u << (p - p0);
+        if (m & 1u) bsh |= bit;
+        if (m & 2u) bvs |= bit;
+        const uint32_t vbv = m >> 2;
+        if (vbv == 1u) bv1 |= bit;
+        if (vbv == 2u) bv2 |= bit;
+    }
+    if (valid) {
+        masks[(long)(0 * SW_MASK_WORDS + w) * n + t] = bsh;
+        masks[(long)(1 * SW_MASK_WORDS + w) * n + t] = bvs;
+        masks[(long)(2 * SW_MASK_WORDS + w) * n + t] = bv1;
+        masks[(long)(3 * SW_MASK_WORDS + w) * n + t] = bv2;
+    }
+}
+
+__device__ __forceinline__ void load_masks(const uint32_t *__restrict__ masks, long n, long t,
+                                           uint32_t *msh, uint32_t *mvs, uint32_t *mv1, uint32_t *mv2) {
+#pragma unroll
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        msh[w] = masks[(long)(0 * SW_MASK_WORDS + w) * n + t];
+        mvs[w] = masks[(long)(1 * SW_MASK_WORDS + w) * n + t];
+        mv1[w] = masks[(long)(2 * SW_MASK_WORDS + w) * n + t];
+        mv2[w] = masks[(long)(3 * SW_MASK_WORDS + w) * n + t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SVF fold, literal per-annulus order (bit-exact with the reference's sums)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_svf_fold(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
+           const float *__restrict__ w_iso, const float *__restrict__ w_aniso,
+           const int32_t *__restrict__ ring, const int32_t *__restrict__ nann,
+           const int32_t *__restrict__ dirm, float *__restrict__ out, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
+    load_masks(masks, n, t, msh, mvs, mv1, mv2);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f;
+    for (int p = 0; p < SW_NPATCH; ++p) {
+        const int w = p >> 5;
+        const uint32_t bit = 1u << (p & 31);
+        const float sh = (msh[w] & bit) ? 1.f : 0.f;
+        const float vs = (mvs[w] & bit) ? 1.f : 0.f;
+        const float vb = (mv1[w] & bit) ? 1.f : ((mv2[w] & bit) ? 2.f : 0.f);
+        const int i = ring[p], dm = dirm[p], na = nann[i];
+        for (int k = 0; k < na; ++k) {
+            const float wi = w_iso[i * 12 + k], wa = w_aniso[i * 12 + k];
+            s0 = s0 + wi * sh;
+            const float wsh = wa * sh;
+            if (dm & 1) s1 = s1 + wsh;
+            if (dm & 2) s2 = s2 + wsh;
+            if (dm & 4) s3 = s3 + wsh;
+            if (dm & 8) s4 = s4 + wsh;
+            v0 = v0 + wi * vs;
+            b0 = b0 + wi * vb;
+            if (dm & 1) { v1 = v1 + wa * vs; b1 = b1 + wa * vb; }
+            if (dm & 2) { v2 = v2 + wa * vs; b2 = b2 + wa * vb; }
+            if (dm & 4) { v3 = v3 + wa * vs; b3 = b3 + wa * vb; }
+            if (dm & 8) { v4 = v4 + wa * vs; b4 = b4 + wa * vb; }
+        }
+    }
+    const float cst = 3.0459e-004f;
+    s2 = s2 + cst;
+    s3 = s3 + cst;
+    s0 = s0 > 1.f ? 1.f : s0; s1 = s1 > 1.f ? 1.f : s1; s2 = s2 > 1.f ? 1.f : s2;
+    s3 = s3 > 1.f ? 1.f : s3; s4 = s4 > 1.f ? 1.f : s4;
+    const float last = (veg2[t] == 0.f) ? cst : 0.f;
+    v2 = v2 + last; v3 = v3 + last; b2 = b2 + last; b3 = b3 + last;
+    v0 = v0 > 1.f ? 1.f : v0; v1 = v1 > 1.f ? 1.f : v1; v2 = v2 > 1.f ? 1.f : v2;
+    v3 = v3 > 1.f ? 1.f : v3; v4 = v4 > 1.f ? 1.f : v4;
+    b0 = b0 > 1.f ? 1.f : b0; b1 = b1 > 1.f ? 1.f : b1; b2 = b2 > 1.f ? 1.f : b2;
+    b3 = b3 > 1.f ? 1.f : b3; b4 = b4 > 1.f ? 1.f : b4;
+    out[0 * n + t] = s0; out[1 * n + t] = s1; out[2 * n + t] = s2; out[3 * n + t] = s3; out[4 * n + t] = s4;
+    out[5 * n + t] = v0; out[6 * n + t] = v1; out[7 * n + t] = v2; out[8 * n + t] = v3; out[9 * n + t] = v4;
+    out[10 * n + t] = b0; out[11 * n + t] = b1; out[12 * n + t] = b2; out[13 * n + t] = b3; out[14 * n + t] = b4;
+    out[15 * n + t] = s0 - (1.f - v0) * (1.f - 0.03f);
+}
+
+__global__ void k_unpack(const uint32_t *__restrict__ masks, float *__restrict__ shm,
+                         float *__restrict__ vsm, float *__restrict__ vbm, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * SW_NPATCH) return;
+    const long t = i / SW_NPATCH;
+    const int p = (int)(i - t * SW_NPATCH);
+    const int w = p >> 5;
+    const uint32_t bit = 1u << (p & 31);
+    if (shm) shm[i] = (masks[(long)(0 * SW_MASK_WORDS + w) * n + t] & bit) ? 1.f : 0.f;
+    if (vsm) vsm[i] = (masks[(long)(1 * SW_MASK_WORDS + w) * n + t] & bit) ? 1.f : 0.f;
+    if (vbm) {
+        const bool one = masks[(long)(2 * SW_MASK_WORDS + w) * n + t] & bit;
+        const bool two = masks[(long)(3 * SW_MASK_WORDS + w) * n + t] & bit;
+        vbm[i] = one ? 1.f : (two ? 2.f : 0.f);
+    }
+}
+
+// single-patch march with float outputs (reference native ABI form)
+__global__ void k_march_svf_single(const float *__restrict__ a, const float *__restrict__ v1,
+                                   const float *__restrict__ v2, const int32_t *__restrict__ dx,
+                                   const int32_t *__restrict__ dy, const float *__restrict__ dz,
+                                   float *__restrict__ sh, float *__restrict__ vegsh,
+                                   float *__restrict__ vbsh, int rows, int cols, int count) {
+    const long n = (long)rows * cols;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
+    const int r = valid ? (int)(t / cols) : 0;
+    const int c = valid ? (int)(t - (long)r * cols) : 0;
+    const float at = valid ? a[t] : 0.f;
+    const float veg2t = valid ? v2[t] : 0.f;
+    float f = at;
+    int s_sh = 0, vs = 0, vb = 0;
+    for (int s = 0; s < count && valid; ++s) {
+        const int sr = r + dx[s], sc = c + dy[s];
+        const float dzs = dz[s];
+        float tmp = 0.f, tv = 0.f, tv2 = 0.f;
+        if (((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols)) {
+            const long q = (long)sr * cols + sc;
+            tv = v1[q] - dzs;
+            tv2 = v2[q] - dzs;
+            tmp = a[q] - dzs;
+        }
+        f = fmaxf(f, tmp);
+        s_sh = (f > at) ? 1 : 0;
+        const int g2 = (tv > at ? 1 : 0) - (tv2 > at ? 1 : 0);
+        vs = vs > g2 ? vs : g2;
+        if (vs * s_sh > 0) vs = 0;
+        vb = vs + vb;
+        if (s == 0) {
+            float fv = tv - tmp;
+            if (fv <= 0.f) fv = 1000.f;
+            if (fv < dzs) vs = 1;
+            vs = vs * ((veg2t > at) ? 1 : 0);
+            vb = 0;
+        }
+    }
+    if (!valid) return;
+    int b = vb > 0 ? 1 : vb;
+    b = b - vs;
+    const int vsf = vs > 0 ? 1 : vs;
+    sh[t] = (float)(1 - s_sh);
+    vegsh[t] = (float)(1 - vsf);
+    vbsh[t] = (float)(1 - b);
+}
+
+// sun march core shared by the single-step ABI and the day step kernel.
+struct SunOut {
+    int sh, vs, vb;
+    float shvo, shvoveg;  // f - a and the vegetation shadow volume (before * vegsh)
+};
+
+template <bool EXITS>
+__device__ __forceinline__ SunOut march_sun_pixel(const float4 *__restrict__ hp, int rows, int cols, int r,
+                                                  int c, float at, float vegt, bool valid, bool wall,
+                                                  const int4 *__restrict__ tab, int count, float amax,
+                                                  float hmax) {
+    float f = at, svv = vegt;
+    int sh = 0, vs = 0, vb = 0;
+    bool active = valid;
+    const bool exit_ok = EXITS && (at >= 0.f) && (vegt >= 0.f);
+    for (int s = 0; s < count; ++s) {
+        if (EXITS) {
+            if (!wave_any(active)) break;
+        }
+        if (active) {
+            const int4 e = tab[s];
+            const int sr = r + e.x, sc = c + e.y;
+            const float dzs = __int_as_float(e.z), dzp = __int_as_float(e.w);
+            const bool inb = ((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols);
+            float tmp = 0.f, tv = 0.f, tv2 = 0.f, lf = 0.f, lg = 0.f;
+            if (inb) {
+                const float4 h = hp[(long)sr * cols + sc];
+                tv = h.y - dzs;
+                tv2 = h.z - dzs;
+                tmp = h.x - dzs;
+                lf = h.y - dzp;
+                lg = h.z - dzp;
+            }
+            f = fmaxf(f, tmp);
+            svv = fmaxf(svv, tv);
+            sh = (f > at) ? 1 : 0;
+            int v2 = (tv > at ? 1 : 0) + (tv2 > at ? 1 : 0) + (lf > at ? 1 : 0) + (lg > at ? 1 : 0);
+            if (v2 == 4) v2 = 0;
+            if (v2 > 0) v2 = 1;
+            vs = vs > v2 ? vs : v2;
+            if (vs * sh > 0) vs = 0;
+            vb = vb + vs;
+            if (EXITS) {
+                if (exit_ok && !inb) active = false;
+                else if (!wall && sh == 1 && vs == 0) active = false;
+                else if (!wall && exit_ok && (hmax - dzp) <= at) active = false;
+            }
+        }
+        if (!(amax >= __int_as_float(tab[s].z))) break;
+    }
+    SunOut o;
+    o.sh = sh; o.vs = vs; o.vb = vb; o.shvo = f - at; o.shvoveg = svv;
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// static per-pixel fields of the day loop
+// ---------------------------------------------------------------------------
+__global__ void k_tile_static(const float *__restrict__ fields, float *__restrict__ svfbuveg,
+                              float *__restrict__ tan_asvf, float *__restrict__ tan_sa, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float svf = fields[t], svfveg = fields[5 * n + t];
+    svfbuveg[t] = svf - (1.0f - svfveg) * 0.97f;
+    tan_asvf[t] = d_tan(d_acos(sqrtf(svf)));
+    float tmp = svf + svfveg - 1.0f;
+    if (tmp < 0.f) tmp = 0.f;
+    tan_sa[t] = d_tan(d_asin(d_exp(d_log(1.0f - tmp) / 2.0f)));
+}
+
+// ---------------------------------------------------------------------------
+// day step: sun march + wall tail -> walk sources {Lup bits, flags}
+// flags: bit0 sh, bit1 vegsh (march outputs), bit2 sunwall, bit3 ground,
+//        bits 4..6 land-cover class.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pow4(float x) {
+    const double d = (double)x;
+    const double d2 = d * d;
+    return (float)(d2 * d2);
+}
+
+__global__ void __launch_bounds__(256)
+k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
+           const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
+           const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
+           const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
+           const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols) {
+    const long n = (long)rows * cols;
+    const float amax = funkey(stats[0]);
+    const float hmax = funkey(stats[1]);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const long t = (long)tile * 256 + threadIdx.x;
+    const bool valid = (t < n);
+    const sw_step_t *sp = steps + step;
+    const int r = valid ? (int)(t / cols) : 0;
+    const int c = valid ? (int)(t - (long)r * cols) : 0;
+    float at = 0.f, vegt = 0.f, w = 0.f;
+    if (valid) {
+        const float4 h = hp[t];
+        at = h.x;
+        vegt = h.y;
+        w = walls[t];
+    }
+    const SunOut o = march_sun_pixel<true>(hp, rows, cols, r, c, at, vegt, valid, w > 0.f,
+                                           suntab + sp->sun_off, sp->sun_count, amax, hmax);
+    if (!valid) return;
+    // finalisation (solweig.py:1198-1219)
+    const int shv = 1 - o.sh;
+    const int vsf = o.vs > 0 ? 1 : o.vs;
+    const float svv = (o.shvoveg - at) * (float)vsf;
+    const int vso = 1 - vsf;
+    const float wallbol = (w > 0.f) ? 1.f : 0.f;
+    const float asp = aspect[t];
+    float facesh;
+    if (sp->facesh_branch == 0) facesh = (float)((asp < sp->azilow) || (asp >= sp->azihigh)) - wallbol + 1.f;
+    else facesh = (float)((asp > sp->azilow) || (asp <= sp->azihigh)) * -1.f + 1.f;
+    float wallsun = w - o.shvo;
+    if (wallsun < 0.f) wallsun = 0.f;
+    if (facesh == 1.f) wallsun = 0.f;
+    const float wallsh = w - wallsun;
+    float wallshve = svv * wallbol;
+    wallshve = wallshve - wallsh;
+    if (wallshve < 0.f) wallshve = 0.f;
+    wallsun = wallsun - wallshve;
+    if (wallsun < 0.f) wallsun = 0.f;
+    // walk sources (sunonsurface_2018a prologue)
+    const float b = bu[t];
+    const int k = cls[t];
+    const float shadow = (float)shv - (1.f - (float)vso) * sp->one_m_psi;
+    const float se = model->sbc * model->emis_class[k];
+    const float x = sp->tg_class[k] * shadow + sp->ta + 273.15f;
+    const float lup = se * pow4(x) - se * sp->tak4;
+    const float q = wallsun / w * b;
+    const uint32_t sunwall = (q == 1.f) ? 1u : 0u;
+    const uint32_t flags = (uint32_t)shv | ((uint32_t)vso << 1) | (sunwall << 2) |
+                           ((b == 1.f ? 1u : 0u) << 3) | ((uint32_t)k << 4);
+    src[t] = make_uint2(__float_as_uint(lup), flags);
+}
+
+// ---------------------------------------------------------------------------
+// UTCI
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float utci_point(float ta, float rh, float tmrt, float va) {
+    if (ta <= -999.f || rh <= -999.f || va <= -999.f || tmrt <= -999.f) return -999.f;
+    const float tk = ta + 273.15f;
+    float es = 2.7150305f * d_log(tk);
+    es = es + -2.8365744E3f * (1.f / (tk * tk));
+    es = es + -6.028076559E3f * (1.f / tk);
+    es = es + 1.954263612E1f * 1.f;
+    es = es + -2.737830188E-2f * tk;
+    es = es + 1.6261698E-5f * (tk * tk);
+    es = es + 7.0229056E-10f * (tk * tk * tk);
+    es = es + -1.8680009E-13f * d_pow(tk, 4.0);
+    es = d_exp(es) * 0.01f;
+    const float ehPa = es * rh / 100.f;
+    const float dtm = tmrt - ta;
+    const float pa = ehPa / 10.0f;
+    const float ta1 = ta, ta2 = ta * ta, ta3 = ta * ta * ta, ta4 = d_pow(ta, 4.0), ta5 = d_pow(ta, 5.0),
+                ta6 = d_pow(ta, 6.0);
+    const float va1 = va, va2 = va * va, va3 = va * va * va, va4 = d_pow(va, 4.0), va5 = d_pow(va, 5.0),
+                va6 = d_pow(va, 6.0);
+    const float dt1 = dtm, dt2 = dtm * dtm, dt3 = dtm * dtm * dtm, dt4 = d_pow(dtm, 4.0),
+                dt5 = d_pow(dtm, 5.0), dt6 = d_pow(dtm, 6.0);
+    const float pa1 = pa, pa2 = pa * pa, pa3 = pa * pa * pa, pa4 = d_pow(pa, 4.0), pa5 = d_pow(pa, 5.0),
+                pa6 = d_pow(pa, 6.0);
+    float u = ta;
+#include "utci_poly.inc"
+    (void)ta6; (void)va6; (void)dt6; (void)pa6;
+    return u;
+}
+
+__global__ void k_utci(const float *__restrict__ ta, const float *__restrict__ rh,
+                       const float *__restrict__ tmrt, const float *__restrict__ va,
+                       float *__restrict__ out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = utci_point(ta[i], rh[i], tmrt[i], va[i]);
+}
+
+// ---------------------------------------------------------------------------
+// flux step
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mask_bits(const uint32_t *msh, const uint32_t *mvs, const uint32_t *mv1,
+                                          const uint32_t *mv2, int p, float *sh, float *vs, float *vb) {
+    const int w = p >> 5;
+    const uint32_t bit = 1u << (p & 31);
+    *sh = (msh[w] & bit) ? 1.f : 0.f;
+    *vs = (mvs[w] & bit) ? 1.f : 0.f;
+    *vb = (mv1[w] & bit) ? 1.f : ((mv2[w] & bit) ? 2.f : 0.f);
+}
+
+__global__ void __launch_bounds__(256)
+k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
+            const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
+            const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
+            const float *__restrict__ bu, const uint8_t *__restrict__ cls,
+            const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
+            float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int rows,
+            int cols) {
+    __shared__ float s_alb[SW_NCLASS];
+    if (threadIdx.x < SW_NCLASS) s_alb[threadIdx.x] = model->alb_class[threadIdx.x];
+    __syncthreads();
+    const long n = (long)rows * cols;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const long t = (long)tile * 256 + threadIdx.x;
+    if (t >= n) return;
+    const sw_step_t *sp = steps + step;
+    const sw_model_t *md = model;
+    const int r = (int)(t / cols), c = (int)(t - (long)r * cols);
+    const float b_t = bu[t];
+    const int k_t = cls[t];
+    const float se_t = md->sbc * md->emis_class[k_t];
+    float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f, Kside = 0.f;
+    float Lup, LupE, LupS, LupW, LupN, shadow = 0.f;
+    uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
+    load_masks(masks, n, t, msh, mvs, mv1, mv2);
+    const float hsvf = tan_asvf[t];
+    if (sp->is_day) {
+        const uint2 me = src[t];
+        const uint32_t fl_t = me.y;
+        const float lup_t = __uint_as_float(me.x);
+        const float omp = sp->one_m_psi;
+        shadow = (float)(fl_t & 1u) - (1.f - (float)((fl_t >> 1) & 1u)) * omp;
+        const float alb_t = s_alb[k_t];
+        // ---- ground view factors (gvf_2018a): 18 azimuths x nsec steps ----
+        float gL = 0.f, gA = 0.f, gN = 0.f, gLE = 0.f, gAE = 0.f, gNE = 0.f, gLS = 0.f, gAS = 0.f, gNS = 0.f;
+        float gLW = 0.f, gAW = 0.f, gNW = 0.f, gLN = 0.f, gAN = 0.f, gNN = 0.f;
+        const int nsec = md->nsec;
+        const float first = md->first, second = md->second;
+        const float fp1 = first + 1.f, sp1 = second + 1.f;
+        const float lwall = sp->lwall, albb = md->albedo_b;
+        const uint32_t wf = walkface[t];
+        const float nb = b_t * -1.f + 1.f;
+        for (int j = 0; j < SW_WALK_AZ; ++j) {
+            float tb = 0.f, ts = 0.f, tl = 0.f, ta_ = 0.f, tn = 0.f, tw = 0.f;
+            float f = b_t, bub = 0.f, bubw = 0.f;
+            float wsh = 0.f, wwall = 0.f, wlupsh = 0.f, wlwall = 0.f, walbsh = 0.f, walbwall = 0.f,
+                  walbnosh = 0.f, walbwallnosh = 0.f;
+            float f_wwall = 0.f, f_wsh = 0.f, f_wlwall = 0.f, f_wlupsh = 0.f, f_walbwall = 0.f,
+                  f_walbsh = 0.f, f_walbwallnosh = 0.f, f_walbnosh = 0.f;
+            bool live = true;
+            for (int s = 0; s < nsec; ++s) {
+                const int sr = r + md->walk_dx[j * SW_WALK_MAX + s];
+                const int sc = c + md->walk_dy[j * SW_WALK_MAX + s];
+                // a source that left the grid stays out (offsets grow monotonically);
+                // the temporaries then keep the last in-grid values (stale border)
+                live = live && ((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols);
+                if (live) {
+                    const uint2 e = src[(long)sr * cols + sc];
+                    const uint32_t fl = e.y;
+                    const float shd = (float)(fl & 1u) - (1.f - (float)((fl >> 1) & 1u)) * omp;
+                    const float al = s_alb[(fl >> 4) & 7u];
+                    tb = (fl & 8u) ? 1.f : 0.f;
+                    ts = shd;
+                    tl = __uint_as_float(e.x);
+                    ta_ = al * shd;
+                    tn = al;
+                    tw = (fl & 4u) ? 1.f : 0.f;
+                }
+                f = fminf(f, tb);
+                wsh = wsh + ts * f;
+                wlupsh = wlupsh + tl * f;
+                walbsh = walbsh + ta_ * f;
+                walbnosh = walbnosh + tn * f;
+                const float tempb = tw * f;
+                const float tempbwall = f * -1.f + 1.f;
+                bub = ((tempb + bub) > 0.f) ? 1.f : 0.f;
+                bubw = ((tempbwall + bubw) > 0.f) ? 1.f : 0.f;
+                wlwall = wlwall + bub * lwall;
+                walbwall = walbwall + bub * albb;
+                wwall = wwall + bub;
+                walbwallnosh = walbwallnosh + bubw * albb;
+                if ((float)(s + 1) <= first) {
+                    f_wwall = wwall; f_wsh = wsh; f_wlwall = wlwall; f_wlupsh = wlupsh;
+                    f_walbwall = walbwall; f_walbsh = walbsh; f_walbwallnosh = walbwallnosh;
+                    f_walbnosh = walbnosh;
+                }
+            }
+            // keep == 1  <=>  wwall == second and facesh == 0 (bit j of walkface)
+            const bool keep1 = (wwall == second) && ((wf >> j) & 1u);
+            const float wsi1 = (f_wwall > 0.f) ? 1.f : 0.f;
+            const float wi1 = (f_walbwallnosh > 0.f) ? 1.f : 0.f;
+            const float wsi2 = (wwall > 0.f) ? 1.f : 0.f;
+            const float wi2 = (walbwallnosh > 0.f) ? 1.f : 0.f;
+            const float lw2 = keep1 ? 0.f : wlwall;
+            const float aw2 = keep1 ? 0.f : walbwall;
+            const float gl1 = ((f_wlwall + f_wlupsh) / fp1) * wsi1 + (f_wlupsh) / first * (wsi1 * -1.f + 1.f);
+            const float gl2 = ((lw2 + wlupsh) / sp1) * wsi2 + (wlupsh) / second * (wsi2 * -1.f + 1.f);
+            const float ga1 = ((f_walbwall + f_walbsh) / fp1) * wsi1 + (f_walbsh) / first * (wsi1 * -1.f + 1.f);
+            const float ga2 = ((aw2 + walbsh) / sp1) * wsi2 + (walbsh) / second * (wsi2 * -1.f + 1.f);
+            const float gn1 = ((f_walbwallnosh + f_walbnosh) / fp1) * wi1 + (f_walbnosh) / first * (wi1 * -1.f + 1.f);
+            const float gn2 = ((walbwallnosh + walbnosh) / second) * wi2 + (walbnosh) / second * (wi2 * -1.f + 1.f);
+            float gLj = (gl1 * 0.5f + gl2 * 0.4f) / 0.9f;
+            gLj = gLj + lup_t * nb;
+            float gAj = (ga1 * 0.5f + ga2 * 0.4f) / 0.9f;
+            gAj = gAj + alb_t * nb * shadow;
+            float gNj = (gn1 * 0.5f + gn2 * 0.4f) / 0.9f;
+            gNj = gNj * b_t + alb_t * nb;
+            gL += gLj; gA += gAj; gN += gNj;
+            const int dm = md->walk_dir[j];
+            if (dm & 1) { gLE += gLj; gAE += gAj; gNE += gNj; }
+            if (dm & 2) { gLS += gLj; gAS += gAj; gNS += gNj; }
+            if (dm & 4) { gLW += gLj; gAW += gAj; gNW += gNj; }
+            if (dm & 8) { gLN += gLj; gAN += gAj; gNN += gNj; }
+        }
+        const float lup_air = se_t * sp->tak4;
+        const float gLup = gL / 18.f + lup_air;
+        const float galb = gA / 18.f, gnosh = gN / 18.f;
+        const float gLupE = gLE / 9.f + lup_air, galbE = gAE / 9.f, gnoshE = gNE / 9.f;
+        const float gLupS = gLS / 9.f + lup_air, galbS = gAS / 9.f, gnoshS = gNS / 9.f;
+        const float gLupW = gLW / 9.f + lup_air, galbW = gAW / 9.f, gnoshW = gNW / 9.f;
+        const float gLupN = gLN / 9.f + lup_air, galbN = gAN / 9.f, gnoshN = gNN / 9.f;
+        // ---- thermal delay (TsWaveDelay_2015a x 6, shared weights) ----
+        const float tg0[6] = {gLup, gLupE, gLupS, gLupW, gLupN, sp->tg_class[k_t] * shadow + sp->ta};
+        float L[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            float T1 = sp->tw_first ? tg0[q] : state[(long)q * n + t];
+            if (sp->tw_update) {
+                T1 = tg0[q] * sp->tw_omw + T1 * sp->tw_w;
+                L[q] = T1;
+            } else {
+                L[q] = tg0[q] * sp->tw_omw + T1 * sp->tw_w;
+            }
+            state[(long)q * n + t] = T1;
+        }
+        Lup = L[0]; LupE = L[1]; LupS = L[2]; LupW = L[3]; LupN = L[4];
+        // ---- cylinder wedge F_sh (cylindric_wedge) ----
+        const float tan_a = tan_sa[t], tb = sp->tan_zen;
+        const float xa = 1.f - 2.f / (tan_a * tb);
+        const float ha = 2.f / (tan_a * tb);
+        const float ba = 1.f / tan_a;
+        const float hkil = 2.f * ba * ha;
+        float ukil = 0.f;
+        if (xa < 0.f) {
+            const float qa = tb / 2.f;
+            const float Za = sqrtf(ba * ba - (qa * qa) / 4.f);
+            const float phi = d_atan(Za / qa);
+            const float cphi = d_cos(phi);
+            const float A = (d_sin(phi) - phi * cphi) / (1.f - cphi);
+            ukil = 2.f * ba * xa * A;
+        }
+        const float Ssurf = hkil + ukil;
+        const float two_pi = 6.28318530717958647692f;
+        float F_sh = (two_pi * ba - Ssurf) / (two_pi * ba);
+        if (F_sh != F_sh) F_sh = 0.5f;
+        // ---- diffuse sky and Kside over the 153 patches ----
+        float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
+        const float sunlit_k = sp->sunlit_k, shaded_k = sp->shaded_k;
+        for (int p = 0; p < SW_NPATCH; ++p) {
+            float sh, vs, vb;
+            mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
+            const float ster = sp->ster[p], calt = sp->cos_alt[p];
+            const float diffsh = sh - (1.f - vs) * 0.97f;
+            aniLum = aniLum + diffsh * sp->lv[p];
+            KsideD = KsideD + diffsh * sp->lumchi[p] * sp->angl[p] * ster;
+            const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
+            Kref_veg = Kref_veg + shaded_k * tvs * ster * calt;
+            const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
+            const float x = hsvf + sp->yi_k[p];
+            const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
+            const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
+            Kref_sun = Kref_sun + sunlit_k * sunlit * tsh * ster * calt;
+            Kref_sh = Kref_sh + shaded_k * shaded * tsh * ster * calt;
+        }
+        const float dRad = aniLum * sp->radD;
+        const float sb = 1.f - svfbuveg[t];
+        const float mix = sp->radG * (1.f - F_sh) + sp->radD * F_sh;
+        Kdown = sp->radI * shadow * sp->sinalt + dRad + md->albedo_b * sb * mix;
+        const float kdiff = sp->radD * svfbuveg[t] + md->albedo_b * sb * mix;
+        Kup = galb * sp->radI * sp->sinalt + kdiff * gnosh;
+        KupE = galbE * sp->radI * sp->sinalt + kdiff * gnoshE;
+        KupS = galbS * sp->radI * sp->sinalt + kdiff * gnoshS;
+        KupW = galbW * sp->radI * sp->sinalt + kdiff * gnoshW;
+        KupN = galbN * sp->radI * sp->sinalt + kdiff * gnoshN;
+        const float KsideI = shadow * sp->radI * sp->cosalt_k;
+        Kside = KsideI + KsideD + Kref_sun + Kref_sh + Kref_veg;
+    } else {
+        const float x = sp->ta + 273.15f;
+        Lup = se_t * pow4(x);
+        LupE = LupS = LupW = LupN = Lup;
+    }
+    // ---- longwave: define_patch_characteristics, two passes ----
+    float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f, Ldown_ref = 0.f;
+    float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f, Lside_ref = 0.f;
+    const float veg_l = sp->veg_l, sunlit_l = sp->sunlit_l, shaded_l = sp->shaded_l;
+    for (int p = 0; p < SW_NPATCH; ++p) {
+        float sh, vs, vb;
+        mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
+        const float ster = sp->ster[p], calt = sp->cos_alt[p], salt = sp->sin_alt[p];
+        const float tsky = (sh == 1.f && vs == 1.f) ? 1.f : 0.f;
+        Ldown_sky = Ldown_sky + tsky * sp->lsky_down[p];
+        Lside_sky = Lside_sky + tsky * sp->lsky_side[p];
+        const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
+        Lside_veg = Lside_veg + veg_l * ster * calt * tvs;
+        Ldown_veg = Ldown_veg + veg_l * ster * salt * tvs;
+        const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
+        if (sp->l_sun[p]) {
+            const float x = hsvf + sp->yi_l[p];
+            const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
+            const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
+            Lside_sun = Lside_sun + sunlit_l * sunlit * ster * calt * tsh;
+            Lside_sh = Lside_sh + shaded_l * shaded * ster * calt * tsh;
+            Ldown_sun = Ldown_sun + sunlit_l * sunlit * ster * salt * tsh;
+            Ldown_sh = Ldown_sh + shaded_l * shaded * ster * salt * tsh;
+        } else {
+            Lside_sh = Lside_sh + shaded_l * ster * calt * tsh;
+            Ldown_sh = Ldown_sh + shaded_l * ster * salt * tsh;
+        }
+    }
+    const float pi_f = 3.14159265358979323846f;
+    const float refl = ((Ldown_sky + Lup) * (1.f - md->ewall) * 0.5f) / pi_f;
+    for (int p = 0; p < SW_NPATCH; ++p) {
+        float sh, vs, vb;
+        mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
+        const float tr = (sh == 0.f || vs == 0.f || vb == 0.f) ? 1.f : 0.f;
+        Lside_ref = Lside_ref + refl * sp->ster[p] * sp->cos_alt[p] * tr;
+        Ldown_ref = Ldown_ref + refl * sp->ster[p] * sp->sin_alt[p] * tr;
+    }
+    const float Lside = Lside_sky + Lside_veg + Lside_sh + Lside_sun + Lside_ref;
+    const float Ldown = Ldown_sky + Ldown_veg + Ldown_sh + Ldown_sun + Ldown_ref;
+    // ---- Tmrt (standing cylinder, anisotropic sky) ----
+    const float Keast = KupE * 0.5f, Kwest = KupW * 0.5f, Knorth = KupN * 0.5f, Ksouth = KupS * 0.5f;
+    const float Least = LupE * 0.5f, Lwest = LupW * 0.5f, Lnorth = LupN * 0.5f, Lsouth = LupS * 0.5f;
+    const float Sstr = md->absK * (Kside * md->Fcyl + (Kdown + Kup) * md->Fup + (Knorth + Keast + Ksouth + Kwest) * md->Fside)
+                     + md->absL * ((Ldown + Lup) * md->Fup + Lside * md->Fcyl + (Lnorth + Least + Lsouth + Lwest) * md->Fside);
+    const float tmrt = sqrtf(sqrtf(Sstr / (md->absL * md->sbc))) - 273.2f;
+    int k = 0;
+    if (out_mask & 1) out[(long)(k++) * n + t] = tmrt;
+    if (out_mask & 2) {
+        const float u = (b_t == 1.f) ? utci_point(sp->ta_mat, sp->rh, tmrt, sp->va) : __builtin_nanf("");
+        out[(long)(k++) * n + t] = u;
+    }
+    if (out_mask & 4) out[(long)(k++) * n + t] = shadow;
+    if (out_mask & 8) out[(long)(k++) * n + t] = Kdown;
+    if (out_mask & 16) out[(long)(k++) * n + t] = Kup;
+    if (out_mask & 32) out[(long)(k++) * n + t] = Ldown;
+    if (out_mask & 64) out[(long)(k++) * n + t] = Lup;
+}
+
+// walk face-shading bits: bit j set where facesh of walk azimuth j is 0
+__global__ void k_walk_face(const float *__restrict__ walls, const float *__restrict__ aspect,
+                            const sw_model_t *__restrict__ md, uint32_t *__restrict__ out, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float wallbol = (walls[t] > 0.f) ? 1.f : 0.f;
+    const float asp = aspect[t];
+    uint32_t bits = 0;
+    for (int j = 0; j < SW_WALK_AZ; ++j) {
+        const float lo = md->walk_lo[j], hi = md->walk_hi[j];
+        float facesh;
+        if (md->walk_branch[j] == 0) facesh = (float)((asp < lo) || (asp >= hi)) - wallbol + 1.f;
+        else facesh = (float)((asp > lo) || (asp <= hi)) * -1.f + 1.f;
+        if (facesh == 0.f) bits |= 1u << j;
+    }
+    out[t] = bits;
+}
+
+// single sun march with float outputs (reference native ABI form)
+__global__ void k_march_wh23_single(const float *__restrict__ a, const float *__restrict__ v1,
+                                    const float *__restrict__ v2, const int32_t *__restrict__ dx,
+                                    const int32_t *__restrict__ dy, const float *__restrict__ dz,
+                                    const float *__restrict__ dzp, float *__restrict__ sh,
+                                    float *__restrict__ vegsh, float *__restrict__ vbsh, int rows, int cols,
+                                    int count) {
+    const long n = (long)rows * cols;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int r = (int)(t / cols), c = (int)(t - (long)r * cols);
+    const float at = a[t];
+    float f = at;
+    int s_sh = 0, vs = 0, vb = 0;
+    for (int s = 0; s < count; ++s) {
+        const int sr = r + dx[s], sc = c + dy[s];
+        const float dzs = dz[s], dzq = dzp[s];
+        float tmp = 0.f, tv = 0.f, tv2 = 0.f, lf = 0.f, lg = 0.f;
+        if (((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols)) {
+            const long q = (long)sr * cols + sc;
+            tv = v1[q] - dzs;
+            tv2 = v2[q] - dzs;
+            tmp = a[q] - dzs;
+            lf = v1[q] - dzq;
+            lg = v2[q] - dzq;
+        }
+        f = fmaxf(f, tmp);
+        s_sh = (f > at) ? 1 : 0;
+        int g2 = (tv > at ? 1 : 0) + (tv2 > at ? 1 : 0) + (lf > at ? 1 : 0) + (lg > at ? 1 : 0);
+        if (g2 == 4) g2 = 0;
+        if (g2 > 0) g2 = 1;
+        vs = vs > g2 ? vs : g2;
+        if (vs * s_sh > 0) vs = 0;
+        vb = vb + vs;
+    }
+    int b = vb > 0 ? 1 : vb;
+    b = b - vs;
+    const int vsf = vs > 0 ? 1 : vs;
+    sh[t] = (float)(1 - s_sh);
+    vegsh[t] = (float)(1 - vsf);
+    vbsh[t] = (float)(1 - b);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
+static inline int grid_xcd(long n) { return ((grid_for(n, 256) + 7) / 8) * 8; }
+
+extern "C" {
+
+int sw_abi_version(void) { return SW_ABI_VERSION; }
+const char *sw_last_error(void) { return g_err; }
+const char *sw_build_flags(void) { return SW_BUILD_FLAGS; }
+
+int sw_device_count(int *count) {
+    if (!count) return set_err(SW_E_ARG, "sw_device_count: count is NULL");
+    return hip_check(hipGetDeviceCount(count), "hipGetDeviceCount");
+}
+
+int sw_sync(void *stream) {
+    if (stream) return hip_check(hipStreamSynchronize(S(stream)), "hipStreamSynchronize");
+    return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+static int check_shape(int rows, int cols, const char *who) {
+    if (rows <= 0 || cols <= 0 || (long)rows * cols > (1L << 31) - 1) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s: invalid grid %d x %d", who, rows, cols);
+        return set_err(SW_E_ARG, buf);
+    }
+    return SW_OK;
+}
+
+int sw_pack_heights(const float *a, const float *v1, const float *v2, void *hpack, int rows, int cols,
+                    void *stream) {
+    int rc = check_shape(rows, cols, "sw_pack_heights");
+    if (rc) return rc;
+    if (!a || !v1 || !v2 || !hpack) return set_err(SW_E_ARG, "sw_pack_heights: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_pack_heights, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), a, v1, v2,
+                       (float4 *)hpack, n);
+    return hip_check(hipGetLastError(), "k_pack_heights");
+}
+
+int sw_compose(const float *dsm, const float *dem, const float *cdsm, const float *aspect_deg,
+               const float *landcover, void *hpack, float *vegdsm2, float *buildings, uint8_t *cls,
+               float *aspect_rad, uint32_t *stats, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_compose");
+    if (rc) return rc;
+    if (!dsm || !dem || !cdsm || !aspect_deg || !hpack || !vegdsm2 || !buildings || !cls || !aspect_rad || !stats)
+        return set_err(SW_E_ARG, "sw_compose: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, S(stream), stats);
+    rc = hip_check(hipGetLastError(), "k_stats_init");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_compose, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), dsm, dem, cdsm, aspect_deg,
+                       landcover, (float4 *)hpack, vegdsm2, buildings, cls, aspect_rad, stats, n);
+    return hip_check(hipGetLastError(), "k_compose");
+}
+
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *dx, const int32_t *dy,
+                 const float *dz, uint32_t *masks, const uint32_t *stats, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_svf_march");
+    if (rc) return rc;
+    if (!hpack || !offs || !dx || !dy || !dz || !masks || !stats)
+        return set_err(SW_E_ARG, "sw_svf_march: NULL pointer");
+    const long n = (long)rows * cols;
+    dim3 grid(grid_xcd(n), SW_MASK_WORDS);
+    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float4 *)hpack, offs, dx, dy, dz,
+                       masks, stats, rows, cols);
+    return hip_check(hipGetLastError(), "k_svf_march");
+}
+
+int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso, const float *w_aniso,
+                const int32_t *ring, const int32_t *nann, const int32_t *dirm, float *fields, int rows,
+                int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_svf_fold");
+    if (rc) return rc;
+    if (!masks || !vegdsm2 || !w_iso || !w_aniso || !ring || !nann || !dirm || !fields)
+        return set_err(SW_E_ARG, "sw_svf_fold: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_svf_fold, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), masks, vegdsm2, w_iso,
+                       w_aniso, ring, nann, dirm, fields, n);
+    return hip_check(hipGetLastError(), "k_svf_fold");
+}
+
+int sw_unpack_masks(const uint32_t *masks, float *shmat, float *vegshmat, float *vbshmat, int rows, int cols,
+                    void *stream) {
+    int rc = check_shape(rows, cols, "sw_unpack_masks");
+    if (rc) return rc;
+    if (!masks) return set_err(SW_E_ARG, "sw_unpack_masks: NULL masks");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(n * SW_NPATCH, 256)), dim3(256), 0, S(stream), masks, shmat,
+                       vegshmat, vbshmat, n);
+    return hip_check(hipGetLastError(), "k_unpack");
+}
+
+int sw_run_march_svf_shadow(const float *a, const float *vegdem, const float *vegdem2, const int32_t *dxs,
+                            const int32_t *dys, const float *dzs, float *sh, float *vegsh, float *vbsh,
+                            int rows, int cols, int count, void *stream) {
+    int rc = check_shape(rows, cols, "sw_run_march_svf_shadow");
+    if (rc) return rc;
+    if (count < 0) return set_err(SW_E_ARG, "sw_run_march_svf_shadow: negative count");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_march_svf_single, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), a, vegdem,
+                       vegdem2, dxs, dys, dzs, sh, vegsh, vbsh, rows, cols, count);
+    return hip_check(hipGetLastError(), "k_march_svf_single");
+}
+
+int sw_run_march_wallheight23(const float *a, const float *vegdem, const float *vegdem2, const int32_t *dxs,
+                              const int32_t *dys, const float *dzs, const float *dzprevs, float *sh,
+                              float *vegsh, float *vbsh, int rows, int cols, int count, void *stream) {
+    int rc = check_shape(rows, cols, "sw_run_march_wallheight23");
+    if (rc) return rc;
+    if (count < 0) return set_err(SW_E_ARG, "sw_run_march_wallheight23: negative count");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_march_wh23_single, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), a, vegdem,
+                       vegdem2, dxs, dys, dzs, dzprevs, sh, vegsh, vbsh, rows, cols, count);
+    return hip_check(hipGetLastError(), "k_march_wh23_single");
+}
+
+int sw_tile_static(const float *fields, const float *walls, const float *aspect_rad, const sw_model_t *model,
+                   float *svfbuveg, float *tan_asvf, float *tan_svfalfa, uint32_t *walkface, int rows,
+                   int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_tile_static");
+    if (rc) return rc;
+    if (!fields || !walls || !aspect_rad || !model || !svfbuveg || !tan_asvf || !tan_svfalfa || !walkface)
+        return set_err(SW_E_ARG, "sw_tile_static: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_tile_static, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), fields, svfbuveg,
+                       tan_asvf, tan_svfalfa, n);
+    rc = hip_check(hipGetLastError(), "k_tile_static");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_walk_face, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), walls, aspect_rad, model,
+                       walkface, n);
+    return hip_check(hipGetLastError(), "k_walk_face");
+}
+
+int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad, const uint8_t *cls,
+                const float *buildings, const sw_step_t *steps, const sw_model_t *model, const int32_t *sun_tab,
+                const uint32_t *stats, uint32_t *src, int step, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_sun_step");
+    if (rc) return rc;
+    if (!hpack || !walls || !aspect_rad || !cls || !buildings || !steps || !model || !sun_tab || !stats || !src)
+        return set_err(SW_E_ARG, "sw_sun_step: NULL pointer");
+    if (step < 0) return set_err(SW_E_ARG, "sw_sun_step: negative step");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_sun_step, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack, walls,
+                       aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src, step,
+                       rows, cols);
+    return hip_check(hipGetLastError(), "k_sun_step");
+}
+
+int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg, const float *tan_asvf,
+                 const float *tan_svfalfa, const uint32_t *walkface, const float *buildings, const uint8_t *cls,
+                 const sw_step_t *steps, const sw_model_t *model, float *state, float *out, int out_mask,
+                 int step, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_flux_step");
+    if (rc) return rc;
+    if (!masks || !src || !svfbuveg || !tan_asvf || !tan_svfalfa || !walkface || !buildings || !cls || !steps ||
+        !model || !state || !out)
+        return set_err(SW_E_ARG, "sw_flux_step: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_flux_step, dim3(grid_xcd(n)), dim3(256), 0, S(stream), masks, (const uint2 *)src,
+                       svfbuveg, tan_asvf, tan_svfalfa, walkface, buildings, cls, steps, model, state, out,
+                       out_mask, step, rows, cols);
+    return hip_check(hipGetLastError(), "k_flux_step");
+}
+
+int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out, int64_t n,
+            void *stream) {
+    if (n < 0) return set_err(SW_E_ARG, "sw_utci: negative n");
+    if (n == 0) return SW_OK;
+    hipLaunchKernelGGL(k_utci, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), ta, rh, tmrt, va, out, (long)n);
+    return hip_check(hipGetLastError(), "k_utci");
+}
+
+}  // extern "C"

@@ -419,7 +419,18 @@ class PatchGeometry:
     dir_n: np.ndarray
 
 
+_GEOM_CACHE: dict = {}
+
+
 def patch_geometry(round_trip: bool = False) -> PatchGeometry:
+    if round_trip in _GEOM_CACHE:
+        return _GEOM_CACHE[round_trip]
+    g = _patch_geometry(round_trip)
+    _GEOM_CACHE[round_trip] = g
+    return g
+
+
+def _patch_geometry(round_trip: bool) -> PatchGeometry:
     alt, azi = _patches(round_trip)
     deg2rad = torch.tensor(np.pi / 180, device=_CPU)
     ster = patch_steradians(round_trip)
@@ -458,7 +469,7 @@ class DayLoop:
         self.lat = float(latitude)
         self.lc = landcover
         self.scale = float(scale)
-        self.amax = float(np.float32(amax))
+        self.amax = None if amax is None else float(np.float32(amax))
         self.rows = rows
         self.cols = cols
         self.state = state or DayState()
@@ -613,20 +624,19 @@ class DayLoop:
             sp.shaded_k = _f((albedo * radD * 0.5) / torch.pi)
             ster_t = torch.tensor(geom.ster)
             palt = torch.tensor(geom.alt)
-            radTot = torch.zeros(1)
-            for p in range(153):
-                radTot += (lv[p] * ster_t[p] * torch.sin(palt[p] * deg2rad_k))
-            lumChi = (lv * radD) / radTot
+            # radTot accumulates in float32, patch by patch (solweig.py:696-704)
+            terms = (lv * ster_t * torch.sin(palt * deg2rad_k)).numpy()
+            radTot = np.float32(0.0)
+            for v in terms:
+                radTot = np.float32(radTot + v)
+            lumChi = (lv * radD) / torch.tensor([radTot])
             sp.lumchi = lumChi.numpy().astype(F32)
             az_py = azimuth.item()
             pazi = torch.tensor(geom.azi)
-            yk = np.zeros(153, F32)
-            for p in range(153):
-                p2s = torch.abs(az_py - pazi[p])
-                xi = torch.cos(p2s * (torch.pi / 180.0))
-                yi = 2 * xi * torch.tan(alt_k * (torch.pi / 180.0))
-                yk[p] = float(torch.where(yi > 0, 0.0, yi))
-            sp.yi_k = yk
+            p2s = torch.abs(az_py - pazi)
+            xi = torch.cos(p2s * (torch.pi / 180.0))
+            yi = 2 * xi * torch.tan(alt_k * (torch.pi / 180.0))
+            sp.yi_k = torch.where(yi > 0, 0.0, yi).numpy().astype(F32)
             d.update(I0=_f(I0), CI=float(CI), radI=sp.radI, radD=sp.radD,
                      CI_Tg=float(CI_Tg), CI_TgG=float(CI_TgG))
         else:
@@ -664,18 +674,15 @@ class DayLoop:
         if sp.is_day:
             tgw = torch.tensor([[sp.tgwall]], dtype=torch.float32)
             sp.sunlit_l = float(((ewall_t * SBC * ((Ta + tgw + 273.15) ** 4)) / pi_t).reshape(-1)[0])
-            pazi = torch.tensor(geom.azi)
-            yl = np.zeros(153, F32)
-            lsun = np.zeros(153, np.int32)
-            for p in range(153):
-                azd = torch.abs(azimuth - pazi[p])
-                if (azd > 90) and (azd < 270) and (altitude > 0):
-                    lsun[p] = 1
-                    xi = torch.cos(azd * (torch.pi / 180.0))
-                    yi = 2 * xi * torch.tan(altitude * (torch.pi / 180.0))
-                    yl[p] = float(torch.where(yi > 0, 0.0, yi))
-            sp.yi_l = yl
-            sp.l_sun_side = lsun
+            # float64 like the reference: 0-dim float64 azimuth minus a float32 patch azimuth
+            pazi = torch.tensor(geom.azi).double()
+            azd = torch.abs(azimuth - pazi)
+            lsun = ((azd > 90) & (azd < 270) & (altitude > 0)).numpy()
+            xi = torch.cos(azd * (torch.pi / 180.0))
+            yi = 2 * xi * torch.tan(altitude * (torch.pi / 180.0))
+            yl = torch.where(yi > 0, 0.0, yi).float().numpy()
+            sp.yi_l = np.where(lsun, yl, F32(0)).astype(F32)
+            sp.l_sun_side = lsun.astype(np.int32)
         else:
             sp.sunlit_l = 0.0
             sp.l_sun_side = np.zeros(153, np.int32)

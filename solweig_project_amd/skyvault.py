@@ -67,8 +67,18 @@ def _patch_tensors():
     return annulino, alt_int, azistart, count
 
 
+_PATCHES = None
+
+
 def create_patches():
-    """Float32 patch altitudes/azimuths exactly as ``create_patches(2)``."""
+    """Float32 patch altitudes/azimuths exactly as ``create_patches(2)`` (cached copies)."""
+    global _PATCHES
+    if _PATCHES is None:
+        _PATCHES = _create_patches()
+    return _PATCHES[0].clone(), _PATCHES[1].clone()
+
+
+def _create_patches():
     annulino, alt_int, azistart, count = _patch_tensors()
     aziint = 360 / count
     alts = torch.tensor([], device=_CPU)
@@ -162,55 +172,62 @@ def _as_f32_tensor(x):
     return torch.tensor(x, device=_CPU)
 
 
+def _steps(ssin, scos, tanaz, ds_sin, ds_cos, tanalt, branch: bool, amax, rows: int, cols: int,
+           start: int, neg_one):
+    """Vectorised step loop shared by both marches.
+
+    Evaluates every candidate index at once with the same float32 element
+    operations as the reference's scalar loop, then keeps the executed
+    prefix: a step runs while the previous step still satisfied
+    ``amax >= dz and |dx| < rows and |dy| < cols`` (so the first failing
+    step, the overshoot, is kept). With ``amax=None`` only the geometric
+    part is applied; the device kernels then apply the ``amax`` test with
+    the tile maximum they computed themselves.
+    """
+    kmax = max(rows, cols) + 2
+    index = torch.arange(start, start + kmax, dtype=torch.float32)
+    if branch:
+        dy = ssin * index
+        dx = (neg_one * scos) * torch.abs(torch.round(index / tanaz))
+        ds = ds_sin
+    else:
+        dy = ssin * torch.abs(torch.round(index * tanaz))
+        dx = (neg_one * scos) * index
+        ds = ds_cos
+    dz = (ds * index) * tanalt
+    ok = (torch.abs(dx) < rows) & (torch.abs(dy) < cols)
+    if amax is not None:
+        ok = ok & (amax >= dz)
+    bad = torch.nonzero(~ok)
+    count = int(bad[0, 0]) + 1 if bad.numel() else kmax
+    return (dx[:count].numpy().astype(np.int32), dy[:count].numpy().astype(np.int32),
+            dz[:count].numpy().astype(np.float32))
+
+
 def svf_patch_table(azimuth, altitude, scale: float, amaxvalue, rows: int, cols: int) -> StepTable:
     """Step table of one SVF patch march (``shadow()``, ``shadow.py:194-301``).
 
     ``azimuth`` is the float32 patch azimuth (tensor or float, degrees),
     ``altitude`` the integer ring altitude, ``amaxvalue`` the float32
-    maximum height used by the stop test.
+    maximum height used by the stop test. Index starts at 1.
     """
     degrees = torch.pi / 180.0
     if azimuth == 0.0:
         azimuth = 1e-12
     azimuth = _as_f32_tensor(azimuth)
     altitude = _as_f32_tensor(altitude)
-    amax = _as_f32_tensor(amaxvalue)
+    amax = None if amaxvalue is None else _as_f32_tensor(amaxvalue)
     azimuth = azimuth * degrees
     altitude = altitude * degrees
-    dx = torch.tensor(0.0, device=_CPU)
-    dy = torch.tensor(0.0, device=_CPU)
-    dz = torch.tensor(0.0, device=_CPU)
     pibyfour = torch.pi / 4.0
-    three = 3.0 * pibyfour
-    five = 5.0 * pibyfour
-    seven = 7.0 * pibyfour
+    branch = bool(pibyfour <= azimuth < 3.0 * pibyfour or 5.0 * pibyfour <= azimuth < 7.0 * pibyfour)
     sinaz = torch.sin(azimuth)
     cosaz = torch.cos(azimuth)
     tanaz = torch.tan(azimuth)
-    ssin = torch.sign(sinaz)
-    scos = torch.sign(cosaz)
-    dssin = torch.abs(1.0 / sinaz)
-    dscos = torch.abs(1.0 / cosaz)
-    tanalt = torch.tan(altitude) / scale
-    index = 1
-    dxs, dys, dzs = [], [], []
-    while amax >= dz and torch.abs(dx) < rows and torch.abs(dy) < cols:
-        if pibyfour <= azimuth < three or five <= azimuth < seven:
-            dy = ssin * index
-            dx = -1.0 * scos * torch.abs(torch.round(index / tanaz))
-            ds = dssin
-        else:
-            dy = ssin * torch.abs(torch.round(index * tanaz))
-            dx = -1.0 * scos * index
-            ds = dscos
-        dz = ds * index * tanalt
-        dxs.append(int(dx))
-        dys.append(int(dy))
-        dzs.append(float(dz))
-        index += 1.0
-    dz_arr = np.asarray(dzs, dtype=np.float32)
-    return StepTable(np.asarray(dxs, np.int32), np.asarray(dys, np.int32), dz_arr,
-                     np.zeros_like(dz_arr))
+    dx, dy, dz = _steps(torch.sign(sinaz), torch.sign(cosaz), tanaz, torch.abs(1.0 / sinaz),
+                        torch.abs(1.0 / cosaz), torch.tan(altitude) / scale, branch, amax, rows, cols,
+                        1, -1.0)
+    return StepTable(dx, dy, dz, np.zeros_like(dz))
 
 
 def sun_table(azimuth_deg: float, altitude_deg: float, scale: float, amaxvalue: float,
@@ -219,48 +236,23 @@ def sun_table(azimuth_deg: float, altitude_deg: float, scale: float, amaxvalue: 
 
     ``azimuth_deg``/``altitude_deg`` are the Python floats the reference
     passes (``azimuth.item()``); they become float32 tensors before the
-    degree conversion (``solweig.py:1090-1092``).
+    degree conversion (``solweig.py:1090-1092``). Index starts at 0 and each
+    step also carries the previous step's dz.
     """
     degrees = torch.tensor(np.pi / 180.0, device=_CPU)
     azimuth = torch.tensor(azimuth_deg, device=_CPU) * degrees
     altitude = torch.tensor(altitude_deg, device=_CPU) * degrees
-    amax = amaxvalue
-    dx = torch.tensor(0.0, device=_CPU)
-    dy = torch.tensor(0.0, device=_CPU)
-    dz = torch.tensor(0.0, device=_CPU)
     pibyfour = torch.tensor(np.pi / 4.0, device=_CPU)
-    three = 3 * pibyfour
-    five = 5 * pibyfour
-    seven = 7 * pibyfour
+    branch = bool(((pibyfour <= azimuth) and (azimuth < 3 * pibyfour))
+                  or ((5 * pibyfour <= azimuth) and (azimuth < 7 * pibyfour)))
     sinaz = torch.sin(azimuth)
     cosaz = torch.cos(azimuth)
     tanaz = torch.tan(azimuth)
-    ssin = torch.sign(sinaz)
-    scos = torch.sign(cosaz)
-    dssin = torch.abs(1 / sinaz)
-    dscos = torch.abs(1 / cosaz)
-    tanalt = torch.tan(altitude) / scale
-    index = 0
-    dzprev = torch.tensor(0.0, device=_CPU)
-    dxs, dys, dzs, dzp = [], [], [], []
-    while (amax >= dz) and (torch.abs(dx) < rows) and (torch.abs(dy) < cols):
-        if ((pibyfour <= azimuth) and (azimuth < three)) or ((five <= azimuth) and (azimuth < seven)):
-            dy = ssin * index
-            dx = -1 * scos * torch.abs(torch.round(index / tanaz))
-            ds = dssin
-        else:
-            dy = ssin * torch.abs(torch.round(index * tanaz))
-            dx = -1 * scos * index
-            ds = dscos
-        dz = (ds * index) * tanalt
-        dxs.append(int(dx))
-        dys.append(int(dy))
-        dzs.append(float(dz))
-        dzp.append(float(dzprev))
-        dzprev = dz
-        index += 1
-    return StepTable(np.asarray(dxs, np.int32), np.asarray(dys, np.int32),
-                     np.asarray(dzs, np.float32), np.asarray(dzp, np.float32))
+    dx, dy, dz = _steps(torch.sign(sinaz), torch.sign(cosaz), tanaz, torch.abs(1 / sinaz),
+                        torch.abs(1 / cosaz), torch.tan(altitude) / scale, branch, amaxvalue, rows, cols,
+                        0, -1)
+    dzprev = np.concatenate([np.zeros(1, np.float32), dz[:-1]]).astype(np.float32)
+    return StepTable(dx, dy, dz, dzprev)
 
 
 def sun_facesh_bounds(azimuth_deg: float):
@@ -300,7 +292,17 @@ class WalkTables:
     second: float           # rounded second-height steps
 
 
+_WALK_CACHE: dict = {}
+
+
 def walk_tables(scale: float, height: float = 1.1) -> WalkTables:
+    key = (float(scale), float(height))
+    if key not in _WALK_CACHE:
+        _WALK_CACHE[key] = _walk_tables(scale, height)
+    return _WALK_CACHE[key]
+
+
+def _walk_tables(scale: float, height: float) -> WalkTables:
     """Offsets of ``sunonsurface_2018a`` for the 18 GVF azimuths.
 
     ``first = round(height)``, ``second = round(height*20)`` as in the
@@ -374,7 +376,9 @@ def walk_tables(scale: float, height: float = 1.1) -> WalkTables:
 
 
 def svf_tables(vault: SkyVault, scale: float, amaxvalue, rows: int, cols: int):
-    """All 153 SVF patch tables concatenated: (offsets, dx, dy, dz)."""
+    """All 153 SVF patch tables concatenated: (offsets, dx, dy, dz).
+
+    ``amaxvalue=None`` keeps every geometrically valid step (device form)."""
     iaz = sweep_azimuths()
     annulino, alt_int, azistart, count = _patch_tensors()
     tabs = []

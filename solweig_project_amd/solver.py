@@ -1,0 +1,479 @@
+"""Full SOLWEIG solve on one MI355X through libsolweig_hip.
+
+Host work per tile: scene compose and validation, the 153 sky-patch march
+tables, the per-step scalar chain (``met.DayLoop``) for the whole day, and
+the walk tables. All of it is O(patches x steps) and is uploaded once; the
+day is then a sequence of two kernels per step (sun march + walk sources,
+flux -> Tmrt -> UTCI) with no host synchronisation between steps.
+
+Device memory per tile (n pixels): heights 16 B/px, masks 80 B/px, SVF
+fields 64 B/px, static planes ~30 B/px, walk sources 8 B/px, thermal state
+24 B/px, outputs 4 B/px per requested plane per step.
+
+The reference entry points this replaces: ``svf_calculator``
+(``solweig_gpu/shadow.py:409-625``) and ``run_utci_window``
+(``solweig_gpu/utci_process.py:477-1090``).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import met as M
+from . import skyvault as SV
+
+F32 = np.float32
+
+#: output plane bits of sw_flux_step, in packing order
+OUT_BITS = {"tmrt": 1, "utci": 2, "shadow": 4, "kdown": 8, "kup": 16, "ldown": 32, "lup": 64}
+OUT_ORDER = ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup")
+
+
+class UnsupportedInput(ValueError):
+    """The input lies outside the domain the HIP backend reproduces exactly."""
+
+
+@dataclass
+class TileInputs:
+    """Host-resident inputs of one tile (the rasters the reference loads).
+
+    dsm: building DSM incl. ground; dem: ground; cdsm: tree canopy height
+    above ground ("temp1"); walls: wall heights; aspect_deg: wall aspect in
+    degrees; landcover: SOLWEIG land-cover raster or None. scale is pixels
+    per metre (1 / pixel size).
+    """
+
+    dsm: np.ndarray
+    dem: np.ndarray
+    cdsm: np.ndarray
+    walls: np.ndarray
+    aspect_deg: np.ndarray
+    landcover: np.ndarray | None
+    scale: float
+
+    @property
+    def shape(self):
+        return self.dsm.shape
+
+    @classmethod
+    def from_site(cls, site):
+        return cls(dsm=site.dsm, dem=site.dem, cdsm=site.cdsm, walls=site.walls, aspect_deg=site.aspect,
+                   landcover=site.landcover, scale=site.scale)
+
+
+def compose_scene(dsm, dem, cdsm, walls, aspect_deg, landcover=None, *, scale: float) -> TileInputs:
+    """Validate shapes and wrap the host rasters (compose itself runs on the device)."""
+    arrs = [np.ascontiguousarray(x, F32) for x in (dsm, dem, cdsm, walls, aspect_deg)]
+    if any(x.shape != arrs[0].shape or x.ndim != 2 for x in arrs):
+        raise ValueError("input rasters must be 2-D and of one shape")
+    lc = None if landcover is None else np.ascontiguousarray(landcover, F32)
+    if lc is not None and lc.shape != arrs[0].shape:
+        raise ValueError("land cover shape differs from the DSM")
+    return TileInputs(*arrs, landcover=lc, scale=float(scale))
+
+
+# ---------------------------------------------------------------------------
+# sunlit / shaded patch thresholds
+# ---------------------------------------------------------------------------
+_RAD2DEG = np.float32(180.0 / math.pi)
+
+
+def _g(x: float) -> np.float32:
+    return np.float32(np.float32(math.atan(x)) * _RAD2DEG)
+
+
+def _key_to_f32(k: int) -> float:
+    u = (k - 0x80000000) if k >= 0x80000000 else (0xFFFFFFFF - k)
+    return float(np.array([u], dtype=np.uint32).view(np.float32)[0])
+
+
+def atan_thresholds(alt: float):
+    """(T_lt, T_gt) with  atan(x)*180/pi < alt <=> x < T_lt  and  > alt <=> x > T_gt.
+
+    ``shaded_or_sunlit`` (``solweig.py:541-561``) compares
+    ``float32(atan(x)) * float32(180/pi)`` with the patch altitude; the map
+    x -> that product is monotone in x, so each comparison is a threshold on
+    x. Found by bisection over the ordered float32 values.
+    """
+    alt = np.float32(alt)
+    neg_inf = 0xFFFFFFFF - 0xFF800000      # key of -inf
+    pos_inf = 0x7F800000 + 0x80000000      # key of +inf
+
+    def first_key(pred):
+        lo, hi = neg_inf, pos_inf + 1
+        while lo < hi:
+            mid = (lo + hi) // 2
+            if pred(_g(_key_to_f32(mid))):
+                hi = mid
+            else:
+                lo = mid + 1
+        return lo
+
+    k_ge = first_key(lambda v: v >= alt)    # smallest x with g(x) >= alt
+    t_lt = _key_to_f32(k_ge) if k_ge <= pos_inf else float("inf")
+    k_gt = first_key(lambda v: v > alt)     # smallest x with g(x) > alt
+    t_gt = _key_to_f32(k_gt - 1) if k_gt - 1 >= neg_inf else float("-inf")
+    return np.float32(t_lt), np.float32(t_gt)
+
+
+_THR_CACHE: dict = {}
+
+
+def patch_thresholds(alt: np.ndarray):
+    key = alt.tobytes()
+    if key not in _THR_CACHE:
+        lt = np.empty(alt.shape, F32)
+        gt = np.empty(alt.shape, F32)
+        memo = {}
+        for i, a in enumerate(alt):
+            if float(a) not in memo:
+                memo[float(a)] = atan_thresholds(float(a))
+            lt[i], gt[i] = memo[float(a)]
+        _THR_CACHE[key] = (lt, gt)
+    return _THR_CACHE[key]
+
+
+# ---------------------------------------------------------------------------
+# per-step / model structs
+# ---------------------------------------------------------------------------
+def _fill(arr, vals):
+    arr[:] = [float(v) for v in vals]
+
+
+def step_struct(sp: M.StepParams, sun_off: int) -> L.StepStruct:
+    s = L.StepStruct()
+    s.is_day = int(sp.is_day)
+    s.sun_off = sun_off
+    s.sun_count = sp.table.count if (sp.is_day and sp.table is not None) else 0
+    s.facesh_branch = int(sp.facesh_branch)
+    s.azilow, s.azihigh = sp.azilow, sp.azihigh
+    for k in ("ta", "ta_mat", "rh", "va", "tak4", "radI", "radD", "radG", "sinalt", "cosalt_k",
+              "tan_zen", "one_m_psi", "tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
+              "shaded_l", "veg_l", "tgwall", "lwall"):
+        setattr(s, k, float(getattr(sp, k)))
+    s.tw_first = int(sp.tw_first)
+    s.tw_update = int(sp.tw_update)
+    _fill(s.tg_class, sp.tg_class)
+    for k in ("lv", "lumchi", "yi_k", "yi_l", "lsky_down", "lsky_side"):
+        _fill(getattr(s, k), getattr(sp, k))
+    g = sp.geom
+    _fill(s.alt, g.alt)
+    _fill(s.ster, g.ster)
+    _fill(s.cos_alt, g.cos_alt)
+    _fill(s.sin_alt, g.sin_alt)
+    _fill(s.angl, g.angl_inc)
+    lt, gt = patch_thresholds(np.asarray(g.alt, F32))
+    _fill(s.sun_lt, lt)
+    _fill(s.sun_gt, gt)
+    s.l_sun[:] = [int(v) for v in sp.l_sun_side]
+    return s
+
+
+def model_struct(scale: float, lcc: M.LandCover) -> L.ModelStruct:
+    wt = SV.walk_tables(scale)
+    nsec = wt.dx.shape[1]
+    if nsec > L.WALK_MAX:
+        raise UnsupportedInput(f"ground-view walk of {nsec} steps exceeds {L.WALK_MAX} (pixel too small)")
+    if (wt.branch < 0).any():
+        raise UnsupportedInput("walk azimuth without a face-shading branch")
+    m = L.ModelStruct()
+    m.absK, m.absL, m.Fside, m.Fup, m.Fcyl = M.ABSK, M.ABSL, M.FSIDE, M.FUP, M.FCYL
+    m.albedo_b, m.ewall, m.sbc = M.ALBEDO_B, M.EWALL, M.SBC_PY
+    m.first, m.second, m.nsec = wt.first, wt.second, nsec
+    _fill(m.alb_class, lcc.alb)
+    _fill(m.emis_class, lcc.emis)
+    dx = np.zeros((L.WALK_AZ, L.WALK_MAX), np.int32)
+    dy = np.zeros((L.WALK_AZ, L.WALK_MAX), np.int32)
+    dx[:, :nsec] = wt.dx
+    dy[:, :nsec] = wt.dy
+    m.walk_dx[:] = dx.ravel().tolist()
+    m.walk_dy[:] = dy.ravel().tolist()
+    m.walk_branch[:] = wt.branch.tolist()
+    m.walk_dir[:] = wt.dir_mask.tolist()
+    _fill(m.walk_lo, wt.azilow)
+    _fill(m.walk_hi, wt.azihigh)
+    return m
+
+
+@dataclass
+class DayPlan:
+    """All host-side inputs of one day, ready to upload."""
+
+    steps: np.ndarray          # uint8 bytes of sw_step_t[T]
+    sun_tab: np.ndarray        # int32 (S, 4)
+    model: np.ndarray          # uint8 bytes of sw_model_t
+    n_steps: int
+    is_day: list
+    params: list = field(default_factory=list)
+
+
+def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float,
+             state: M.DayState | None = None, time_start: int = 0, time_stop: int | None = None) -> DayPlan:
+    """Host scalar chain for the whole day; the sun tables stop only at the grid
+    edge (the kernels apply the amax stop test on the device)."""
+    lcc = M.landcover_constants(tile.landcover is not None)
+    rows, cols = tile.shape
+    loop = M.DayLoop(met, sun, latitude=latitude, landcover=lcc, scale=tile.scale, amax=None,
+                     rows=rows, cols=cols, state=state)
+    stop = met.shape[0] if time_stop is None else time_stop
+    structs, tabs, params = [], [], []
+    off = 0
+    for i in range(time_start, stop):
+        sp = loop.step(i)
+        params.append(sp)
+        if sp.is_day:
+            if sp.facesh_branch < 0:
+                raise UnsupportedInput(f"step {i}: sun azimuth without a face-shading branch")
+            t = sp.table
+            tab = np.stack([t.dx, t.dy, t.dz.view(np.int32), t.dzprev.view(np.int32)], axis=1).astype(np.int32)
+            tabs.append(tab)
+            structs.append(step_struct(sp, off))
+            off += tab.shape[0]
+        else:
+            structs.append(step_struct(sp, 0))
+    steps = np.concatenate([L.struct_bytes(s) for s in structs])
+    sun_tab = np.concatenate(tabs) if tabs else np.zeros((1, 4), np.int32)
+    return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=L.struct_bytes(model_struct(tile.scale, lcc)),
+                   n_steps=len(structs), is_day=[p.is_day for p in params], params=params)
+
+
+# ---------------------------------------------------------------------------
+# device solver
+# ---------------------------------------------------------------------------
+def _stream_ptr(stream) -> int:
+    return stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+
+
+FLAG_TEXT = {1: "non-finite surface heights", 4: "bush layer present (DEM = -0.25 * trees somewhere)",
+             8: "non-integer land-cover codes"}
+
+
+class DeviceScene:
+    """Device-resident tile: inputs, composed planes, SVF masks and fields."""
+
+    def __init__(self, tile: TileInputs, device, inputs=None):
+        self.tile = tile
+        self.device = device
+        rows, cols = tile.shape
+        self.rows, self.cols, self.n = rows, cols, rows * cols
+        dev = device
+        if inputs is None:
+            inputs = upload_inputs(tile, device)
+        self.inputs = inputs
+        n = self.n
+        self.hpack = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        self.vegdsm2 = torch.empty(n, dtype=torch.float32, device=dev)
+        self.buildings = torch.empty(n, dtype=torch.float32, device=dev)
+        self.cls = torch.empty(n, dtype=torch.uint8, device=dev)
+        self.aspect = torch.empty(n, dtype=torch.float32, device=dev)
+        self.stats = torch.empty(4, dtype=torch.int32, device=dev)
+        self.masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
+        self.fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
+        self.svfbuveg = torch.empty(n, dtype=torch.float32, device=dev)
+        self.tan_asvf = torch.empty(n, dtype=torch.float32, device=dev)
+        self.tan_sa = torch.empty(n, dtype=torch.float32, device=dev)
+        self.walkface = torch.empty(n, dtype=torch.int32, device=dev)
+        self.src = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        self.state = torch.zeros((6, n), dtype=torch.float32, device=dev)
+
+    @property
+    def walls(self):
+        return self.inputs["walls"]
+
+    def check_flags(self):
+        """Refuse the solve if compose flagged inputs outside the supported domain (syncs)."""
+        flags = int(self.stats[2].item()) & 0xFFFFFFFF
+        if flags:
+            msgs = [t for b, t in FLAG_TEXT.items() if flags & b]
+            raise UnsupportedInput("; ".join(msgs) or f"input flags {flags:#x}")
+
+    def amax_hmax(self):
+        k = self.stats[:2].cpu().numpy().view(np.uint32)
+        vals = []
+        for key in k:
+            key = int(key)
+            u = (key & 0x7FFFFFFF) if key & 0x80000000 else (~key & 0xFFFFFFFF)
+            vals.append(float(np.array([u], np.uint32).view(np.float32)[0]))
+        return tuple(vals)
+
+
+def upload_inputs(tile: TileInputs, device):
+    """Host -> HBM copy of the input rasters (outside the device-resident timed region)."""
+    d = {k: torch.from_numpy(np.ascontiguousarray(getattr(tile, k), F32)).to(device)
+         for k in ("dsm", "dem", "cdsm", "walls", "aspect_deg")}
+    d["landcover"] = None if tile.landcover is None else torch.from_numpy(
+        np.ascontiguousarray(tile.landcover, F32)).to(device)
+    return d
+
+
+class KernelTimer:
+    """Records HIP events around each launch on the launch stream."""
+
+    def __init__(self):
+        self.records = []
+
+    def span(self, name, stream=None):
+        timer = self
+
+        class _Span:
+            def __enter__(self_):
+                self_.e0 = torch.cuda.Event(enable_timing=True)
+                self_.e1 = torch.cuda.Event(enable_timing=True)
+                self_.e0.record(stream)
+                return self_
+
+            def __exit__(self_, *exc):
+                self_.e1.record(stream)
+                timer.records.append((name, self_.e0, self_.e1))
+                return False
+        return _Span()
+
+    def summary(self):
+        """{name: (launches, total_ms)} (call after synchronising)."""
+        out = {}
+        for name, e0, e1 in self.records:
+            n, t = out.get(name, (0, 0.0))
+            out[name] = (n + 1, t + e0.elapsed_time(e1))
+        return out
+
+
+class _NoTimer:
+    def span(self, name, stream=None):
+        import contextlib
+        return contextlib.nullcontext()
+
+
+class HipSolver:
+    """Drives the HIP kernels for one tile at a time on one GPU."""
+
+    def __init__(self, device: str | torch.device = "cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HipSolver needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device)
+        self.lib = L.load()
+        self.vault = SV.build_skyvault()
+        v = self.vault
+        self._fold_consts = {k: torch.from_numpy(np.ascontiguousarray(x)).to(self.device) for k, x in (
+            ("w_iso", v.w_iso), ("w_aniso", v.w_aniso), ("ring", v.ring), ("nann", v.n_annuli),
+            ("dirm", v.dir_mask))}
+        self._tables = {}
+        self.timer = _NoTimer()
+
+    # -- SVF ---------------------------------------------------------------
+    def upload(self, tile: TileInputs) -> DeviceScene:
+        return DeviceScene(tile, self.device)
+
+    def svf_tables(self, rows: int, cols: int, scale: float):
+        """Geometric march tables of the 153 patches (host, torch float32 scalar chain)."""
+        key = (rows, cols, scale)
+        if key not in self._tables:
+            offs, dx, dy, dz = SV.svf_tables(self.vault, scale, None, rows, cols)
+            self._tables[key] = tuple(torch.from_numpy(np.ascontiguousarray(x)).to(self.device)
+                                      for x in (offs, dx, dy, dz))
+        return self._tables[key]
+
+    def compose(self, ds: DeviceScene, stream=None):
+        sp = _stream_ptr(stream)
+        P = L.ptr
+        i = ds.inputs
+        lc = 0 if i["landcover"] is None else P(i["landcover"])
+        with self.timer.span("compose", stream):
+            L.call("sw_compose", P(i["dsm"]), P(i["dem"]), P(i["cdsm"]), P(i["aspect_deg"]), lc, P(ds.hpack),
+                   P(ds.vegdsm2), P(ds.buildings), P(ds.cls), P(ds.aspect), P(ds.stats), ds.rows, ds.cols, sp)
+
+    def svf(self, ds: DeviceScene, stream=None, tables=None):
+        sp = _stream_ptr(stream)
+        P = L.ptr
+        self.compose(ds, stream)
+        offs, dx, dy, dz = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
+        with self.timer.span("svf_march", stream):
+            L.call("sw_svf_march", P(ds.hpack), P(offs), P(dx), P(dy), P(dz), P(ds.masks), P(ds.stats),
+                   ds.rows, ds.cols, sp)
+        c = self._fold_consts
+        with self.timer.span("svf_fold", stream):
+            L.call("sw_svf_fold", P(ds.masks), P(ds.vegdsm2), P(c["w_iso"]), P(c["w_aniso"]), P(c["ring"]),
+                   P(c["nann"]), P(c["dirm"]), P(ds.fields), ds.rows, ds.cols, sp)
+        return ds.fields
+
+    def unpack_masks(self, ds: DeviceScene, stream=None):
+        """(shmat, vegshmat, vbshvegshmat) float32 cubes (rows, cols, 153)."""
+        cubes = [torch.empty((ds.rows, ds.cols, L.NPATCH), dtype=torch.float32, device=self.device) for _ in range(3)]
+        L.call("sw_unpack_masks", L.ptr(ds.masks), *[L.ptr(c) for c in cubes], ds.rows, ds.cols, _stream_ptr(stream))
+        return cubes
+
+    # -- day ---------------------------------------------------------------
+    def upload_plan(self, plan: DayPlan):
+        dev = self.device
+        return dict(steps=torch.from_numpy(plan.steps).to(dev),
+                    sun_tab=torch.from_numpy(plan.sun_tab).to(dev),
+                    model=torch.from_numpy(plan.model.copy()).to(dev))
+
+    def tile_static(self, ds: DeviceScene, dplan, stream=None):
+        P = L.ptr
+        L.call("sw_tile_static", P(ds.fields), P(ds.walls), P(ds.aspect), P(dplan["model"]), P(ds.svfbuveg),
+               P(ds.tan_asvf), P(ds.tan_sa), P(ds.walkface), ds.rows, ds.cols, _stream_ptr(stream))
+
+    def run_day(self, ds: DeviceScene, plan: DayPlan, dplan, outputs=("tmrt", "utci", "shadow"),
+                out=None, stream=None, reset_state=True):
+        """Enqueue the time loop; returns the device output tensor (T, k, rows, cols)."""
+        mask = 0
+        for name in outputs:
+            mask |= OUT_BITS[name]
+        k = bin(mask).count("1")
+        T = plan.n_steps
+        if out is None:
+            out = torch.empty((T, k, ds.rows, ds.cols), dtype=torch.float32, device=self.device)
+        sp = _stream_ptr(stream)
+        P = L.ptr
+        if reset_state:
+            ds.state.zero_()
+        stride = k * ds.n * 4
+        base = P(out)
+        for i in range(T):
+            if plan.is_day[i]:
+                with self.timer.span("sun_step", stream):
+                    L.call("sw_sun_step", P(ds.hpack), P(ds.walls), P(ds.aspect), P(ds.cls), P(ds.buildings),
+                           P(dplan["steps"]), P(dplan["model"]), P(dplan["sun_tab"]), P(ds.stats), P(ds.src), i,
+                           ds.rows, ds.cols, sp)
+            with self.timer.span("flux_day" if plan.is_day[i] else "flux_night", stream):
+                L.call("sw_flux_step", P(ds.masks), P(ds.src), P(ds.svfbuveg), P(ds.tan_asvf), P(ds.tan_sa),
+                       P(ds.walkface), P(ds.buildings), P(ds.cls), P(dplan["steps"]), P(dplan["model"]),
+                       P(ds.state), base + i * stride, mask, i, ds.rows, ds.cols, sp)
+        return out
+
+    def output_names(self, outputs):
+        mask = 0
+        for name in outputs:
+            mask |= OUT_BITS[name]
+        return [n for n in OUT_ORDER if mask & OUT_BITS[n]]
+
+    def solve_device(self, ds: DeviceScene, met, sun: M.SunTable, latitude: float,
+                     outputs=("tmrt", "utci", "shadow"), out=None, stream=None):
+        """Enqueue a full solve of a device-resident tile; returns (fields, out) device tensors.
+
+        The SVF sweep is enqueued first; the day's host scalar chain runs
+        while the GPU marches.
+        """
+        self.svf(ds, stream)
+        plan = plan_day(ds.tile, met, sun, latitude)
+        dplan = self.upload_plan(plan)
+        self.tile_static(ds, dplan, stream)
+        out = self.run_day(ds, plan, dplan, outputs, out=out, stream=stream)
+        return ds.fields, out
+
+    def solve(self, tile: TileInputs, met, sun: M.SunTable, latitude: float,
+              outputs=("tmrt", "utci", "shadow")):
+        """Full solve of one tile; returns (fields (16, rows, cols), {name: (T, rows, cols)}, DeviceScene)."""
+        ds = self.upload(tile)
+        fields, out = self.solve_device(ds, met, sun, latitude, outputs)
+        torch.cuda.synchronize()
+        ds.check_flags()
+        names = self.output_names(outputs)
+        host = out.cpu().numpy()
+        res = {nm: host[:, j] for j, nm in enumerate(names)}
+        return fields.reshape(L.NFIELDS, ds.rows, ds.cols).cpu().numpy(), res, ds

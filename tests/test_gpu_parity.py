@@ -1,0 +1,165 @@
+"""HIP path (through the C ABI) against the reference's golden vectors and the CPU oracle.
+
+Bars: shadow / vegetation / wall masks bit-exact; SVF <= 1e-5; Tmrt <= 0.01 K;
+UTCI <= 0.01 C. The oracle is only the checker here.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+SCENES = ["a48", "b40x56"]
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def solver():
+    from solweig_project_amd import _lib
+    from solweig_project_amd.solver import HipSolver
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    _lib.load()
+    return HipSolver("cuda:0")
+
+
+def _scene(g):
+    from solweig_project_amd.solver import compose_scene
+    return compose_scene(g["dsm"], g["dem"], g["cdsm"], g["walls"], g["aspect"], g["landcover"],
+                         scale=float(g["scale"]))
+
+
+def _sun(d):
+    from solweig_project_amd import met as M
+    return M.SunTable(d["altitude"][0], d["azimuth"][0], d["zen"][0], d["jday"][0], d["dectime"], d["altmax"][0])
+
+
+def test_library_loads_and_reports(solver):
+    from solweig_project_amd import _lib
+    lib = _lib.load()
+    assert lib.sw_abi_version() == 1
+    n = _lib.ctypes.c_int(0)
+    _lib.call("sw_device_count", _lib.ctypes.byref(n))
+    assert n.value >= 1
+    assert b"gfx950" in lib.sw_build_flags()
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_svf_sweep_matches_reference(solver, golden_dir, name):
+    _say(f"svf {name}")
+    g = np.load(golden_dir / f"svf_{name}.npz")
+    ds = solver.upload(_scene(g))
+    solver.svf(ds)
+    cubes = [c.cpu().numpy() for c in solver.unpack_masks(ds)]
+    torch.cuda.synchronize()
+    for got, key in zip(cubes, ("shmat", "vegshmat", "vbshvegshmat")):
+        assert np.array_equal(got, g[key].astype(np.float32)), key
+    fields = ds.fields.reshape(16, ds.rows, ds.cols).cpu().numpy()
+    from oracle.driver import FOLD_NAMES
+    for k, nm in enumerate(FOLD_NAMES):
+        assert np.abs(fields[k] - g[nm]).max() <= 1e-5, nm
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_full_day_matches_reference(solver, golden_dir, name):
+    _say(f"day {name}")
+    g = np.load(golden_dir / f"svf_{name}.npz")
+    d = np.load(golden_dir / f"day_{name}.npz")
+    fields, out, _ = solver.solve(_scene(g), d["met"], _sun(d), float(d["latitude"]),
+                                  outputs=("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup"))
+    assert np.array_equal(out["shadow"], d["o_shadow"])
+    assert np.abs(out["tmrt"] - d["o_tmrt"]).max() <= 0.01
+    ref = d["o_utci"]
+    assert np.array_equal(np.isnan(out["utci"]), np.isnan(ref))
+    m = ~np.isnan(ref)
+    assert np.abs(out["utci"][m] - ref[m]).max() <= 0.01
+    for k in ("kdown", "kup", "ldown", "lup"):
+        assert np.abs(out[k] - d["o_" + k]).max() <= 0.01, k
+
+
+def test_sun_march_single_abi_matches_reference(solver, golden_dir):
+    """sw_run_march_wallheight23 against the reference's recorded sun-march masks."""
+    from solweig_project_amd import _lib as L
+    from solweig_project_amd import skyvault as SV
+    g = np.load(golden_dir / "svf_a48.npz")
+    d = np.load(golden_dir / "day_a48.npz")
+    sc = _scene(g)
+    rows, cols = sc.shape
+    dev = torch.device("cuda:0")
+    a, v1, v2 = (torch.from_numpy(x).to(dev) for x in (sc.a, sc.vegdsm, sc.vegdsm2))
+    for j in range(d["m_sh"].shape[0]):
+        t = SV.sun_table(float(d["m_az"][j]), float(d["m_alt"][j]), sc.scale, float(d["m_amax"][j]), rows, cols)
+        dx, dy, dz, dzp = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (t.dx, t.dy, t.dz, t.dzprev))
+        sh, vs, vb = (torch.empty((rows, cols), dtype=torch.float32, device=dev) for _ in range(3))
+        L.call("sw_run_march_wallheight23", L.ptr(a), L.ptr(v1), L.ptr(v2), L.ptr(dx), L.ptr(dy), L.ptr(dz),
+               L.ptr(dzp), L.ptr(sh), L.ptr(vs), L.ptr(vb), rows, cols, t.count, 0)
+        torch.cuda.synchronize()
+        assert np.array_equal(sh.cpu().numpy(), d["m_sh"][j]), j
+        assert np.array_equal(vs.cpu().numpy(), d["m_vegsh"][j]), j
+        assert np.array_equal(vb.cpu().numpy(), d["m_vbsh"][j]), j
+
+
+def test_utci_kernel_matches_reference(solver, golden_dir):
+    from solweig_project_amd import _lib as L
+    z = np.load(golden_dir / "utci_grid.npz")
+    dev = torch.device("cuda:0")
+    ins = [torch.from_numpy(z[k]).to(dev) for k in ("ta", "rh", "tmrt", "va")]
+    out = torch.empty_like(ins[0])
+    L.call("sw_utci", *[L.ptr(x) for x in ins], L.ptr(out), out.numel(), 0)
+    got = out.cpu().numpy()
+    ref = z["utci"]
+    inv = ref == -999
+    assert np.array_equal(got == -999, inv)
+    assert np.abs(got[~inv] - ref[~inv]).max() <= 0.01
+
+
+@pytest.fixture(scope="module")
+def c1_case():
+    """BASELINE config 1 generator at 200 x 200 x 24 h (seeded) with the oracle's answer."""
+    from oracle import driver as D
+    from solweig_project_amd import synth
+    from solweig_project_amd import met as M
+    import json
+    site = synth.make_site(200, 200, seed=20261203)
+    z = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "sun_tables.npz")
+    sun = M.SunTable(z["c0_alt"], z["c0_azi"], z["c0_zen"], z["c0_jday"], z["c0_dectime"], z["c0_altmax"])
+    met = synth.make_met(z["c0_alt"], seed=20261204)
+    sc = D.compose(site.dsm, site.dem, site.cdsm, site.scale)
+    fields, shm, vm, bm = D.svf(sc)
+    outs = D.day(sc, fields, shm, vm, bm, site.walls, site.aspect, site.landcover, met, sun, site.latitude)
+    return dict(site=site, sun=sun, met=met, fields=fields, cubes=(shm, vm, bm), outs=outs)
+
+
+def test_c1_matches_oracle(solver, c1_case):
+    from solweig_project_amd.solver import compose_scene
+    _say("c1 200x200x24 vs oracle")
+    s = c1_case["site"]
+    scene = compose_scene(s.dsm, s.dem, s.cdsm, s.walls, s.aspect, s.landcover, scale=s.scale)
+    fields, out, ds = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude,
+                                   outputs=("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup"))
+    cubes = [c.cpu().numpy() for c in solver.unpack_masks(ds)]
+    rows, cols = scene.shape
+    for got, ref in zip(cubes, c1_case["cubes"]):
+        assert np.array_equal(got, ref.reshape(153, rows, cols).transpose(1, 2, 0))
+    assert np.abs(fields - c1_case["fields"]).max() <= 1e-5
+    o = c1_case["outs"]
+    assert np.array_equal(out["shadow"], o["shadow"])
+    assert np.abs(out["tmrt"] - o["tmrt"]).max() <= 0.01
+    m = ~np.isnan(o["utci"])
+    assert np.array_equal(np.isnan(out["utci"]), ~m)
+    assert np.abs(out["utci"][m] - o["utci"][m]).max() <= 0.01
+
+
+def test_repeat_runs_bitwise_identical(solver, c1_case):
+    from solweig_project_amd.solver import compose_scene
+    s = c1_case["site"]
+    scene = compose_scene(s.dsm, s.dem, s.cdsm, s.walls, s.aspect, s.landcover, scale=s.scale)
+    f1, o1, _ = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude)
+    f2, o2, _ = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude)
+    assert np.array_equal(f1, f2)
+    for k in o1:
+        assert np.array_equal(o1[k], o2[k], equal_nan=True), k
