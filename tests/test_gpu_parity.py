@@ -87,8 +87,9 @@ def test_sun_march_single_abi_matches_reference(solver, golden_dir):
     from solweig_project_amd import skyvault as SV
     g = np.load(golden_dir / "svf_a48.npz")
     d = np.load(golden_dir / "day_a48.npz")
-    sc = _scene(g)
-    rows, cols = sc.shape
+    from oracle.driver import compose
+    sc = compose(g["dsm"], g["dem"], g["cdsm"], float(g["scale"]))
+    rows, cols = sc.a.shape
     dev = torch.device("cuda:0")
     a, v1, v2 = (torch.from_numpy(x).to(dev) for x in (sc.a, sc.vegdsm, sc.vegdsm2))
     for j in range(d["m_sh"].shape[0]):
