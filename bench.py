@@ -36,7 +36,11 @@ def parse():
     ap.add_argument("--cols", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=200, help="side of the CPU-oracle sample tile")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="side of the CPU-oracle tile (0: the benchmark tile itself)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run k_pack_heights over 16 Mi pixels (12 B read + 16 B written per pixel) "
+                         "as the known-bytes reference for rocprof FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -49,14 +53,10 @@ def sun_for_site(lat, lon, utc, year, doy):
     return sunpos.day_sun_table(lat, lon, utc, year, doy)
 
 
-def cpu_baseline(rows: int, seed: int):
-    """CPU oracle (C restatement of the reference algorithm, OpenMP) on a bounded sample."""
-    import numpy as np
+def cpu_baseline(site, sun, met):
+    """CPU oracle (C restatement of the reference algorithm, OpenMP, all cores) on the same workload."""
     from oracle import driver as D
-    from solweig_project_amd import synth
-    site = synth.make_site(rows, rows, seed=seed)
-    sun = sun_for_site(site.latitude, site.longitude, site.utc_offset, 2021, 180)
-    met = synth.make_met(sun.altitude, seed=seed + 1)
+    rows = site.dsm.shape[0]
     D.lib()
     t0 = time.perf_counter()
     sc = D.compose(site.dsm, site.dem, site.cdsm, site.scale)
@@ -64,8 +64,10 @@ def cpu_baseline(rows: int, seed: int):
     D.day(sc, fields, shm, vm, bm, site.walls, site.aspect, site.landcover, met, sun, site.latitude)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=round(rows * rows * met.shape[0] / dt / 1e6, 5), unit="Mpixel-steps/s", cores=cores,
-                kind="port", sample=f"C oracle full solve of a {rows}x{rows}x24h tile (same generator), "
+    cols = site.dsm.shape[1]
+    return dict(value=round(rows * cols * met.shape[0] / dt / 1e6, 5), unit="Mpixel-steps/s", cores=cores,
+                kind="port", sample=f"C oracle (plane-sweep restatement of the reference CPU path) full solve "
+                                    f"of a {rows}x{cols}x{met.shape[0]}h tile from the same generator, "
                                     f"{dt:.1f} s wall")
 
 
@@ -148,10 +150,25 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": int(alg[dom])}
+    if args.calibrate_copy:
+        from solweig_project_amd import _lib as L
+        m = 16 * 1024 * 1024
+        planes = [torch.ones(m, dtype=torch.float32, device=dev) for _ in range(3)]
+        hp = torch.empty((m, 4), dtype=torch.float32, device=dev)
+        L.call("sw_pack_heights", L.ptr(planes[0]), L.ptr(planes[1]), L.ptr(planes[2]), L.ptr(hp), 4096, 4096,
+               torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        log(f"copy calibration: k_pack_heights read {12 * m} B, wrote {16 * m} B")
     cpu = None
     if not args.no_cpu_baseline:
-        log("CPU oracle baseline")
-        cpu = cpu_baseline(args.cpu_rows, args.seed)
+        if args.cpu_rows:
+            csite = synth.make_site(args.cpu_rows, args.cpu_rows, seed=args.seed)
+            csun = sun_for_site(csite.latitude, csite.longitude, csite.utc_offset, 2021, 180)
+            cmet = synth.make_met(csun.altitude, seed=args.seed + 1)
+        else:
+            csite, csun, cmet = site, sun, met
+        log(f"CPU oracle baseline on {csite.dsm.shape}")
+        cpu = cpu_baseline(csite, csun, cmet)
     line = {
         "metric": "Mpixel-steps/s (Tmrt+UTCI, 1000²×24h)",
         "value": round(value, 3),

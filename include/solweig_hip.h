@@ -74,7 +74,7 @@ typedef struct sw_model {
     float absK, absL, Fside, Fup, Fcyl, albedo_b, ewall, sbc;
     float first, second;      /* walk snapshot / length (rounded, scaled) */
     int32_t nsec;             /* walk steps per azimuth (<= SW_WALK_MAX) */
-    int32_t pad0;
+    int32_t walk_halo;        /* max |walk offset| over all azimuths (<= 48) */
     float alb_class[SW_NCLASS], emis_class[SW_NCLASS];
     int32_t walk_dx[SW_WALK_AZ * SW_WALK_MAX], walk_dy[SW_WALK_AZ * SW_WALK_MAX];
     int32_t walk_branch[SW_WALK_AZ], walk_dir[SW_WALK_AZ];
@@ -108,11 +108,13 @@ int sw_pack_heights(const float *a, const float *vegdsm, const float *vegdsm2, v
 int sw_compose(const float *dsm, const float *dem, const float *cdsm, const float *aspect_deg,
                const float *landcover, void *hpack, float *vegdsm2, float *buildings, uint8_t *cls,
                float *aspect_rad, uint32_t *stats, int rows, int cols, void *stream);
-/* Patch tables carry every geometrically valid step; the kernel applies
- * the reference's amax stop test with amax from stats. */
-int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *dx, const int32_t *dy,
-                 const float *dz, uint32_t *masks, const uint32_t *stats, int rows, int cols,
-                 void *stream);
+/* Patch tables: tab[offs[p] .. offs[p+1]) holds int4 {dx, dy, dz bits,
+ * dx * cols + dy} for every geometrically valid step of patch p; the
+ * library applies the reference's amax stop test with amax from stats,
+ * writing the executed step count of each patch to counts[153]
+ * (caller-owned workspace). */
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts,
+                 uint32_t *masks, const uint32_t *stats, int rows, int cols, void *stream);
 /* Fold the masks into the 16 fields [svf, E, S, W, N, veg, vegE.., aveg, avegE.., total]
  * (shadow.py:545-613), in the reference's per-annulus summation order. */
 int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso,
@@ -157,12 +159,13 @@ int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
  * state: 6 planes (Tgmap1, E, S, W, N, TgOut1), read and updated.
  * out: planes selected by out_mask bits: 1 tmrt, 2 utci, 4 shadow,
  * 8 kdown, 16 kup, 32 ldown, 64 lup, written at out + k * n for the k-th
- * selected plane in that bit order. */
+ * selected plane in that bit order. walk_halo must equal model->walk_halo
+ * (it sizes the shared-memory tile of walk sources). */
 int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg,
                  const float *tan_asvf, const float *tan_svfalfa, const uint32_t *walkface,
                  const float *buildings, const uint8_t *cls, const sw_step_t *steps,
                  const sw_model_t *model, float *state, float *out, int out_mask, int step,
-                 int rows, int cols, void *stream);
+                 int walk_halo, int rows, int cols, void *stream);
 
 /* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,

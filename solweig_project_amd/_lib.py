@@ -54,7 +54,7 @@ class ModelStruct(ctypes.Structure):
     _fields_ = [
         ("absK", _c_f), ("absL", _c_f), ("Fside", _c_f), ("Fup", _c_f), ("Fcyl", _c_f),
         ("albedo_b", _c_f), ("ewall", _c_f), ("sbc", _c_f), ("first", _c_f), ("second", _c_f),
-        ("nsec", _c_i32), ("pad0", _c_i32),
+        ("nsec", _c_i32), ("walk_halo", _c_i32),
         ("alb_class", _c_f * NCLASS), ("emis_class", _c_f * NCLASS),
         ("walk_dx", _c_i32 * (WALK_AZ * WALK_MAX)), ("walk_dy", _c_i32 * (WALK_AZ * WALK_MAX)),
         ("walk_branch", _c_i32 * WALK_AZ), ("walk_dir", _c_i32 * WALK_AZ),
@@ -74,14 +74,14 @@ SIGNATURES = {
     "sw_sync": (_I, [_VP]),
     "sw_pack_heights": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_compose": (_I, [_VP] * 11 + [_I, _I, _VP]),
-    "sw_svf_march": (_I, [_VP] * 7 + [_I, _I, _VP]),
+    "sw_svf_march": (_I, [_VP] * 6 + [_I, _I, _VP]),
     "sw_svf_fold": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_unpack_masks": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_run_march_svf_shadow": (_I, [_VP] * 9 + [_I, _I, _I, _VP]),
     "sw_run_march_wallheight23": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_tile_static": (_I, [_VP] * 8 + [_I, _I, _VP]),
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
-    "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _VP]),
+    "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
 }
 

@@ -52,6 +52,22 @@ __device__ __forceinline__ float d_asin(float x) { return (float)asin((double)x)
 __device__ __forceinline__ float d_exp(float x) { return (float)exp((double)x); }
 __device__ __forceinline__ float d_log(float x) { return (float)log((double)x); }
 __device__ __forceinline__ float d_pow(float x, double e) { return (float)pow((double)x, e); }
+// x^4, x^5, x^6 of a float, rounded once to float from a double product
+// chain: x^2 is exact in double, each further product rounds at 2^-53,
+// far below the float rounding step.
+struct Pow456 {
+    float p4, p5, p6;
+};
+__device__ __forceinline__ Pow456 pow456(float x) {
+    const double d = (double)x;
+    const double d2 = d * d;
+    const double d4 = d2 * d2;
+    Pow456 r;
+    r.p4 = (float)d4;
+    r.p5 = (float)(d4 * d);
+    r.p6 = (float)(d4 * d2);
+    return r;
+}
 
 // Map a launch-order block index to a tile so that consecutive tiles land on
 // the same XCD (blocks are dealt round-robin over the 8 XCDs, each with its
@@ -142,60 +158,124 @@ k_compose(const float *__restrict__ dsm, const float *__restrict__ dem, const fl
 }
 
 // ---------------------------------------------------------------------------
-// one SVF patch march for pixel (r, c); returns sh | vegsh<<1 | vbsh<<2
-// (vbsh in 0..2). Semantics of shadow(): out-of-bounds sources read an
+// SVF patch march. Semantics of shadow(): out-of-bounds sources read an
 // exact 0, the first step carries the vegetation condition and the
-// target-local trunk gate and clears the accumulator.
+// target-local trunk gate and clears the accumulator, and a step runs only
+// while the previous step had amax >= dz (per-patch counts computed on the
+// device by k_svf_counts). Tables: int4 {dx, dy, dz bits, dx * cols + dy}.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t march_svf_pixel(const float4 *__restrict__ hp, int rows, int cols,
+__global__ void k_svf_counts(const int4 *__restrict__ tab, const int32_t *__restrict__ offs,
+                             const uint32_t *__restrict__ stats, int32_t *__restrict__ counts) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= SW_NPATCH) return;
+    const float amax = funkey(stats[0]);
+    const int o = offs[p], len = offs[p + 1] - o;
+    int c = len;
+    for (int s = 0; s < len; ++s) {
+        if (!(amax >= __int_as_float(tab[o + s].z))) {
+            c = s + 1;
+            break;
+        }
+    }
+    counts[p] = c;
+}
+
+struct SvfState {
+    float f;
+    int sh, vs, vb;
+};
+
+// one march step; returns whether the source was inside the grid
+__device__ __forceinline__ bool svf_gather(const float4 *__restrict__ hp, long t, int r, int c, int rows, int cols,
+                                           int4 e, float *ha, float *hv, float *hv2, bool want) {
+    const bool inb = ((unsigned)(r + e.x) < (unsigned)rows) && ((unsigned)(c + e.y) < (unsigned)cols);
+    *ha = 0.f; *hv = 0.f; *hv2 = 0.f;
+    if (inb && want) {
+        const float4 h = hp[t + e.w];
+        *ha = h.x; *hv = h.y; *hv2 = h.z;
+    }
+    return inb;
+}
+
+__device__ __forceinline__ void svf_update(SvfState &st, bool inb, float ha, float hv, float hv2, float dzs,
+                                           float at, float veg2t, bool first) {
+    float tmp = 0.f, tv = 0.f, tv2 = 0.f;
+    if (inb) {
+        tv = hv - dzs;
+        tv2 = hv2 - dzs;
+        tmp = ha - dzs;
+    }
+    st.f = fmaxf(st.f, tmp);
+    st.sh = (st.f > at) ? 1 : 0;
+    const int v2 = (tv > at ? 1 : 0) - (tv2 > at ? 1 : 0);
+    st.vs = st.vs > v2 ? st.vs : v2;
+    if (st.vs * st.sh > 0) st.vs = 0;
+    st.vb = st.vs + st.vb;
+    if (first) {
+        float fv = tv - tmp;
+        if (fv <= 0.f) fv = 1000.f;
+        if (fv < dzs) st.vs = 1;
+        st.vs = st.vs * ((veg2t > at) ? 1 : 0);
+        st.vb = 0;
+    }
+}
+
+// exact early exits (state is a fixed point of every later step):
+// sh == 1 && vs == 0, or (after the first step, for a non-negative target)
+// every later source is out of the grid or below the target.
+__device__ __forceinline__ bool svf_exit(const SvfState &st, int s, bool exit_ok, bool inb, float hmax, float dzs,
+                                         float at) {
+    if (st.sh == 1 && st.vs == 0) return true;
+    return s >= 1 && exit_ok && (!inb || (hmax - dzs) <= at);
+}
+
+__device__ __forceinline__ uint32_t march_svf_pixel(const float4 *__restrict__ hp, int rows, int cols, long t,
                                                     int r, int c, float at, float veg2t, bool valid,
-                                                    const int32_t *__restrict__ dx,
-                                                    const int32_t *__restrict__ dy,
-                                                    const float *__restrict__ dz, int count,
-                                                    float amax, float hmax) {
-    float f = at;
-    int sh = 0, vs = 0, vb = 0;
+                                                    const int4 *__restrict__ T, int count, float hmax) {
+    SvfState st;
+    st.f = at; st.sh = 0; st.vs = 0; st.vb = 0;
     bool active = valid;
     const bool exit_ok = (at >= 0.f);
-    for (int s = 0; s < count; ++s) {
+    int s = 0;
+    // chunks of four steps: four independent gathers in flight per lane
+    for (; s + 4 <= count; s += 4) {
         if (!wave_any(active)) break;
+        const int4 e0 = T[s], e1 = T[s + 1], e2 = T[s + 2], e3 = T[s + 3];
+        float a0, v0, w0, a1, v1, w1, a2, v2, w2, a3, v3, w3;
+        const bool i0 = svf_gather(hp, t, r, c, rows, cols, e0, &a0, &v0, &w0, active);
+        const bool i1 = svf_gather(hp, t, r, c, rows, cols, e1, &a1, &v1, &w1, active);
+        const bool i2 = svf_gather(hp, t, r, c, rows, cols, e2, &a2, &v2, &w2, active);
+        const bool i3 = svf_gather(hp, t, r, c, rows, cols, e3, &a3, &v3, &w3, active);
         if (active) {
-            const int sr = r + dx[s], sc = c + dy[s];
-            const float dzs = dz[s];
-            const bool inb = ((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols);
-            float tmp = 0.f, tv = 0.f, tv2 = 0.f;
-            if (inb) {
-                const float4 h = hp[(long)sr * cols + sc];
-                tv = h.y - dzs;
-                tv2 = h.z - dzs;
-                tmp = h.x - dzs;
-            }
-            f = fmaxf(f, tmp);
-            sh = (f > at) ? 1 : 0;
-            const int v2 = (tv > at ? 1 : 0) - (tv2 > at ? 1 : 0);
-            vs = vs > v2 ? vs : v2;
-            if (vs * sh > 0) vs = 0;
-            vb = vs + vb;
-            if (s == 0) {
-                float fv = tv - tmp;
-                if (fv <= 0.f) fv = 1000.f;
-                if (fv < dzs) vs = 1;
-                vs = vs * ((veg2t > at) ? 1 : 0);
-                vb = 0;
-            }
-            // exact early exits: fixed point, or no source can rise above
-            // the target any more (all later sources out of bounds or lower).
-            if (sh == 1 && vs == 0) active = false;
-            else if (s >= 1 && exit_ok && (!inb || (hmax - dzs) <= at)) active = false;
+            const float d0 = __int_as_float(e0.z), d1 = __int_as_float(e1.z), d2 = __int_as_float(e2.z),
+                        d3 = __int_as_float(e3.z);
+            bool ex;
+            svf_update(st, i0, a0, v0, w0, d0, at, veg2t, s == 0);
+            ex = svf_exit(st, s, exit_ok, i0, hmax, d0, at);
+            svf_update(st, i1, a1, v1, w1, d1, at, veg2t, false);
+            ex = ex || svf_exit(st, s + 1, exit_ok, i1, hmax, d1, at);
+            svf_update(st, i2, a2, v2, w2, d2, at, veg2t, false);
+            ex = ex || svf_exit(st, s + 2, exit_ok, i2, hmax, d2, at);
+            svf_update(st, i3, a3, v3, w3, d3, at, veg2t, false);
+            ex = ex || svf_exit(st, s + 3, exit_ok, i3, hmax, d3, at);
+            if (ex) active = false;
         }
-        // stop test of the reference loop: the next step runs only while
-        // amax >= dz of this step (the geometric part is cut on the host)
-        if (!(amax >= dz[s])) break;
     }
-    const int out_sh = 1 - sh;
-    int b = vb > 0 ? 1 : vb;
-    b = b - vs;
-    const int vsf = vs > 0 ? 1 : vs;
+    for (; s < count; ++s) {
+        if (!wave_any(active)) break;
+        const int4 e = T[s];
+        float ha, hv, hv2;
+        const bool inb = svf_gather(hp, t, r, c, rows, cols, e, &ha, &hv, &hv2, active);
+        if (active) {
+            const float dzs = __int_as_float(e.z);
+            svf_update(st, inb, ha, hv, hv2, dzs, at, veg2t, s == 0);
+            if (svf_exit(st, s, exit_ok, inb, hmax, dzs, at)) active = false;
+        }
+    }
+    const int out_sh = 1 - st.sh;
+    int b = st.vb > 0 ? 1 : st.vb;
+    b = b - st.vs;
+    const int vsf = st.vs > 0 ? 1 : st.vs;
     const int out_vs = 1 - vsf;
     const int out_vb = 1 - b;
     return (uint32_t)out_sh | ((uint32_t)out_vs << 1) | ((uint32_t)out_vb << 2);
@@ -203,12 +283,10 @@ __device__ __forceinline__ uint32_t march_svf_pixel(const float4 *__restrict__ h
 
 // grid: x = pixel tiles of 256 (XCD-remapped), y = 32-patch word group.
 __global__ void __launch_bounds__(256)
-k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs,
-            const int32_t *__restrict__ dx, const int32_t *__restrict__ dy,
-            const float *__restrict__ dz, uint32_t *__restrict__ masks,
+k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
+            const int32_t *__restrict__ counts, uint32_t *__restrict__ masks,
             const uint32_t *__restrict__ stats, int rows, int cols) {
     const long n = (long)rows * cols;
-    const float amax = funkey(stats[0]);
     const float hmax = funkey(stats[1]);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const long t = (long)tile * 256 + threadIdx.x;
@@ -226,10 +304,8 @@ k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs,
     const int p1 = min(p0 + 32, SW_NPATCH);
     uint32_t bsh = 0, bvs = 0, bv1 = 0, bv2 = 0;
     for (int p = p0; p < p1; ++p) {
-        const int o = offs[p];
-        const int cnt = offs[p + 1] - o;
-        const uint32_t m = march_svf_pixel(hp, rows, cols, r, c, at, veg2t, valid, dx + o, dy + o, dz + o,
-                                           cnt, amax, hmax);
+        const uint32_t m = march_svf_pixel(hp, rows, cols, valid ? t : 0, r, c, at, veg2t, valid, tab + offs[p],
+                                           counts[p], hmax);
         const uint32_t bit = 1u << (p - p0);
         if (m & 1u) bsh |= bit;
         if (m & 2u) bvs |= bit;
@@ -457,6 +533,10 @@ __device__ __forceinline__ float pow4(float x) {
     return (float)(d2 * d2);
 }
 
+// Walk-source tile in LDS: a 64 x 4 pixel tile plus the walk halo.
+#define FLUX_TX 64
+#define FLUX_TY 4
+
 __global__ void __launch_bounds__(256)
 k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
            const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
@@ -528,19 +608,16 @@ __device__ __forceinline__ float utci_point(float ta, float rh, float tmrt, floa
     es = es + -2.737830188E-2f * tk;
     es = es + 1.6261698E-5f * (tk * tk);
     es = es + 7.0229056E-10f * (tk * tk * tk);
-    es = es + -1.8680009E-13f * d_pow(tk, 4.0);
+    es = es + -1.8680009E-13f * pow456(tk).p4;
     es = d_exp(es) * 0.01f;
     const float ehPa = es * rh / 100.f;
     const float dtm = tmrt - ta;
     const float pa = ehPa / 10.0f;
-    const float ta1 = ta, ta2 = ta * ta, ta3 = ta * ta * ta, ta4 = d_pow(ta, 4.0), ta5 = d_pow(ta, 5.0),
-                ta6 = d_pow(ta, 6.0);
-    const float va1 = va, va2 = va * va, va3 = va * va * va, va4 = d_pow(va, 4.0), va5 = d_pow(va, 5.0),
-                va6 = d_pow(va, 6.0);
-    const float dt1 = dtm, dt2 = dtm * dtm, dt3 = dtm * dtm * dtm, dt4 = d_pow(dtm, 4.0),
-                dt5 = d_pow(dtm, 5.0), dt6 = d_pow(dtm, 6.0);
-    const float pa1 = pa, pa2 = pa * pa, pa3 = pa * pa * pa, pa4 = d_pow(pa, 4.0), pa5 = d_pow(pa, 5.0),
-                pa6 = d_pow(pa, 6.0);
+    const Pow456 qt = pow456(ta), qv = pow456(va), qd = pow456(dtm), qp = pow456(pa);
+    const float ta1 = ta, ta2 = ta * ta, ta3 = ta * ta * ta, ta4 = qt.p4, ta5 = qt.p5, ta6 = qt.p6;
+    const float va1 = va, va2 = va * va, va3 = va * va * va, va4 = qv.p4, va5 = qv.p5, va6 = qv.p6;
+    const float dt1 = dtm, dt2 = dtm * dtm, dt3 = dtm * dtm * dtm, dt4 = qd.p4, dt5 = qd.p5, dt6 = qd.p6;
+    const float pa1 = pa, pa2 = pa * pa, pa3 = pa * pa * pa, pa4 = qp.p4, pa5 = qp.p5, pa6 = qp.p6;
     float u = ta;
 #include "utci_poly.inc"
     (void)ta6; (void)va6; (void)dt6; (void)pa6;
@@ -572,18 +649,45 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
             const float *__restrict__ bu, const uint8_t *__restrict__ cls,
             const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
-            float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int rows,
-            int cols) {
+            float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int halo,
+            int rows, int cols) {
+    extern __shared__ uint2 s_src[];
     __shared__ float s_alb[SW_NCLASS];
-    if (threadIdx.x < SW_NCLASS) s_alb[threadIdx.x] = model->alb_class[threadIdx.x];
-    __syncthreads();
-    const long n = (long)rows * cols;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const long t = (long)tile * 256 + threadIdx.x;
-    if (t >= n) return;
     const sw_step_t *sp = steps + step;
     const sw_model_t *md = model;
-    const int r = (int)(t / cols), c = (int)(t - (long)r * cols);
+    const int tiles_x = (cols + FLUX_TX - 1) / FLUX_TX;
+    const int tiles_y = (rows + FLUX_TY - 1) / FLUX_TY;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= tiles_x * tiles_y) return;  // whole block idle (uniform)
+    const int r0 = (tile / tiles_x) * FLUX_TY, c0 = (tile % tiles_x) * FLUX_TX;
+    const int tx = threadIdx.x % FLUX_TX, ty = threadIdx.x / FLUX_TX;
+    const int r = r0 + ty, c = c0 + tx;
+    const bool valid = (r < rows) && (c < cols);
+    const long n = (long)rows * cols;
+    const long t = valid ? (long)r * cols + c : 0;
+    const bool day = sp->is_day != 0;
+    const int lw = FLUX_TX + 2 * halo, lh = FLUX_TY + 2 * halo;
+    if (threadIdx.x < SW_NCLASS) s_alb[threadIdx.x] = md->alb_class[threadIdx.x];
+    if (day) {
+        // stage the walk sources of the tile + halo (out-of-grid cells are
+        // never read: the walk tracks liveness on global coordinates)
+        for (int i = threadIdx.x; i < lw * lh; i += blockDim.x) {
+            const int ly = i / lw, lx = i - ly * lw;
+            const int gr = r0 - halo + ly, gc = c0 - halo + lx;
+            uint2 v = make_uint2(0u, 0u);
+            if ((unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols) v = src[(long)gr * cols + gc];
+            s_src[i] = v;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    if (md->walk_halo > halo) {
+        // the shared tile is smaller than the walk reach: refuse loudly
+        int k = 0;
+        for (int b = 1; b <= 64; b <<= 1)
+            if (out_mask & b) out[(long)(k++) * n + t] = __builtin_nanf("");
+        return;
+    }
     const float b_t = bu[t];
     const int k_t = cls[t];
     const float se_t = md->sbc * md->emis_class[k_t];
@@ -592,7 +696,10 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
     load_masks(masks, n, t, msh, mvs, mv1, mv2);
     const float hsvf = tan_asvf[t];
-    if (sp->is_day) {
+    float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
+    float F_sh = 0.f, galb = 0.f, gnosh = 0.f, galbE = 0.f, gnoshE = 0.f, galbS = 0.f, gnoshS = 0.f,
+          galbW = 0.f, gnoshW = 0.f, galbN = 0.f, gnoshN = 0.f;
+    if (day) {
         const uint2 me = src[t];
         const uint32_t fl_t = me.y;
         const float lup_t = __uint_as_float(me.x);
@@ -608,6 +715,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         const float lwall = sp->lwall, albb = md->albedo_b;
         const uint32_t wf = walkface[t];
         const float nb = b_t * -1.f + 1.f;
+        const int lbase = (ty + halo) * lw + (tx + halo);
         for (int j = 0; j < SW_WALK_AZ; ++j) {
             float tb = 0.f, ts = 0.f, tl = 0.f, ta_ = 0.f, tn = 0.f, tw = 0.f;
             float f = b_t, bub = 0.f, bubw = 0.f;
@@ -617,13 +725,13 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
                   f_walbsh = 0.f, f_walbwallnosh = 0.f, f_walbnosh = 0.f;
             bool live = true;
             for (int s = 0; s < nsec; ++s) {
-                const int sr = r + md->walk_dx[j * SW_WALK_MAX + s];
-                const int sc = c + md->walk_dy[j * SW_WALK_MAX + s];
+                const int ddx = md->walk_dx[j * SW_WALK_MAX + s];
+                const int ddy = md->walk_dy[j * SW_WALK_MAX + s];
                 // a source that left the grid stays out (offsets grow monotonically);
                 // the temporaries then keep the last in-grid values (stale border)
-                live = live && ((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols);
+                live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols);
                 if (live) {
-                    const uint2 e = src[(long)sr * cols + sc];
+                    const uint2 e = s_src[lbase + ddx * lw + ddy];
                     const uint32_t fl = e.y;
                     const float shd = (float)(fl & 1u) - (1.f - (float)((fl >> 1) & 1u)) * omp;
                     const float al = s_alb[(fl >> 4) & 7u];
@@ -682,11 +790,15 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         }
         const float lup_air = se_t * sp->tak4;
         const float gLup = gL / 18.f + lup_air;
-        const float galb = gA / 18.f, gnosh = gN / 18.f;
-        const float gLupE = gLE / 9.f + lup_air, galbE = gAE / 9.f, gnoshE = gNE / 9.f;
-        const float gLupS = gLS / 9.f + lup_air, galbS = gAS / 9.f, gnoshS = gNS / 9.f;
-        const float gLupW = gLW / 9.f + lup_air, galbW = gAW / 9.f, gnoshW = gNW / 9.f;
-        const float gLupN = gLN / 9.f + lup_air, galbN = gAN / 9.f, gnoshN = gNN / 9.f;
+        galb = gA / 18.f; gnosh = gN / 18.f;
+        const float gLupE = gLE / 9.f + lup_air;
+        galbE = gAE / 9.f; gnoshE = gNE / 9.f;
+        const float gLupS = gLS / 9.f + lup_air;
+        galbS = gAS / 9.f; gnoshS = gNS / 9.f;
+        const float gLupW = gLW / 9.f + lup_air;
+        galbW = gAW / 9.f; gnoshW = gNW / 9.f;
+        const float gLupN = gLN / 9.f + lup_air;
+        galbN = gAN / 9.f; gnoshN = gNN / 9.f;
         // ---- thermal delay (TsWaveDelay_2015a x 6, shared weights) ----
         const float tg0[6] = {gLup, gLupE, gLupS, gLupW, gLupN, sp->tg_class[k_t] * shadow + sp->ta};
         float L[6];
@@ -719,27 +831,67 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         }
         const float Ssurf = hkil + ukil;
         const float two_pi = 6.28318530717958647692f;
-        float F_sh = (two_pi * ba - Ssurf) / (two_pi * ba);
+        F_sh = (two_pi * ba - Ssurf) / (two_pi * ba);
         if (F_sh != F_sh) F_sh = 0.5f;
-        // ---- diffuse sky and Kside over the 153 patches ----
-        float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
-        const float sunlit_k = sp->sunlit_k, shaded_k = sp->shaded_k;
-        for (int p = 0; p < SW_NPATCH; ++p) {
-            float sh, vs, vb;
-            mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
-            const float ster = sp->ster[p], calt = sp->cos_alt[p];
-            const float diffsh = sh - (1.f - vs) * 0.97f;
-            aniLum = aniLum + diffsh * sp->lv[p];
-            KsideD = KsideD + diffsh * sp->lumchi[p] * sp->angl[p] * ster;
+    } else {
+        const float x = sp->ta + 273.15f;
+        Lup = se_t * pow4(x);
+        LupE = LupS = LupW = LupN = Lup;
+    }
+    // ---- pass A over the 153 patches: diffuse sky + Kside (day) and the
+    // first define_patch_characteristics pass (every step); each accumulator
+    // keeps the reference's patch order ----
+    float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f, Ldown_ref = 0.f;
+    float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f, Lside_ref = 0.f;
+    const float veg_l = sp->veg_l, sunlit_l = sp->sunlit_l, shaded_l = sp->shaded_l;
+    const float sunlit_k = sp->sunlit_k, shaded_k = sp->shaded_k;
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        // this word's bits, shifted down one patch per iteration
+        uint32_t bsh = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
+        uint32_t bvs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
+        uint32_t bv1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
+        uint32_t bv2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
+        const int pend = min(w * 32 + 32, SW_NPATCH);
+#pragma unroll 4
+        for (int p = w * 32; p < pend; ++p) {
+            const float sh = (bsh & 1u) ? 1.f : 0.f;
+            const float vs = (bvs & 1u) ? 1.f : 0.f;
+            const float vb = (bv1 & 1u) ? 1.f : ((bv2 & 1u) ? 2.f : 0.f);
+            bsh >>= 1; bvs >>= 1; bv1 >>= 1; bv2 >>= 1;
+            const float ster = sp->ster[p], calt = sp->cos_alt[p], salt = sp->sin_alt[p];
             const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
-            Kref_veg = Kref_veg + shaded_k * tvs * ster * calt;
             const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
-            const float x = hsvf + sp->yi_k[p];
-            const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
-            const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
-            Kref_sun = Kref_sun + sunlit_k * sunlit * tsh * ster * calt;
-            Kref_sh = Kref_sh + shaded_k * shaded * tsh * ster * calt;
+            if (day) {
+                const float diffsh = sh - (1.f - vs) * 0.97f;
+                aniLum = aniLum + diffsh * sp->lv[p];
+                KsideD = KsideD + diffsh * sp->lumchi[p] * sp->angl[p] * ster;
+                Kref_veg = Kref_veg + shaded_k * tvs * ster * calt;
+                const float x = hsvf + sp->yi_k[p];
+                const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
+                const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
+                Kref_sun = Kref_sun + sunlit_k * sunlit * tsh * ster * calt;
+                Kref_sh = Kref_sh + shaded_k * shaded * tsh * ster * calt;
+            }
+            const float tsky = (sh == 1.f && vs == 1.f) ? 1.f : 0.f;
+            Ldown_sky = Ldown_sky + tsky * sp->lsky_down[p];
+            Lside_sky = Lside_sky + tsky * sp->lsky_side[p];
+            Lside_veg = Lside_veg + veg_l * ster * calt * tvs;
+            Ldown_veg = Ldown_veg + veg_l * ster * salt * tvs;
+            if (sp->l_sun[p]) {
+                const float x = hsvf + sp->yi_l[p];
+                const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
+                const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
+                Lside_sun = Lside_sun + sunlit_l * sunlit * ster * calt * tsh;
+                Lside_sh = Lside_sh + shaded_l * shaded * ster * calt * tsh;
+                Ldown_sun = Ldown_sun + sunlit_l * sunlit * ster * salt * tsh;
+                Ldown_sh = Ldown_sh + shaded_l * shaded * ster * salt * tsh;
+            } else {
+                Lside_sh = Lside_sh + shaded_l * ster * calt * tsh;
+                Ldown_sh = Ldown_sh + shaded_l * ster * salt * tsh;
+            }
         }
+    }
+    if (day) {
         const float dRad = aniLum * sp->radD;
         const float sb = 1.f - svfbuveg[t];
         const float mix = sp->radG * (1.f - F_sh) + sp->radD * F_sh;
@@ -752,47 +904,24 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         KupN = galbN * sp->radI * sp->sinalt + kdiff * gnoshN;
         const float KsideI = shadow * sp->radI * sp->cosalt_k;
         Kside = KsideI + KsideD + Kref_sun + Kref_sh + Kref_veg;
-    } else {
-        const float x = sp->ta + 273.15f;
-        Lup = se_t * pow4(x);
-        LupE = LupS = LupW = LupN = Lup;
     }
-    // ---- longwave: define_patch_characteristics, two passes ----
-    float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f, Ldown_ref = 0.f;
-    float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f, Lside_ref = 0.f;
-    const float veg_l = sp->veg_l, sunlit_l = sp->sunlit_l, shaded_l = sp->shaded_l;
-    for (int p = 0; p < SW_NPATCH; ++p) {
-        float sh, vs, vb;
-        mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
-        const float ster = sp->ster[p], calt = sp->cos_alt[p], salt = sp->sin_alt[p];
-        const float tsky = (sh == 1.f && vs == 1.f) ? 1.f : 0.f;
-        Ldown_sky = Ldown_sky + tsky * sp->lsky_down[p];
-        Lside_sky = Lside_sky + tsky * sp->lsky_side[p];
-        const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
-        Lside_veg = Lside_veg + veg_l * ster * calt * tvs;
-        Ldown_veg = Ldown_veg + veg_l * ster * salt * tvs;
-        const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
-        if (sp->l_sun[p]) {
-            const float x = hsvf + sp->yi_l[p];
-            const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
-            const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
-            Lside_sun = Lside_sun + sunlit_l * sunlit * ster * calt * tsh;
-            Lside_sh = Lside_sh + shaded_l * shaded * ster * calt * tsh;
-            Ldown_sun = Ldown_sun + sunlit_l * sunlit * ster * salt * tsh;
-            Ldown_sh = Ldown_sh + shaded_l * shaded * ster * salt * tsh;
-        } else {
-            Lside_sh = Lside_sh + shaded_l * ster * calt * tsh;
-            Ldown_sh = Ldown_sh + shaded_l * ster * salt * tsh;
-        }
-    }
+    // ---- pass B: reflected longwave ----
     const float pi_f = 3.14159265358979323846f;
     const float refl = ((Ldown_sky + Lup) * (1.f - md->ewall) * 0.5f) / pi_f;
-    for (int p = 0; p < SW_NPATCH; ++p) {
-        float sh, vs, vb;
-        mask_bits(msh, mvs, mv1, mv2, p, &sh, &vs, &vb);
-        const float tr = (sh == 0.f || vs == 0.f || vb == 0.f) ? 1.f : 0.f;
-        Lside_ref = Lside_ref + refl * sp->ster[p] * sp->cos_alt[p] * tr;
-        Ldown_ref = Ldown_ref + refl * sp->ster[p] * sp->sin_alt[p] * tr;
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        // reflected term uses patches where any of sh, vegsh, vbsh is 0
+        uint32_t tr = ~((w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4]) &
+                        (w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4]) &
+                        ((w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4]) |
+                         (w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4])));
+        const int pend = min(w * 32 + 32, SW_NPATCH);
+#pragma unroll 4
+        for (int p = w * 32; p < pend; ++p) {
+            const float trf = (tr & 1u) ? 1.f : 0.f;
+            tr >>= 1;
+            Lside_ref = Lside_ref + refl * sp->ster[p] * sp->cos_alt[p] * trf;
+            Ldown_ref = Ldown_ref + refl * sp->ster[p] * sp->sin_alt[p] * trf;
+        }
     }
     const float Lside = Lside_sky + Lside_veg + Lside_sh + Lside_sun + Lside_ref;
     const float Ldown = Ldown_sky + Ldown_veg + Ldown_sh + Ldown_sun + Ldown_ref;
@@ -934,16 +1063,19 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
     return hip_check(hipGetLastError(), "k_compose");
 }
 
-int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *dx, const int32_t *dy,
-                 const float *dz, uint32_t *masks, const uint32_t *stats, int rows, int cols, void *stream) {
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, uint32_t *masks,
+                 const uint32_t *stats, int rows, int cols, void *stream) {
     int rc = check_shape(rows, cols, "sw_svf_march");
     if (rc) return rc;
-    if (!hpack || !offs || !dx || !dy || !dz || !masks || !stats)
+    if (!hpack || !offs || !tab || !counts || !masks || !stats)
         return set_err(SW_E_ARG, "sw_svf_march: NULL pointer");
     const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_svf_counts, dim3(1), dim3(256), 0, S(stream), (const int4 *)tab, offs, stats, counts);
+    rc = hip_check(hipGetLastError(), "k_svf_counts");
+    if (rc) return rc;
     dim3 grid(grid_xcd(n), SW_MASK_WORDS);
-    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float4 *)hpack, offs, dx, dy, dz,
-                       masks, stats, rows, cols);
+    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float4 *)hpack, offs, (const int4 *)tab,
+                       counts, masks, stats, rows, cols);
     return hip_check(hipGetLastError(), "k_svf_march");
 }
 
@@ -1030,16 +1162,24 @@ int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad, 
 int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg, const float *tan_asvf,
                  const float *tan_svfalfa, const uint32_t *walkface, const float *buildings, const uint8_t *cls,
                  const sw_step_t *steps, const sw_model_t *model, float *state, float *out, int out_mask,
-                 int step, int rows, int cols, void *stream) {
+                 int step, int walk_halo, int rows, int cols, void *stream) {
     int rc = check_shape(rows, cols, "sw_flux_step");
     if (rc) return rc;
     if (!masks || !src || !svfbuveg || !tan_asvf || !tan_svfalfa || !walkface || !buildings || !cls || !steps ||
         !model || !state || !out)
         return set_err(SW_E_ARG, "sw_flux_step: NULL pointer");
-    const long n = (long)rows * cols;
-    hipLaunchKernelGGL(k_flux_step, dim3(grid_xcd(n)), dim3(256), 0, S(stream), masks, (const uint2 *)src,
+    if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_flux_step: walk halo out of range");
+    const int tiles = ((cols + FLUX_TX - 1) / FLUX_TX) * ((rows + FLUX_TY - 1) / FLUX_TY);
+    const int grid = ((tiles + 7) / 8) * 8;
+    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    if (lds > 65536) {
+        rc = hip_check(hipFuncSetAttribute((const void *)k_flux_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds), "hipFuncSetAttribute");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_flux_step, dim3(grid), dim3(256), lds, S(stream), masks, (const uint2 *)src,
                        svfbuveg, tan_asvf, tan_svfalfa, walkface, buildings, cls, steps, model, state, out,
-                       out_mask, step, rows, cols);
+                       out_mask, step, walk_halo, rows, cols);
     return hip_check(hipGetLastError(), "k_flux_step");
 }
 

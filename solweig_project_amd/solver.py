@@ -185,6 +185,9 @@ def model_struct(scale: float, lcc: M.LandCover) -> L.ModelStruct:
     m.absK, m.absL, m.Fside, m.Fup, m.Fcyl = M.ABSK, M.ABSL, M.FSIDE, M.FUP, M.FCYL
     m.albedo_b, m.ewall, m.sbc = M.ALBEDO_B, M.EWALL, M.SBC_PY
     m.first, m.second, m.nsec = wt.first, wt.second, nsec
+    m.walk_halo = int(max(np.abs(wt.dx).max(), np.abs(wt.dy).max()))
+    if m.walk_halo > 48:
+        raise UnsupportedInput(f"ground-view walk halo {m.walk_halo} px exceeds 48 (pixel too small)")
     _fill(m.alb_class, lcc.alb)
     _fill(m.emis_class, lcc.emis)
     dx = np.zeros((L.WALK_AZ, L.WALK_MAX), np.int32)
@@ -208,6 +211,7 @@ class DayPlan:
     sun_tab: np.ndarray        # int32 (S, 4)
     model: np.ndarray          # uint8 bytes of sw_model_t
     n_steps: int
+    walk_halo: int
     is_day: list
     params: list = field(default_factory=list)
 
@@ -238,8 +242,10 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
             structs.append(step_struct(sp, 0))
     steps = np.concatenate([L.struct_bytes(s) for s in structs])
     sun_tab = np.concatenate(tabs) if tabs else np.zeros((1, 4), np.int32)
-    return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=L.struct_bytes(model_struct(tile.scale, lcc)),
-                   n_steps=len(structs), is_day=[p.is_day for p in params], params=params)
+    mstruct = model_struct(tile.scale, lcc)
+    return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=L.struct_bytes(mstruct),
+                   n_steps=len(structs), walk_halo=int(mstruct.walk_halo), is_day=[p.is_day for p in params],
+                   params=params)
 
 
 # ---------------------------------------------------------------------------
@@ -272,6 +278,7 @@ class DeviceScene:
         self.cls = torch.empty(n, dtype=torch.uint8, device=dev)
         self.aspect = torch.empty(n, dtype=torch.float32, device=dev)
         self.stats = torch.empty(4, dtype=torch.int32, device=dev)
+        self.counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
         self.masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
         self.fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
         self.svfbuveg = torch.empty(n, dtype=torch.float32, device=dev)
@@ -373,8 +380,9 @@ class HipSolver:
         key = (rows, cols, scale)
         if key not in self._tables:
             offs, dx, dy, dz = SV.svf_tables(self.vault, scale, None, rows, cols)
-            self._tables[key] = tuple(torch.from_numpy(np.ascontiguousarray(x)).to(self.device)
-                                      for x in (offs, dx, dy, dz))
+            tab = np.stack([dx, dy, dz.view(np.int32), dx * cols + dy], axis=1).astype(np.int32)
+            self._tables[key] = (torch.from_numpy(np.ascontiguousarray(offs)).to(self.device),
+                                 torch.from_numpy(np.ascontiguousarray(tab)).to(self.device))
         return self._tables[key]
 
     def compose(self, ds: DeviceScene, stream=None):
@@ -390,9 +398,9 @@ class HipSolver:
         sp = _stream_ptr(stream)
         P = L.ptr
         self.compose(ds, stream)
-        offs, dx, dy, dz = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
+        offs, tab = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
         with self.timer.span("svf_march", stream):
-            L.call("sw_svf_march", P(ds.hpack), P(offs), P(dx), P(dy), P(dz), P(ds.masks), P(ds.stats),
+            L.call("sw_svf_march", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.masks), P(ds.stats),
                    ds.rows, ds.cols, sp)
         c = self._fold_consts
         with self.timer.span("svf_fold", stream):
@@ -443,7 +451,7 @@ class HipSolver:
             with self.timer.span("flux_day" if plan.is_day[i] else "flux_night", stream):
                 L.call("sw_flux_step", P(ds.masks), P(ds.src), P(ds.svfbuveg), P(ds.tan_asvf), P(ds.tan_sa),
                        P(ds.walkface), P(ds.buildings), P(ds.cls), P(dplan["steps"]), P(dplan["model"]),
-                       P(ds.state), base + i * stride, mask, i, ds.rows, ds.cols, sp)
+                       P(ds.state), base + i * stride, mask, i, plan.walk_halo, ds.rows, ds.cols, sp)
         return out
 
     def output_names(self, outputs):
