@@ -48,6 +48,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PROFILE_KEYS = {"svf_march": "k_svf_march", "svf_fold": "k_svf_fold", "compose": "k_compose",
+                "flux_day": "k_flux_step [day]", "flux_night": "k_flux_step [night]", "sun_step": "k_sun_step"}
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_full_solve_1000x1000x24.md"
+
+
+def load_traffic(rows, cols, seed):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary ({} if the
+    summary is absent or was measured on another workload)."""
+    if (rows, cols, seed) != (1000, 1000, 20261203) or not TRAFFIC_FILE.exists():
+        return {}
+    text = TRAFFIC_FILE.read_text()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return {k: int(v) for k, v in json.loads(block)["bytes_per_launch"].items()}
+    except (IndexError, KeyError, ValueError):
+        return {}
+
+
 def sun_for_site(lat, lon, utc, year, doy):
     from solweig_project_amd import sunpos
     return sunpos.day_sun_table(lat, lon, utc, year, doy)
@@ -104,6 +122,7 @@ def main():
     timer = KernelTimer()
     solver.timer = timer
     log(f"timing {args.steps} steps")
+    solver.plan_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -111,6 +130,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solver.timer = _NoTimer()
+    plan_ms = sum(solver.plan_ms) / max(len(solver.plan_ms), 1)
     ms = (t1 - t0) / args.steps * 1e3
     value = rows * cols * T / (ms / 1e3) / 1e6
     ksum = timer.summary()
@@ -127,29 +147,37 @@ def main():
     host_value = rows * cols * T / (h1 - h0) / 1e6
     del f_host, o_host
 
-    # roofline of the dominant kernel (algorithmic bytes / measured launch time)
+    # rooflines: algorithmic bytes per launch / average launch time (HIP events on the launch stream)
     amax, hmax = ds.amax_hmax()
     offs, dx, dy, dz = SV.svf_tables(solver.vault, tile.scale, np.float32(amax), rows, cols)
     n = rows * cols
     pixel_steps = int(offs[-1]) * n
     alg = {
-        # 3 float32 heights per executed march step (reference semantics) + heights in + masks out
+        # 3 float32 heights per executed march step (the reference's per-step shifted-slice read)
+        # + heights in (16 B/px) + packed masks out (80 B/px)
         "svf_march": 12.0 * pixel_steps + (16 + 80) * n,
-        # 18 x 22 walk gathers of 6 float planes + per-pixel compulsory traffic
+        # 18 x 22 walk gathers of 6 float planes + ~220 B/px compulsory per step
         "flux_day": (24.0 * 18 * 22 + 220) * n,
         "flux_night": 220.0 * n,
+        # packed masks in, 16 float fields out
         "svf_fold": (80 + 64) * n,
         "compose": (24 + 16 + 4 + 4 + 4 + 1) * n,
     }
+    traffic = load_traffic(rows, cols, args.seed)
+    rooflines = {}
+    for name, (launches, tot) in ksum.items():
+        if name not in alg:
+            continue
+        avg_ms = tot / launches
+        ach = alg[name] / (avg_ms / 1e3) / 1e9
+        tb = traffic.get(PROFILE_KEYS.get(name, ""))
+        rooflines[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
+                           "avg_launch_ms": round(avg_ms, 4), "launches_per_solve": launches // args.steps,
+                           "algorithmic_bytes_per_launch": int(alg[name])}
+    # dominant kernel = largest total time per solve
     dom = max(ksum, key=lambda k: ksum[k][1])
-    launches, tot = ksum[dom]
-    avg_ms = tot / launches
-    roof = None
-    if dom in alg:
-        ach = alg[dom] / (avg_ms / 1e3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": int(alg[dom])}
+    roof = dict(kernel=dom, **rooflines[dom]) if dom in rooflines else None
     if args.calibrate_copy:
         from solweig_project_amd import _lib as L
         m = 16 * 1024 * 1024
@@ -186,9 +214,11 @@ def main():
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
                    "time_steps": T, "pixel_size_m": site.pixel_size},
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "value_host_to_host": round(host_value, 3),
         "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
+        "host_plan_ms": round(plan_ms, 3),
     }
     print(json.dumps(line), flush=True)
 

@@ -306,14 +306,32 @@ _PEREZ = np.array([
 ], dtype=np.float64)
 
 
+_PEREZ_CONSTS = None
+
+
+def _perez_consts():
+    """Coefficient columns and the step-invariant patch trigonometry of ``Perez_v3``."""
+    global _PEREZ_CONSTS
+    if _PEREZ_CONSTS is None:
+        coef = torch.tensor(_PEREZ.astype(np.float32))
+        cols = tuple(coef[:, 4 * k: 4 * k + 4] for k in range(5))
+        deg2rad = torch.tensor(np.pi / 180, device=_CPU)
+        skyalt, skyazi = skyvault.create_patches()
+        skyzen = (90 - skyalt) * deg2rad
+        skyalt = skyalt * deg2rad
+        sky = dict(cos_zen=torch.cos(skyzen), sin_alt=torch.sin(skyalt), cos_alt=torch.cos(skyalt),
+                   azi=skyazi * deg2rad)
+        _PEREZ_CONSTS = cols + (sky,)
+    return _PEREZ_CONSTS
+
+
 def perez_luminance(zen_deg: float, azimuth_deg: float, radD, radI, jday: float):
     """Relative Perez luminance of the 153 patches (``Perez_v3``, patchchoice 1).
 
     Coefficient table: Perez et al. (1993), Solar Energy 50(3) 235-245.
     Returns a (153,) float32 tensor normalised to sum 1.
     """
-    coef = torch.tensor(_PEREZ.astype(np.float32))
-    acoeff, bcoeff, ccoeff, dcoeff, ecoeff = (coef[:, 4 * k: 4 * k + 4] for k in range(5))
+    acoeff, bcoeff, ccoeff, dcoeff, ecoeff, sky = _perez_consts()
     deg2rad = torch.tensor(np.pi / 180, device=_CPU)
     altitude = 90 - zen_deg
     zen = torch.tensor(zen_deg) * deg2rad
@@ -351,13 +369,9 @@ def perez_luminance(zen_deg: float, azimuth_deg: float, radD, radI, jday: float)
         m_c = torch.exp(torch.pow(PerezBrightness * (ccoeff[k, 0] + ccoeff[k, 1] * zen), ccoeff[k, 2])) - 1
         m_d = -torch.exp(PerezBrightness * (dcoeff[k, 0] + dcoeff[k, 1] * zen)) + dcoeff[k, 2] + \
             PerezBrightness * dcoeff[k, 3] * PerezBrightness
-    skyalt, skyazi = skyvault.create_patches()
-    skyzen = (90 - skyalt) * deg2rad
-    skyalt = skyalt * deg2rad
-    skyazi = skyazi * deg2rad
-    cosSky = torch.sin(skyalt) * torch.sin(altitude) + \
-        torch.cos(altitude) * torch.cos(skyalt) * torch.cos(torch.abs(skyazi - azimuth))
-    lv = (1 + m_a * torch.exp(m_b / torch.cos(skyzen))) * \
+    cosSky = sky["sin_alt"] * torch.sin(altitude) + \
+        torch.cos(altitude) * sky["cos_alt"] * torch.cos(torch.abs(sky["azi"] - azimuth))
+    lv = (1 + m_a * torch.exp(m_b / sky["cos_zen"])) * \
         ((1 + m_c * torch.exp(m_d * torch.arccos(cosSky)) + m_e * cosSky * cosSky))
     lv = lv / torch.sum(lv)
     return lv
@@ -417,6 +431,16 @@ class PatchGeometry:
     dir_s: np.ndarray
     dir_w: np.ndarray
     dir_n: np.ndarray
+    # step-invariant tensors of the per-step chain (same dtypes as there)
+    t_ster: torch.Tensor = None        # f32 (153,)
+    t_alt: torch.Tensor = None         # f32 (153,)
+    t_azi: torch.Tensor = None         # f32 (153,)
+    t_azi64: torch.Tensor = None       # f64 (153,)
+    sin_alt_k: torch.Tensor = None     # sin(alt * f32(pi/180)), f32 (radTot terms)
+    band_idx: np.ndarray = None        # ring index of each patch (unique altitudes)
+    band_zterm: torch.Tensor = None    # 1.7 - 1/cos((90 - band alt) * pi/180), f32 (8,)
+    band_sin: torch.Tensor = None      # f32(sin(band alt * pi/180) in float64), (8,)
+    band_cos: torch.Tensor = None      # f32(cos(band alt * pi/180) in float64), (8,)
 
 
 _GEOM_CACHE: dict = {}
@@ -442,7 +466,17 @@ def _patch_geometry(round_trip: bool) -> PatchGeometry:
     cw = torch.stack([torch.cos((270 - azi[i]) * deg2rad) for i in range(alt.shape[0])])
     cn = torch.stack([torch.cos((0 - azi[i]) * deg2rad) for i in range(alt.shape[0])])
     a = azi.numpy()
+    bands = np.unique(alt.numpy())
+    skyzen = 90 - torch.tensor(bands)
+    band_sin = torch.stack([torch.sin(torch.tensor(b) * deg2rad) for b in bands]).float()
+    band_cos = torch.stack([torch.cos(torch.tensor(b) * deg2rad) for b in bands]).float()
+    t_alt = alt.clone()
     return PatchGeometry(
+        t_ster=ster.clone(), t_alt=t_alt, t_azi=azi.clone(), t_azi64=azi.double(),
+        sin_alt_k=torch.sin(t_alt * torch.tensor(torch.pi / 180.0)),
+        band_idx=np.searchsorted(bands, alt.numpy()),
+        band_zterm=1.7 - (1 / torch.cos(skyzen * deg2rad)),
+        band_sin=band_sin, band_cos=band_cos,
         alt=alt.numpy().astype(F32), azi=a.astype(F32), ster=ster.numpy().astype(F32),
         cos_alt=cos_alt.numpy().astype(F32), sin_alt=sin_alt.numpy().astype(F32),
         angl_inc=angl.numpy().astype(F32),
@@ -546,8 +580,7 @@ class DayLoop:
         if sp.is_day:
             zt = _t(zen.item())
             jt = _t(jday)
-            I0, CI, Kt = clearness_index(zt, jt, _t(Ta.item()), _t(RH.item()) / 100.0, _t(radG.item()), self.lat, _t(P.item()))
-            CI = min(CI, 1.0)
+            # (the reference evaluates this twice; the first result is overwritten)
             I0, CI, Kt = clearness_index(zt, jt, Ta.item(), _t(RH.item()) / 100.0, _t(radG.item()), self.lat, _t(P.item()))
             CI = min(CI, 1.0)
             radI, radD = diffuse_fraction(_t(radG.item()), _t(altitude.item()), Kt, _t(Ta.item()), _t(RH.item()))
@@ -622,18 +655,14 @@ class DayLoop:
             albedo = ALBEDO_B
             sp.sunlit_k = _f((albedo * (radI * torch.cos(alt_k * deg2rad_k)) + (radD * 0.5)) / torch.pi)
             sp.shaded_k = _f((albedo * radD * 0.5) / torch.pi)
-            ster_t = torch.tensor(geom.ster)
-            palt = torch.tensor(geom.alt)
-            # radTot accumulates in float32, patch by patch (solweig.py:696-704)
-            terms = (lv * ster_t * torch.sin(palt * deg2rad_k)).numpy()
-            radTot = np.float32(0.0)
-            for v in terms:
-                radTot = np.float32(radTot + v)
+            # radTot accumulates in float32, patch by patch (solweig.py:696-704);
+            # add.accumulate is that sequential float32 sum
+            terms = (lv * geom.t_ster * geom.sin_alt_k).numpy()
+            radTot = np.add.accumulate(terms, dtype=np.float32)[-1]
             lumChi = (lv * radD) / torch.tensor([radTot])
             sp.lumchi = lumChi.numpy().astype(F32)
             az_py = azimuth.item()
-            pazi = torch.tensor(geom.azi)
-            p2s = torch.abs(az_py - pazi)
+            p2s = torch.abs(az_py - geom.t_azi)
             xi = torch.cos(p2s * (torch.pi / 180.0))
             yi = 2 * xi * torch.tan(alt_k * (torch.pi / 180.0))
             sp.yi_k = torch.where(yi > 0, 0.0, yi).numpy().astype(F32)
@@ -653,21 +682,13 @@ class DayLoop:
         sp.diag["esky"] = float(esky)
         ewall_t = torch.tensor(EWALL)
         pi_t = torch.tensor(np.pi)
-        skyalt_np = np.unique(geom.alt)
-        skyzen = 90 - torch.tensor(skyalt_np)
-        esky_band = 1 - (1 - esky) * torch.exp(0.308 * (1.7 - (1 / torch.cos(skyzen * deg2rad_t))))
-        ster_t = torch.tensor(geom.ster)
-        palt = torch.tensor(geom.alt)
-        ldown = torch.zeros(153)
-        lside = torch.zeros(153)
-        for k, a in enumerate(skyalt_np):
-            sel = palt == float(a)
-            at = torch.tensor(a)
-            base = (esky_band[k] * SBC * ((Ta + 273.15) ** 4)) / pi_t
-            ldown[sel] = base * ster_t[sel] * torch.sin(at * deg2rad_t)
-            lside[sel] = base * ster_t[sel] * torch.cos(at * deg2rad_t)
-        sp.lsky_down = ldown.numpy().astype(F32)
-        sp.lsky_side = lside.numpy().astype(F32)
+        esky_band = 1 - (1 - esky) * torch.exp(0.308 * geom.band_zterm)
+        # per ring: (esky_band * SBC) in float32, then * (Ta + 273.15)^4 / pi in float64;
+        # per patch: float32(base) * ster * float32(sin or cos of the ring altitude)
+        base = ((esky_band * SBC).double() * ((Ta + 273.15) ** 4)) / pi_t
+        b32 = base.float()[geom.band_idx]
+        sp.lsky_down = ((b32 * geom.t_ster) * geom.band_sin[geom.band_idx]).numpy().astype(F32)
+        sp.lsky_side = ((b32 * geom.t_ster) * geom.band_cos[geom.band_idx]).numpy().astype(F32)
         veg_surf = (ewall_t * SBC * ((Ta + 273.15) ** 4)) / pi_t
         sp.veg_l = float(veg_surf)
         sp.shaded_l = float((ewall_t * SBC * ((Ta + 273.15) ** 4)) / pi_t)
@@ -675,8 +696,7 @@ class DayLoop:
             tgw = torch.tensor([[sp.tgwall]], dtype=torch.float32)
             sp.sunlit_l = float(((ewall_t * SBC * ((Ta + tgw + 273.15) ** 4)) / pi_t).reshape(-1)[0])
             # float64 like the reference: 0-dim float64 azimuth minus a float32 patch azimuth
-            pazi = torch.tensor(geom.azi).double()
-            azd = torch.abs(azimuth - pazi)
+            azd = torch.abs(azimuth - geom.t_azi64)
             lsun = ((azd > 90) & (azd < 270) & (altitude > 0)).numpy()
             xi = torch.cos(azd * (torch.pi / 180.0))
             yi = 2 * xi * torch.tan(altitude * (torch.pi / 180.0))

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -145,33 +146,45 @@ def _fill(arr, vals):
     arr[:] = [float(v) for v in vals]
 
 
+STEP_DTYPE = np.dtype(L.StepStruct)
+_SCALARS = ("ta", "ta_mat", "rh", "va", "tak4", "radI", "radD", "radG", "sinalt", "cosalt_k",
+            "tan_zen", "one_m_psi", "tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
+            "shaded_l", "veg_l", "tgwall", "lwall")
+_PATCH_ROWS = ("lv", "lumchi", "yi_k", "yi_l", "lsky_down", "lsky_side")
+
+
+def step_records(params, sun_offs) -> np.ndarray:
+    """``sw_step_t`` records (numpy structured array, one per step) of a day."""
+    rec = np.zeros(len(params), STEP_DTYPE)
+    geo_memo = {}
+    for i, (sp, off) in enumerate(zip(params, sun_offs)):
+        r = rec[i]
+        r["is_day"] = int(sp.is_day)
+        r["sun_off"] = off
+        r["sun_count"] = sp.table.count if (sp.is_day and sp.table is not None) else 0
+        r["facesh_branch"] = int(sp.facesh_branch)
+        r["azilow"], r["azihigh"] = sp.azilow, sp.azihigh
+        for k in _SCALARS:
+            r[k] = getattr(sp, k)
+        r["tw_first"] = int(sp.tw_first)
+        r["tw_update"] = int(sp.tw_update)
+        r["tg_class"] = sp.tg_class
+        for k in _PATCH_ROWS:
+            r[k] = getattr(sp, k)
+        g = sp.geom
+        if id(g) not in geo_memo:
+            lt, gt = patch_thresholds(np.asarray(g.alt, F32))
+            geo_memo[id(g)] = dict(alt=g.alt, ster=g.ster, cos_alt=g.cos_alt, sin_alt=g.sin_alt,
+                                   angl=g.angl_inc, sun_lt=lt, sun_gt=gt)
+        for k, v in geo_memo[id(g)].items():
+            r[k] = v
+        r["l_sun"] = sp.l_sun_side
+    return rec
+
+
 def step_struct(sp: M.StepParams, sun_off: int) -> L.StepStruct:
-    s = L.StepStruct()
-    s.is_day = int(sp.is_day)
-    s.sun_off = sun_off
-    s.sun_count = sp.table.count if (sp.is_day and sp.table is not None) else 0
-    s.facesh_branch = int(sp.facesh_branch)
-    s.azilow, s.azihigh = sp.azilow, sp.azihigh
-    for k in ("ta", "ta_mat", "rh", "va", "tak4", "radI", "radD", "radG", "sinalt", "cosalt_k",
-              "tan_zen", "one_m_psi", "tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
-              "shaded_l", "veg_l", "tgwall", "lwall"):
-        setattr(s, k, float(getattr(sp, k)))
-    s.tw_first = int(sp.tw_first)
-    s.tw_update = int(sp.tw_update)
-    _fill(s.tg_class, sp.tg_class)
-    for k in ("lv", "lumchi", "yi_k", "yi_l", "lsky_down", "lsky_side"):
-        _fill(getattr(s, k), getattr(sp, k))
-    g = sp.geom
-    _fill(s.alt, g.alt)
-    _fill(s.ster, g.ster)
-    _fill(s.cos_alt, g.cos_alt)
-    _fill(s.sin_alt, g.sin_alt)
-    _fill(s.angl, g.angl_inc)
-    lt, gt = patch_thresholds(np.asarray(g.alt, F32))
-    _fill(s.sun_lt, lt)
-    _fill(s.sun_gt, gt)
-    s.l_sun[:] = [int(v) for v in sp.l_sun_side]
-    return s
+    """One ``sw_step_t`` as a ctypes structure."""
+    return L.StepStruct.from_buffer_copy(step_records([sp], [sun_off]).tobytes())
 
 
 def model_struct(scale: float, lcc: M.LandCover) -> L.ModelStruct:
@@ -203,6 +216,16 @@ def model_struct(scale: float, lcc: M.LandCover) -> L.ModelStruct:
     return m
 
 
+_MODEL_CACHE: dict = {}
+
+
+def _model_struct_cached(scale: float, lcc: M.LandCover) -> L.ModelStruct:
+    key = (float(scale), bool(lcc.present))
+    if key not in _MODEL_CACHE:
+        _MODEL_CACHE[key] = model_struct(scale, lcc)
+    return _MODEL_CACHE[key]
+
+
 @dataclass
 class DayPlan:
     """All host-side inputs of one day, ready to upload."""
@@ -225,7 +248,7 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
     loop = M.DayLoop(met, sun, latitude=latitude, landcover=lcc, scale=tile.scale, amax=None,
                      rows=rows, cols=cols, state=state)
     stop = met.shape[0] if time_stop is None else time_stop
-    structs, tabs, params = [], [], []
+    tabs, params, offs = [], [], []
     off = 0
     for i in range(time_start, stop):
         sp = loop.step(i)
@@ -236,15 +259,15 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
             t = sp.table
             tab = np.stack([t.dx, t.dy, t.dz.view(np.int32), t.dzprev.view(np.int32)], axis=1).astype(np.int32)
             tabs.append(tab)
-            structs.append(step_struct(sp, off))
+            offs.append(off)
             off += tab.shape[0]
         else:
-            structs.append(step_struct(sp, 0))
-    steps = np.concatenate([L.struct_bytes(s) for s in structs])
+            offs.append(0)
+    steps = step_records(params, offs).view(np.uint8)
     sun_tab = np.concatenate(tabs) if tabs else np.zeros((1, 4), np.int32)
-    mstruct = model_struct(tile.scale, lcc)
+    mstruct = _model_struct_cached(tile.scale, lcc)
     return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=L.struct_bytes(mstruct),
-                   n_steps=len(structs), walk_halo=int(mstruct.walk_halo), is_day=[p.is_day for p in params],
+                   n_steps=len(params), walk_halo=int(mstruct.walk_halo), is_day=[p.is_day for p in params],
                    params=params)
 
 
@@ -370,6 +393,7 @@ class HipSolver:
             ("dirm", v.dir_mask))}
         self._tables = {}
         self.timer = _NoTimer()
+        self.plan_ms: list[float] = []      # host planning time of each solve
 
     # -- SVF ---------------------------------------------------------------
     def upload(self, tile: TileInputs) -> DeviceScene:
@@ -416,10 +440,13 @@ class HipSolver:
 
     # -- day ---------------------------------------------------------------
     def upload_plan(self, plan: DayPlan):
+        """Copy the day's tables to HBM from pinned staging, without a host sync."""
         dev = self.device
-        return dict(steps=torch.from_numpy(plan.steps).to(dev),
-                    sun_tab=torch.from_numpy(plan.sun_tab).to(dev),
-                    model=torch.from_numpy(plan.model.copy()).to(dev))
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
+
+        return dict(steps=up(plan.steps), sun_tab=up(plan.sun_tab), model=up(plan.model.copy()))
 
     def tile_static(self, ds: DeviceScene, dplan, stream=None):
         P = L.ptr
@@ -468,7 +495,10 @@ class HipSolver:
         while the GPU marches.
         """
         self.svf(ds, stream)
-        plan = plan_day(ds.tile, met, sun, latitude)
+        t0 = time.perf_counter()
+        with torch.inference_mode():
+            plan = plan_day(ds.tile, met, sun, latitude)
+        self.plan_ms.append((time.perf_counter() - t0) * 1e3)
         dplan = self.upload_plan(plan)
         self.tile_static(ds, dplan, stream)
         out = self.run_day(ds, plan, dplan, outputs, out=out, stream=stream)
