@@ -81,19 +81,60 @@ typedef struct sw_model {
     float walk_lo[SW_WALK_AZ], walk_hi[SW_WALK_AZ];
 } sw_model_t;
 
-/* ---- runtime ---------------------------------------------------------- */
+/* Device buffers of one tile's day loop (host struct of device pointers;
+ * see the entry points below for each buffer's layout). */
+typedef struct sw_tile {
+    int32_t rows, cols;
+    const void *hpack;
+    const float *walls, *aspect_rad, *buildings;
+    const uint8_t *cls;
+    const uint32_t *stats, *masks;
+    const float *svfbuveg, *tan_asvf, *tan_svfalfa;
+    const uint32_t *walkface;
+    uint32_t *src;            /* 2 x uint32 per pixel, rewritten each day step */
+    float *state;             /* 6 planes, carried across steps */
+} sw_tile_t;
+
+/* ---- runtime ------------------------------------------------------------
+ * Mirrors the reference native ABI's runtime surface (sw_api.cu:24-171,
+ * sw_residency.cu:53-331): queries, counted device allocation, pinned host
+ * memory, sync/async/2D copies, streams (default priority), events and
+ * stream capture into executable graphs. */
 int sw_abi_version(void);
 const char *sw_last_error(void);
 const char *sw_build_flags(void);
 int sw_device_count(int *count);
+int sw_device_name(int device, char *buf, int len);
+int sw_mem_info(int64_t *free_bytes, int64_t *total_bytes);
 int sw_sync(void *stream);
+int sw_alloc(void **ptr, int64_t bytes);
+int sw_free(void *ptr);
+int sw_alloc_stats(int64_t *live_bytes, int64_t *n_alloc, int64_t *n_free);
+int sw_host_alloc(void **ptr, int64_t bytes);
+int sw_host_free(void *ptr);
+int sw_copy(void *dst, const void *src, int64_t bytes);
+int sw_copy_async(void *dst, const void *src, int64_t bytes, void *stream);
+int sw_copy2d_async(void *dst, int64_t dpitch, const void *src, int64_t spitch, int64_t width_bytes,
+                    int64_t height, void *stream);
+int sw_fill_f32(float *dst, float value, int64_t n, void *stream);
+int sw_stream_create(void **stream);
+int sw_stream_destroy(void *stream);
+int sw_event_create(void **event);
+int sw_event_destroy(void *event);
+int sw_event_record(void *event, void *stream);
+int sw_event_sync(void *event);
+int sw_event_elapsed(void *start, void *end, float *ms);
+int sw_stream_wait_event(void *stream, void *event);
+int sw_graph_begin(void *stream);
+int sw_graph_end(void *stream, void **graph_exec);
+int sw_graph_launch(void *graph_exec, void *stream);
+int sw_graph_destroy(void *graph_exec);
 
 /* ---- SVF sweep ----------------------------------------------------------
  * Replaces shadow.svf_calculator (solweig_gpu/shadow.py:409-625) and the
  * per-patch shadow() march (shadow.py:166-322).
  * hpack: float4 per pixel {dsm, vegdsm, vegdsm2, 0} (sw_pack_heights).
- * Patch tables concatenated: offs[154], dx/dy/dz[offs[153]].
- * hmax: max over all three planes (enables the exact early exits).
+ * stats[1] (hmax, max over the three planes) enables the exact early exits.
  * masks: SW_MASK_PLANES*SW_MASK_WORDS*n uint32. */
 int sw_pack_heights(const float *a, const float *vegdsm, const float *vegdsm2, void *hpack,
                     int rows, int cols, void *stream);
@@ -166,6 +207,21 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
                  const float *buildings, const uint8_t *cls, const sw_step_t *steps,
                  const sw_model_t *model, float *state, float *out, int out_mask, int step,
                  int walk_halo, int rows, int cols, void *stream);
+
+/* ---- whole stages ---------------------------------------------------------
+ * SVF sweep = sw_svf_march + sw_svf_fold (svf_calculator, shadow.py:409-625). */
+int sw_svf_sweep(const void *hpack, const uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                 const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                 const int32_t *nann, const int32_t *dirm, int32_t *counts, uint32_t *masks, float *fields,
+                 int rows, int cols, void *stream);
+/* Day loop over steps [t0, t1) (the time loop of run_utci_window,
+ * utci_process.py:917-1049): per step the sun march when is_day[i - t0]
+ * (host array) and the flux step; outputs of step i at
+ * out + (i - t0) * popcount(out_mask) * n. reset_state clears the 6 state
+ * planes first (fill kernel, capture-safe). No host synchronisation. */
+int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                 const int32_t *sun_tab, const int32_t *is_day, int t0, int t1, float *out, int out_mask,
+                 int walk_halo, int reset_state, void *stream);
 
 /* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,

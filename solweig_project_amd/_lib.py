@@ -62,16 +62,55 @@ class ModelStruct(ctypes.Structure):
     ]
 
 
+class TileStruct(ctypes.Structure):
+    """Mirror of ``sw_tile_t`` (device pointers of one tile's day loop)."""
+
+    _fields_ = [
+        ("rows", _c_i32), ("cols", _c_i32),
+        ("hpack", ctypes.c_void_p), ("walls", ctypes.c_void_p), ("aspect_rad", ctypes.c_void_p),
+        ("buildings", ctypes.c_void_p), ("cls", ctypes.c_void_p), ("stats", ctypes.c_void_p),
+        ("masks", ctypes.c_void_p), ("svfbuveg", ctypes.c_void_p), ("tan_asvf", ctypes.c_void_p),
+        ("tan_svfalfa", ctypes.c_void_p), ("walkface", ctypes.c_void_p), ("src", ctypes.c_void_p),
+        ("state", ctypes.c_void_p),
+    ]
+
+
 #: exported symbols and their (restype, argtypes)
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _F = ctypes.c_float
+_I64 = ctypes.c_int64
 SIGNATURES = {
     "sw_abi_version": (_I, []),
     "sw_last_error": (ctypes.c_char_p, []),
     "sw_build_flags": (ctypes.c_char_p, []),
     "sw_device_count": (_I, [ctypes.POINTER(_I)]),
+    "sw_device_name": (_I, [_I, ctypes.c_char_p, _I]),
+    "sw_mem_info": (_I, [_VP, _VP]),
     "sw_sync": (_I, [_VP]),
+    "sw_alloc": (_I, [_VP, _I64]),
+    "sw_free": (_I, [_VP]),
+    "sw_alloc_stats": (_I, [_VP, _VP, _VP]),
+    "sw_host_alloc": (_I, [_VP, _I64]),
+    "sw_host_free": (_I, [_VP]),
+    "sw_copy": (_I, [_VP, _VP, _I64]),
+    "sw_copy_async": (_I, [_VP, _VP, _I64, _VP]),
+    "sw_copy2d_async": (_I, [_VP, _I64, _VP, _I64, _I64, _I64, _VP]),
+    "sw_fill_f32": (_I, [_VP, _F, _I64, _VP]),
+    "sw_stream_create": (_I, [_VP]),
+    "sw_stream_destroy": (_I, [_VP]),
+    "sw_event_create": (_I, [_VP]),
+    "sw_event_destroy": (_I, [_VP]),
+    "sw_event_record": (_I, [_VP, _VP]),
+    "sw_event_sync": (_I, [_VP]),
+    "sw_event_elapsed": (_I, [_VP, _VP, _VP]),
+    "sw_stream_wait_event": (_I, [_VP, _VP]),
+    "sw_graph_begin": (_I, [_VP]),
+    "sw_graph_end": (_I, [_VP, _VP]),
+    "sw_graph_launch": (_I, [_VP, _VP]),
+    "sw_graph_destroy": (_I, [_VP]),
+    "sw_svf_sweep": (_I, [_VP] * 13 + [_I, _I, _VP]),
+    "sw_solve_day": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP]),
     "sw_pack_heights": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_compose": (_I, [_VP] * 11 + [_I, _I, _VP]),
     "sw_svf_march": (_I, [_VP] * 6 + [_I, _I, _VP]),

@@ -19,18 +19,18 @@
 #include <stdio.h>
 #include <string.h>
 
-#include "../../include/solweig_hip.h"
+#include "sw_internal.h"
 
 #define SW_BUILD_FLAGS "hipcc -O3 --offload-arch=gfx950 -ffp-contract=off -fno-fast-math -packed-fp32-ops(off)"
 
 static thread_local char g_err[512];
 
-static int set_err(int code, const char *msg) {
+int swi_set_err(int code, const char *msg) {
     snprintf(g_err, sizeof(g_err), "%s", msg);
     return code;
 }
 
-static int hip_check(hipError_t e, const char *where) {
+int swi_hip_check(hipError_t e, const char *where) {
     if (e != hipSuccess) {
         snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
         return SW_E_HIP;
@@ -38,7 +38,9 @@ static int hip_check(hipError_t e, const char *where) {
     return SW_OK;
 }
 
-static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+#define set_err swi_set_err
+#define hip_check swi_hip_check
+#define S swi_stream
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -1027,7 +1029,7 @@ int sw_sync(void *stream) {
     return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
-static int check_shape(int rows, int cols, const char *who) {
+int swi_check_shape(int rows, int cols, const char *who) {
     if (rows <= 0 || cols <= 0 || (long)rows * cols > (1L << 31) - 1) {
         char buf[160];
         snprintf(buf, sizeof(buf), "%s: invalid grid %d x %d", who, rows, cols);
@@ -1035,6 +1037,8 @@ static int check_shape(int rows, int cols, const char *who) {
     }
     return SW_OK;
 }
+
+#define check_shape swi_check_shape
 
 int sw_pack_heights(const float *a, const float *v1, const float *v2, void *hpack, int rows, int cols,
                     void *stream) {

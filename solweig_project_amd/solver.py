@@ -315,6 +315,13 @@ class DeviceScene:
     def walls(self):
         return self.inputs["walls"]
 
+    def tile_struct(self) -> L.TileStruct:
+        """``sw_tile_t`` of this scene's device buffers."""
+        P = L.ptr
+        return L.TileStruct(self.rows, self.cols, P(self.hpack), P(self.walls), P(self.aspect), P(self.buildings),
+                            P(self.cls), P(self.stats), P(self.masks), P(self.svfbuveg), P(self.tan_asvf),
+                            P(self.tan_sa), P(self.walkface), P(self.src), P(self.state))
+
     def check_flags(self):
         """Refuse the solve if compose flagged inputs outside the supported domain (syncs)."""
         flags = int(self.stats[2].item()) & 0xFFFFFFFF
@@ -467,6 +474,13 @@ class HipSolver:
         P = L.ptr
         if reset_state:
             ds.state.zero_()
+        if isinstance(self.timer, _NoTimer):
+            # one library call enqueues the whole day
+            is_day = (ctypes.c_int32 * T)(*[int(x) for x in plan.is_day])
+            L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
+                   P(dplan["sun_tab"]), is_day, 0, T, P(out), mask, plan.walk_halo, 0, sp)
+            return out
+        # per-launch path (kernel timing spans around each launch)
         stride = k * ds.n * 4
         base = P(out)
         for i in range(T):
