@@ -166,6 +166,20 @@ int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso,
 int sw_unpack_masks(const uint32_t *masks, float *shmat, float *vegshmat, float *vbshmat,
                     int rows, int cols, void *stream);
 
+/* Pack the reference's float cubes shmat, vegshmat, vbshvegshmat
+ * (rows, cols, 153; the cubes of svf_calculator's return tuple,
+ * shadow.py:625) into the mask planes. flags[0] gets bit 1 when a value is
+ * outside {0, 1} ({0, 1, 2} for vbsh); the caller refuses such a bundle. */
+int sw_pack_masks(const float *shmat, const float *vegshmat, const float *vbshmat, uint32_t *masks,
+                  uint32_t *flags, int rows, int cols, void *stream);
+/* Tile statistics for marches on caller-packed heights (the svf_calculator
+ * entry, where the caller supplies amaxvalue): stats = {key(amax),
+ * key(max of the three planes), flags (1: non-finite height), 0}. */
+int sw_height_stats(const void *hpack, float amax, uint32_t *stats, int rows, int cols, void *stream);
+/* Replace stats[0] by key(amax): a caller-given domain maximum
+ * (run_utci_window's amaxvalue, utci_process.py:675-676). */
+int sw_set_amax(uint32_t *stats, float amax, void *stream);
+
 /* Single-patch marches with float outputs, the drop-in form of the
  * reference native ABI (sw_api.cu:209-228). */
 int sw_run_march_svf_shadow(const float *a, const float *vegdem, const float *vegdem2,

@@ -116,6 +116,9 @@ SIGNATURES = {
     "sw_svf_march": (_I, [_VP] * 6 + [_I, _I, _VP]),
     "sw_svf_fold": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_unpack_masks": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
+    "sw_pack_masks": (_I, [_VP] * 5 + [_I, _I, _VP]),
+    "sw_height_stats": (_I, [_VP, _F, _VP, _I, _I, _VP]),
+    "sw_set_amax": (_I, [_VP, _F, _VP]),
     "sw_run_march_svf_shadow": (_I, [_VP] * 9 + [_I, _I, _I, _VP]),
     "sw_run_march_wallheight23": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_tile_static": (_I, [_VP] * 8 + [_I, _I, _VP]),

@@ -1,0 +1,351 @@
+"""Drop-in mirrors of the reference's in-memory run API, executed on the GPU.
+
+``svf_calculator`` mirrors ``solweig_gpu/shadow.py:409-625`` (same
+arguments, same 19-tuple order) and ``run_utci_window`` mirrors
+``solweig_gpu/utci_process.py:477-1090`` (same keyword arguments, same
+validation errors, same ``{variable: (T, rows, cols) float32}`` result and
+the same warm-start state dictionary). Both run the HIP kernels through the
+C ABI; there is no CPU path.
+
+Differences a caller can observe, all refusals rather than silent changes:
+
+* only ``patch_option=2`` (153 patches), a zero bush layer, spatially
+  uniform wind (``windcoeff`` None or all ones) and default model
+  parameters are supported; anything else raises
+  :class:`~solweig_project_amd.solver.UnsupportedInput`;
+* ``save_rasters`` / ``save_wbgt`` (file output, WBGT) are not supported;
+* ``temp1`` is not modified in place (the reference clamps the caller's
+  tensor, ``utci_process.py:667``); the clamp is applied on the device.
+
+``svf_calculator`` returns an :class:`SvfBundle`: a 19-element sequence in
+the reference's order whose three cubes are expanded from the packed masks
+only when indexed. Passing the bundle itself to ``run_utci_window`` keeps
+the masks packed in HBM.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Any, Mapping, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import met as M
+from .solver import (OUT_BITS, DeviceScene, HipSolver, TileSpec, UnsupportedInput, _stream_ptr, plan_day)
+
+F32 = np.float32
+
+#: our field order on the device (index into the 16 field planes)
+FIELD_NAMES = ("svf", "svfE", "svfS", "svfW", "svfN", "svfveg", "svfEveg", "svfSveg", "svfWveg", "svfNveg",
+               "svfaveg", "svfEaveg", "svfSaveg", "svfWaveg", "svfNaveg", "svftotal")
+#: the reference's return order (shadow.py:625)
+REF_ORDER = ("svf", "svfaveg", "svfE", "svfEaveg", "svfEveg", "svfN", "svfNaveg", "svfNveg", "svfS", "svfSaveg",
+             "svfSveg", "svfveg", "svfW", "svfWaveg", "svfWveg", "vegshmat", "vbshvegshmat", "shmat", "svftotal")
+_CUBES = ("vegshmat", "vbshvegshmat", "shmat")
+
+VALID_VARIABLES = frozenset({"utci", "tmrt", "kup", "kdown", "lup", "ldown", "shadow", "ta", "wind"})
+#: accepted override names (utci_process.py:78-82) and the defaults this backend reproduces
+MODEL_PARAMETER_DEFAULTS = {
+    "albedo_b": 0.2, "ewall": 0.9, "absK": 0.7, "absL": 0.95, "Fside": 0.22, "Fup": 0.06, "Fcyl": 0.28,
+    "cyl": True, "elvis": 0, "anisotropic_sky": 1, "transVeg": 3.0 / 100.0, "height": 1.1,
+    "firstdayleaf": 97, "lastdayleaf": 300,
+}
+STATE_PLANES = ("Tgmap1", "Tgmap1E", "Tgmap1S", "Tgmap1W", "Tgmap1N", "TgOut1")
+
+_SOLVERS: dict = {}
+
+
+def _solver(device) -> HipSolver:
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise UnsupportedInput(f"device {dev} is not a ROCm GPU; this backend has no CPU path")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    if dev not in _SOLVERS:
+        _SOLVERS[dev] = HipSolver(dev)
+    return _SOLVERS[dev]
+
+
+def _plane(x, device, name) -> torch.Tensor:
+    """A contiguous float32 device copy (or view) of a 2-D raster."""
+    if x is None:
+        raise ValueError(f"{name} is required")
+    t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.asarray(x))
+    if t.ndim != 2:
+        raise ValueError(f"{name} must be 2-D, got shape {tuple(t.shape)}")
+    return t.to(device=device, dtype=torch.float32).contiguous()
+
+
+def _scalar(x) -> float:
+    if isinstance(x, torch.Tensor):
+        return float(x.detach().cpu().reshape(-1)[0])
+    return float(np.asarray(x).reshape(-1)[0])
+
+
+# ---------------------------------------------------------------------------
+# SVF
+# ---------------------------------------------------------------------------
+@dataclass
+class SvfBundle(Sequence):
+    """Device-resident SVF result: packed masks (20, n) int32 and fields (16, n)."""
+
+    rows: int
+    cols: int
+    masks: torch.Tensor
+    fields: torch.Tensor
+    solver: HipSolver
+
+    def field(self, name: str) -> torch.Tensor:
+        return self.fields[FIELD_NAMES.index(name)].reshape(self.rows, self.cols)
+
+    def cubes(self):
+        """(vegshmat, vbshvegshmat, shmat) float32 (rows, cols, 153), expanded on the device."""
+        sh, vs, vb = self.solver.unpack_masks(self)
+        return vs, vb, sh
+
+    def __len__(self) -> int:
+        return len(REF_ORDER)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return tuple(self[k] for k in range(*i.indices(len(self))))
+        name = REF_ORDER[i]
+        if name in _CUBES:
+            return self.cubes()[_CUBES.index(name)]
+        return self.field(name)
+
+
+def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=None, bush=None, scale=None,
+                   save_rasters=False, building_dsm_path=None, tree_path=None, dem_path=None, output_dir=None,
+                   number=None, gdal_dsm=None, *, device="cuda"):
+    """153-patch sky-view factors of a DSM with vegetation (``shadow.py:409-625``).
+
+    a: building DSM; vegdem / vegdem2: absolute canopy / trunk-zone heights
+    (0 where there is no vegetation); scale: pixels per metre. Returns an
+    :class:`SvfBundle` (19 elements in the reference's order).
+    """
+    if int(patch_option) != 2:
+        raise UnsupportedInput(f"patch_option {patch_option}: only 2 (153 patches) is implemented")
+    if save_rasters:
+        raise UnsupportedInput("save_rasters: raster output is not part of this backend")
+    if scale is None:
+        raise ValueError("scale is required")
+    if a is None or vegdem is None or vegdem2 is None:
+        raise ValueError("a, vegdem and vegdem2 are required")
+    solver = _solver(device)
+    dev = solver.device
+    a_t = _plane(a, dev, "a")
+    v1 = _plane(vegdem, dev, "vegdem")
+    v2 = _plane(vegdem2, dev, "vegdem2")
+    if v1.shape != a_t.shape or v2.shape != a_t.shape:
+        raise ValueError("a, vegdem and vegdem2 must have one shape")
+    if bush is not None and bool((_plane(bush, dev, "bush") != 0).any()):
+        raise UnsupportedInput("a non-zero bush layer is not supported")
+    rows, cols = a_t.shape
+    amax = float(torch.maximum(a_t.max(), v1.max())) if amaxvalue is None else _scalar(amaxvalue)
+    n = rows * cols
+    hpack = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    stats = torch.empty(4, dtype=torch.int32, device=dev)
+    counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
+    masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
+    fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
+    sp = _stream_ptr(None)
+    P = L.ptr
+    L.call("sw_pack_heights", P(a_t), P(v1), P(v2), P(hpack), rows, cols, sp)
+    L.call("sw_height_stats", P(hpack), F32(amax), P(stats), rows, cols, sp)
+    offs, tab = solver.svf_tables(rows, cols, float(scale))
+    c = solver._fold_consts
+    L.call("sw_svf_sweep", P(hpack), P(stats), P(offs), P(tab), P(v2), P(c["w_iso"]), P(c["w_aniso"]),
+           P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(counts), P(masks), P(fields), rows, cols, sp)
+    if int(stats[2].item()) & 1:
+        raise UnsupportedInput("non-finite height in a, vegdem or vegdem2")
+    return SvfBundle(rows, cols, masks, fields, solver)
+
+
+def _bundle_to_device(bundle, rows, cols, solver, out_window, cubes_cropped):
+    """(masks (20, n), fields (16, n)) on the device from an SvfBundle or a reference 19-tuple."""
+    dev = solver.device
+    n = rows * cols
+    if isinstance(bundle, SvfBundle):
+        if (bundle.rows, bundle.cols) != (rows, cols):
+            raise ValueError(f"svf_bundle is {bundle.rows}x{bundle.cols}, rasters are {rows}x{cols}")
+        if cubes_cropped:
+            raise ValueError("svf_bundle_cubes_cropped applies to cube tuples, not to a packed bundle")
+        return bundle.masks, bundle.fields
+    items = list(bundle)
+    if len(items) != len(REF_ORDER):
+        raise ValueError(f"svf_bundle must have {len(REF_ORDER)} elements, got {len(items)}")
+    named = dict(zip(REF_ORDER, items))
+    fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
+    for k, nm in enumerate(FIELD_NAMES):
+        fields[k] = _plane(named[nm], dev, nm).reshape(-1)
+    cube = [named[c] for c in ("shmat", "vegshmat", "vbshvegshmat")]
+    cube = [(c if isinstance(c, torch.Tensor) else torch.from_numpy(np.asarray(c)))
+            .to(device=dev, dtype=torch.float32).contiguous() for c in cube]
+    if cubes_cropped:
+        r0, r1, c0, c1 = out_window
+        shape = (r1 - r0, c1 - c0, L.NPATCH)
+    else:
+        shape = (rows, cols, L.NPATCH)
+    if any(tuple(c.shape) != shape for c in cube):
+        raise ValueError(f"svf_bundle cubes must have shape {shape}")
+    wr, wc = shape[0], shape[1]
+    packed = torch.empty((L.MASK_PLANES * L.MASK_WORDS, wr * wc), dtype=torch.int32, device=dev)
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    L.call("sw_pack_masks", *[L.ptr(c) for c in cube], L.ptr(packed), L.ptr(flags), wr, wc, _stream_ptr(None))
+    if int(flags.item()):
+        raise UnsupportedInput("svf_bundle cubes hold values other than 0/1 (0/1/2 for vbshvegshmat)")
+    if not cubes_cropped:
+        return packed, fields
+    # cubes cover the write window only: the masks outside it never reach a
+    # written pixel (every per-cell term is local; the walk reads sun-march
+    # sources, not masks)
+    masks = torch.zeros((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
+    r0, r1, c0, c1 = out_window
+    masks.view(-1, rows, cols)[:, r0:r1, c0:c1] = packed.view(-1, wr, wc)
+    return masks, fields
+
+
+# ---------------------------------------------------------------------------
+# time loop
+# ---------------------------------------------------------------------------
+def _check_model_parameters(model_parameters):
+    if model_parameters is None:
+        return
+    mp = dict(model_parameters)
+    unknown = sorted(set(mp) - set(MODEL_PARAMETER_DEFAULTS))
+    if unknown:
+        raise ValueError(f"unknown model parameters {unknown}; accepted names are "
+                         f"{sorted(MODEL_PARAMETER_DEFAULTS)}")
+    changed = sorted(k for k, v in mp.items() if float(v) != float(MODEL_PARAMETER_DEFAULTS[k]))
+    if changed:
+        raise UnsupportedInput(f"model parameter overrides {changed} are not supported (defaults only)")
+
+
+def _sun_rows(x) -> np.ndarray:
+    a = x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+    return np.asarray(a, dtype=np.float64).reshape(-1)
+
+
+def run_utci_window(*, a, temp1, temp2, walls, dirwalls, svf_bundle, met_file, altitude, azimuth, zen, jday,
+                    dectime, altmax, location, scale, amaxvalue=None, landcover_grid=None, lc_class=None,
+                    windcoeff=None, windcoeff_by_dir=None, time_start=0, time_stop=None,
+                    requested_variables=("utci",), save_wbgt=False, out_window=None, model_parameters=None,
+                    svf_bundle_cubes_cropped=False, initial_state=None, return_final_state=False,
+                    device="cuda"):
+    """SOLWEIG time loop over one raster window (``utci_process.py:477-1090``) on the GPU.
+
+    Returns ``{variable: ndarray (time_stop - time_start, rows, cols) float32}``
+    (cropped to ``out_window`` when given), or ``(outputs, final_state)``
+    with ``return_final_state``.
+    """
+    met_file = np.asarray(met_file, dtype=np.float64)
+    requested = set(requested_variables) | {"utci"}
+    if not requested.issubset(VALID_VARIABLES):
+        raise ValueError(f"unknown requested variables: {sorted(requested - VALID_VARIABLES)}")
+    if time_stop is None:
+        time_stop = met_file.shape[0]
+    if not 0 <= time_start < time_stop <= met_file.shape[0]:
+        raise ValueError(f"invalid time range [{time_start}, {time_stop}) for {met_file.shape[0]} forcing rows")
+    if a is None or len(tuple(a.shape)) != 2:
+        raise ValueError("a must be a 2-D raster")
+    rows, cols = (int(v) for v in tuple(a.shape))
+    if out_window is not None:
+        r0, r1, c0, c1 = (int(v) for v in out_window)
+        if not (0 <= r0 < r1 <= rows and 0 <= c0 < c1 <= cols):
+            raise ValueError(f"out_window {tuple(out_window)} outside tensor shape {(rows, cols)}")
+    elif svf_bundle_cubes_cropped:
+        raise ValueError("svf_bundle_cubes_cropped=True requires out_window: the pre-cropped cube extent must "
+                         "match a declared write window")
+    _check_model_parameters(model_parameters)
+    if save_wbgt:
+        raise UnsupportedInput("save_wbgt: the WBGT diagnostic is not implemented in this backend")
+    if windcoeff_by_dir is not None:
+        raise UnsupportedInput("windcoeff_by_dir: direction-dependent wind fields are not supported")
+    if initial_state is not None and int(initial_state.get("next_step", time_start)) != time_start:
+        raise ValueError(f"initial_state was captured at step {initial_state.get('next_step')}, "
+                         f"not at time_start {time_start}")
+    solver = _solver(device)
+    dev = solver.device
+    a_t = _plane(a, dev, "a")
+    if windcoeff is not None and not bool((_plane(windcoeff, dev, "windcoeff") == 1).all()):
+        raise UnsupportedInput("windcoeff: only a uniform coefficient of 1 is supported")
+
+    planes = {"dsm": a_t, "dem": _plane(temp2, dev, "temp2"), "cdsm": _plane(temp1, dev, "temp1"),
+              "walls": _plane(walls, dev, "walls"), "aspect_deg": _plane(dirwalls, dev, "dirwalls")}
+    for k, v in planes.items():
+        if v.shape != a_t.shape:
+            raise ValueError(f"{k} has shape {tuple(v.shape)}, expected {(rows, cols)}")
+    lc_t = None
+    if landcover_grid is not None:
+        lc_t = _plane(landcover_grid, dev, "landcover_grid")
+        if lc_t.shape != a_t.shape:
+            raise ValueError("landcover_grid shape differs from a")
+    planes["landcover"] = lc_t
+    lc_table = None if lc_class is None else np.asarray(lc_class, dtype=np.float64)
+    spec = TileSpec(rows, cols, float(scale), lc_t is not None, lc_table)
+    if lc_t is not None and lc_table is not None:
+        lcc = M.landcover_constants(True, lc_table)
+        clean = lc_t.clone()
+        clean[(clean < 1) | (clean > 7)] = 6
+        clean[(clean == 3) | (clean == 4)] = 5
+        present = {int(v) for v in torch.unique(clean).tolist()}
+        missing = sorted(present - set(lcc.classes))
+        if missing:
+            raise UnsupportedInput(f"land-cover classes {missing} are missing from lc_class")
+
+    ds = DeviceScene(spec, dev, planes)
+    solver.compose(ds)
+    masks, fields = _bundle_to_device(svf_bundle, rows, cols, solver, out_window, svf_bundle_cubes_cropped)
+    ds.masks.copy_(masks)
+    ds.fields.copy_(fields)
+    sp = _stream_ptr(None)
+    if amaxvalue is not None:
+        L.call("sw_set_amax", L.ptr(ds.stats), F32(_scalar(amaxvalue)), sp)
+
+    state = M.DayState()
+    ds.state.zero_()
+    if initial_state is not None:
+        state.CI = float(initial_state["CI"])
+        state.firstdaytime = float(initial_state["firstdaytime"])
+        state.timeadd = float(initial_state["timeadd"])
+        tw = initial_state.get("Twater")
+        state.Twater = None if tw is None else float(tw)
+        st = ds.state.view(6, rows, cols)
+        for k, nm in enumerate(STATE_PLANES):
+            pl = _plane(initial_state[nm], dev, nm)
+            if out_window is not None:
+                st[k, r0:r1, c0:c1] = pl
+            else:
+                st[k] = pl
+    sun = M.SunTable(_sun_rows(altitude), _sun_rows(azimuth), _sun_rows(zen), _sun_rows(jday),
+                     np.asarray(dectime, dtype=np.float64).reshape(-1), _sun_rows(altmax))
+    lat = float(location["latitude"])
+    plan = plan_day(spec, met_file, sun, lat, state=state, time_start=time_start, time_stop=time_stop)
+    dplan = solver.upload_plan(plan)
+    solver.tile_static(ds, dplan)
+    kernel_vars = tuple(v for v in ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup") if v in requested)
+    out = solver.run_day(ds, plan, dplan, kernel_vars, reset_state=False)
+    torch.cuda.synchronize(dev)
+    ds.check_flags()
+    names = solver.output_names(kernel_vars)
+    sl = (slice(None), slice(r0, r1), slice(c0, c1)) if out_window is not None else (slice(None),) * 3
+    host = out.cpu().numpy()
+    res = {}
+    for j, nm in enumerate(names):
+        res[nm] = np.ascontiguousarray(host[:, j][sl])
+    ow, oc = (r1 - r0, c1 - c0) if out_window is not None else (rows, cols)
+    if "ta" in requested:
+        res["ta"] = np.stack([np.full((ow, oc), p.ta_mat, F32) for p in plan.params])
+    if "wind" in requested:
+        res["wind"] = np.stack([np.full((ow, oc), p.va, F32) for p in plan.params])
+    if not return_final_state:
+        return res
+    st = ds.state.view(6, rows, cols)
+    final = {nm: (st[k, r0:r1, c0:c1] if out_window is not None else st[k]).clone()
+             for k, nm in enumerate(STATE_PLANES)}
+    final.update(CI=float(state.CI), firstdaytime=float(state.firstdaytime), timeadd=float(state.timeadd),
+                 Twater=None if state.Twater is None else float(state.Twater), next_step=int(time_stop))
+    return res, final

@@ -407,6 +407,58 @@ __global__ void k_unpack(const uint32_t *__restrict__ masks, float *__restrict__
 }
 
 // single-patch march with float outputs (reference native ABI form)
+// Pack the reference's float cubes (rows, cols, 153) into the mask planes;
+// flags bit 1: a value outside {0, 1} (sh, vegsh) or {0, 1, 2} (vbsh).
+__global__ void k_pack_masks(const float *__restrict__ shm, const float *__restrict__ vsm,
+                             const float *__restrict__ vbm, uint32_t *__restrict__ masks,
+                             uint32_t *__restrict__ flags, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float *a = shm + t * SW_NPATCH, *b = vsm + t * SW_NPATCH, *c = vbm + t * SW_NPATCH;
+    bool bad = false;
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
+        const int pend = min(w * 32 + 32, SW_NPATCH);
+        for (int p = w * 32; p < pend; ++p) {
+            const uint32_t bit = 1u << (p & 31);
+            const float x = a[p], y = b[p], z = c[p];
+            if (x == 1.f) m0 |= bit; else bad |= (x != 0.f);
+            if (y == 1.f) m1 |= bit; else bad |= (y != 0.f);
+            if (z == 1.f) m2 |= bit;
+            else if (z == 2.f) m3 |= bit;
+            else bad |= (z != 0.f);
+        }
+        masks[(long)(0 * SW_MASK_WORDS + w) * n + t] = m0;
+        masks[(long)(1 * SW_MASK_WORDS + w) * n + t] = m1;
+        masks[(long)(2 * SW_MASK_WORDS + w) * n + t] = m2;
+        masks[(long)(3 * SW_MASK_WORDS + w) * n + t] = m3;
+    }
+    if (bad) atomicOr(flags, 1u);
+}
+
+// Tile statistics of packed heights with a caller-given amax:
+// stats = {key(amax), key(max of the three planes), flags}; flags 1 marks a
+// non-finite height. stats must be initialised by k_stats_init.
+__global__ void __launch_bounds__(256)
+k_height_stats(const float4 *__restrict__ hp, uint32_t *__restrict__ stats, long n) {
+    __shared__ uint32_t s_hmax, s_flags;
+    if (threadIdx.x == 0) { s_hmax = 0u; s_flags = 0u; }
+    __syncthreads();
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float4 h = hp[i];
+        if (!(isfinite(h.x) && isfinite(h.y) && isfinite(h.z))) atomicOr(&s_flags, 1u);
+        else atomicMax(&s_hmax, fkey(fmaxf(h.x, fmaxf(h.y, h.z))));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(&stats[1], s_hmax);
+        if (s_flags) atomicOr(&stats[2], s_flags);
+    }
+}
+
+__global__ void k_set_key(uint32_t *stats, float v) { stats[0] = fkey(v); }
+
 __global__ void k_march_svf_single(const float *__restrict__ a, const float *__restrict__ v1,
                                    const float *__restrict__ v2, const int32_t *__restrict__ dx,
                                    const int32_t *__restrict__ dy, const float *__restrict__ dz,
@@ -1105,6 +1157,39 @@ int sw_unpack_masks(const uint32_t *masks, float *shmat, float *vegshmat, float 
     hipLaunchKernelGGL(k_unpack, dim3(grid_for(n * SW_NPATCH, 256)), dim3(256), 0, S(stream), masks, shmat,
                        vegshmat, vbshmat, n);
     return hip_check(hipGetLastError(), "k_unpack");
+}
+
+int sw_pack_masks(const float *shmat, const float *vegshmat, const float *vbshmat, uint32_t *masks,
+                  uint32_t *flags, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_pack_masks");
+    if (rc) return rc;
+    if (!shmat || !vegshmat || !vbshmat || !masks || !flags) return set_err(SW_E_ARG, "sw_pack_masks: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_pack_masks, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), shmat, vegshmat, vbshmat,
+                       masks, flags, n);
+    return hip_check(hipGetLastError(), "k_pack_masks");
+}
+
+int sw_height_stats(const void *hpack, float amax, uint32_t *stats, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_height_stats");
+    if (rc) return rc;
+    if (!hpack || !stats) return set_err(SW_E_ARG, "sw_height_stats: NULL pointer");
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, S(stream), stats);
+    rc = hip_check(hipGetLastError(), "k_stats_init");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_set_key, dim3(1), dim3(1), 0, S(stream), stats, amax);
+    rc = hip_check(hipGetLastError(), "k_set_key");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_height_stats, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), (const float4 *)hpack,
+                       stats, n);
+    return hip_check(hipGetLastError(), "k_height_stats");
+}
+
+int sw_set_amax(uint32_t *stats, float amax, void *stream) {
+    if (!stats) return set_err(SW_E_ARG, "sw_set_amax: NULL pointer");
+    hipLaunchKernelGGL(k_set_key, dim3(1), dim3(1), 0, S(stream), stats, amax);
+    return hip_check(hipGetLastError(), "k_set_key");
 }
 
 int sw_run_march_svf_shadow(const float *a, const float *vegdem, const float *vegdem2, const int32_t *dxs,

@@ -84,6 +84,12 @@ class LandCover:
     tgk_wall: float
     tstart_wall: float
     tmaxlst_wall: float
+    classes: frozenset = frozenset(range(8))   # class codes the table defines
+
+    def key(self):
+        """Hashable identity of the constants (for caching derived tables)."""
+        return (self.present, self.alb.tobytes(), self.emis.tobytes(), self.tgk.tobytes(), self.tstart.tobytes(),
+                self.tmaxlst.tobytes(), self.tgk_wall, self.tstart_wall, self.tmaxlst_wall)
 
 
 def clean_landcover(lc: np.ndarray | None) -> np.ndarray | None:
@@ -96,20 +102,34 @@ def clean_landcover(lc: np.ndarray | None) -> np.ndarray | None:
     return g
 
 
-def landcover_constants(present: bool) -> LandCover:
+def landcover_constants(present: bool, table=None) -> LandCover:
+    """Per-class constants; ``table`` rows are [class, albedo, emissivity, TgK,
+    Tstart, TmaxLST] with a wall row 99 (the land-cover class file read by
+    ``utci_process.py:1318-1322`` and applied by ``Tgmaps_v1.py:15-53``);
+    None selects the default SOLWEIG table."""
     alb = np.zeros(N_CLASS, F32)
     emis = np.zeros(N_CLASS, F32)
     tgk = np.zeros(N_CLASS, F32)
     tst = np.zeros(N_CLASS, F32)
     tmx = np.zeros(N_CLASS, F32)
     if present:
+        classes = LC_CLASSES
+        if table is not None:
+            t = np.asarray(table, dtype=np.float64)
+            if t.ndim != 2 or t.shape[1] != 6:
+                raise ValueError("land-cover class table must have 6 columns")
+            classes = {int(r[0]): tuple(float(x) for x in r[1:]) for r in t}
+            if 99 not in classes:
+                raise ValueError("land-cover class table has no wall row (class 99)")
         for code in range(N_CLASS):
-            row = LC_CLASSES.get(code)
+            row = classes.get(code)
             if row is None:
                 continue
             alb[code], emis[code], tgk[code], tst[code], tmx[code] = row
-        w = LC_CLASSES[99]
-        return LandCover(True, alb, emis, tgk, tst, tmx, w[2], w[3], w[4])
+        w = classes[99]
+        lc = LandCover(True, alb, emis, tgk, tst, tmx, w[2], w[3], w[4])
+        lc.classes = frozenset(c for c in classes if 0 <= c < N_CLASS)
+        return lc
     # no land cover: uniform defaults (utci_process.py:757-766)
     alb[:] = F32(0.0) + F32(ALBEDO_G)
     emis[:] = F32(0.0) + F32(EGROUND)

@@ -56,15 +56,39 @@ class TileInputs:
     aspect_deg: np.ndarray
     landcover: np.ndarray | None
     scale: float
+    lc_table: np.ndarray | None = None   # [class, albedo, emissivity, TgK, Tstart, TmaxLST] rows
 
     @property
     def shape(self):
         return self.dsm.shape
 
+    @property
+    def has_landcover(self) -> bool:
+        return self.landcover is not None
+
+    def spec(self) -> "TileSpec":
+        r, c = self.shape
+        return TileSpec(r, c, self.scale, self.has_landcover, self.lc_table)
+
     @classmethod
     def from_site(cls, site):
         return cls(dsm=site.dsm, dem=site.dem, cdsm=site.cdsm, walls=site.walls, aspect_deg=site.aspect,
                    landcover=site.landcover, scale=site.scale)
+
+
+@dataclass
+class TileSpec:
+    """Shape and site constants of a tile whose rasters are already in HBM."""
+
+    rows: int
+    cols: int
+    scale: float
+    has_landcover: bool
+    lc_table: np.ndarray | None = None
+
+    @property
+    def shape(self):
+        return (self.rows, self.cols)
 
 
 def compose_scene(dsm, dem, cdsm, walls, aspect_deg, landcover=None, *, scale: float) -> TileInputs:
@@ -220,7 +244,7 @@ _MODEL_CACHE: dict = {}
 
 
 def _model_struct_cached(scale: float, lcc: M.LandCover) -> L.ModelStruct:
-    key = (float(scale), bool(lcc.present))
+    key = (float(scale), lcc.key())
     if key not in _MODEL_CACHE:
         _MODEL_CACHE[key] = model_struct(scale, lcc)
     return _MODEL_CACHE[key]
@@ -243,7 +267,7 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
              state: M.DayState | None = None, time_start: int = 0, time_stop: int | None = None) -> DayPlan:
     """Host scalar chain for the whole day; the sun tables stop only at the grid
     edge (the kernels apply the amax stop test on the device)."""
-    lcc = M.landcover_constants(tile.landcover is not None)
+    lcc = M.landcover_constants(tile.has_landcover, tile.lc_table)
     rows, cols = tile.shape
     loop = M.DayLoop(met, sun, latitude=latitude, landcover=lcc, scale=tile.scale, amax=None,
                      rows=rows, cols=cols, state=state)

@@ -90,3 +90,26 @@ def test_geometric_tables_extend_amax_tables():
         cnt = o2[p + 1] - o2[p]
         stop = next((s for s in range(cnt) if not (np.float32(25.0) >= dz2[o2[p] + s])), cnt - 1) + 1
         assert stop == o1[p + 1] - o1[p]
+
+
+def test_plan_day_records_for_a_generated_tile():
+    """Host planning of a generator tile: one record per step, day flags from the sun table,
+    sun-march tables only for daytime steps, records decodable as sw_step_t."""
+    import ctypes
+    from solweig_project_amd import _lib as L
+    from solweig_project_amd import sunpos, synth
+    from solweig_project_amd.solver import STEP_DTYPE, TileInputs, plan_day
+    site = synth.make_site(40, 32, seed=5)
+    tile = TileInputs.from_site(site)
+    assert tile.spec().shape == (40, 32) and tile.spec().has_landcover
+    sun = sunpos.day_sun_table(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=6)
+    plan = plan_day(tile, met, sun, site.latitude)
+    assert plan.n_steps == 24 and plan.steps.size == 24 * ctypes.sizeof(L.StepStruct)
+    rec = plan.steps.view(STEP_DTYPE)
+    assert list(rec["is_day"].astype(bool)) == list(sun.altitude > 0)
+    day = rec["is_day"] == 1
+    assert (rec["sun_count"][day] > 0).all() and (rec["sun_count"][~day] == 0).all()
+    assert rec["sun_off"][day][-1] + rec["sun_count"][day][-1] == plan.sun_tab.shape[0]
+    s0 = L.StepStruct.from_buffer_copy(plan.steps[:ctypes.sizeof(L.StepStruct)].tobytes())
+    assert s0.is_day == rec["is_day"][0] and s0.ta == rec["ta"][0]
