@@ -123,17 +123,19 @@ def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=No
     """153-patch sky-view factors of a DSM with vegetation (``shadow.py:409-625``).
 
     a: building DSM; vegdem / vegdem2: absolute canopy / trunk-zone heights
-    (0 where there is no vegetation); scale: pixels per metre. Returns an
-    :class:`SvfBundle` (19 elements in the reference's order).
+    (0 where there is no vegetation); bush: bush layer (must be zero);
+    amaxvalue: domain maximum of the DSM and the ground-relative canopy;
+    scale: pixels per metre. All are required, as in the reference. Returns
+    an :class:`SvfBundle` (19 elements in the reference's order).
     """
     if int(patch_option) != 2:
         raise UnsupportedInput(f"patch_option {patch_option}: only 2 (153 patches) is implemented")
     if save_rasters:
         raise UnsupportedInput("save_rasters: raster output is not part of this backend")
-    if scale is None:
-        raise ValueError("scale is required")
-    if a is None or vegdem is None or vegdem2 is None:
-        raise ValueError("a, vegdem and vegdem2 are required")
+    required = {"amaxvalue": amaxvalue, "a": a, "vegdem": vegdem, "vegdem2": vegdem2, "bush": bush, "scale": scale}
+    missing = [k for k, v in required.items() if v is None]
+    if missing:
+        raise ValueError("When save_rasters=False, these arguments must be provided: " + ", ".join(missing))
     solver = _solver(device)
     dev = solver.device
     a_t = _plane(a, dev, "a")
@@ -141,10 +143,10 @@ def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=No
     v2 = _plane(vegdem2, dev, "vegdem2")
     if v1.shape != a_t.shape or v2.shape != a_t.shape:
         raise ValueError("a, vegdem and vegdem2 must have one shape")
-    if bush is not None and bool((_plane(bush, dev, "bush") != 0).any()):
+    if bool((_plane(bush, dev, "bush") != 0).any()):
         raise UnsupportedInput("a non-zero bush layer is not supported")
     rows, cols = a_t.shape
-    amax = float(torch.maximum(a_t.max(), v1.max())) if amaxvalue is None else _scalar(amaxvalue)
+    amax = _scalar(amaxvalue)
     n = rows * cols
     hpack = torch.empty((n, 4), dtype=torch.float32, device=dev)
     stats = torch.empty(4, dtype=torch.int32, device=dev)

@@ -61,7 +61,7 @@ def test_svf_calculator_refusals():
         api.svf_calculator(1, a=z, vegdem=z, vegdem2=z, scale=1.0)
     with pytest.raises(UnsupportedInput, match="save_rasters"):
         api.svf_calculator(2, a=z, vegdem=z, vegdem2=z, scale=1.0, save_rasters=True)
-    with pytest.raises(ValueError, match="scale"):
+    with pytest.raises(ValueError, match="must be provided: amaxvalue, bush, scale"):
         api.svf_calculator(2, a=z, vegdem=z, vegdem2=z)
 
 

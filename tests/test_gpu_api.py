@@ -46,7 +46,7 @@ def test_svf_calculator_matches_reference(api, golden_dir, name):
     _say(f"svf_calculator {name}")
     g, _ = _golden(golden_dir, name)
     b = api.svf_calculator(2, amaxvalue=g["amax"], a=g["dsm"], vegdem=g["vegdsm"], vegdem2=g["vegdsm2"],
-                           scale=float(g["scale"]))
+                           bush=np.zeros_like(g["dsm"]), scale=float(g["scale"]))
     got = [x.cpu().numpy() for x in b]
     for k, (x, nm) in enumerate(zip(got, api.REF_ORDER)):
         ref = g[nm].astype(np.float32)
@@ -65,7 +65,7 @@ def test_run_utci_window_matches_reference(api, golden_dir, name, bundle):
         svf = _ref_tuple(g)
     else:
         svf = api.svf_calculator(2, amaxvalue=g["amax"], a=g["dsm"], vegdem=g["vegdsm"], vegdem2=g["vegdsm2"],
-                                 scale=float(g["scale"]))
+                                 bush=np.zeros_like(g["dsm"]), scale=float(g["scale"]))
     out = api.run_utci_window(**_window_kw(g, d, svf_bundle=svf))
     assert np.array_equal(out["shadow"], d["o_shadow"])
     assert np.abs(out["tmrt"] - d["o_tmrt"]).max() <= 0.01
@@ -127,4 +127,37 @@ def test_device_refusals(api, golden_dir):
         api.run_utci_window(**_window_kw(g, d, svf_bundle=svf, lc_class=lc))
     z = np.ones_like(g["dsm"])
     with pytest.raises(UnsupportedInput, match="bush"):
-        api.svf_calculator(2, a=g["dsm"], vegdem=g["vegdsm"], vegdem2=g["vegdsm2"], bush=z, scale=1.0)
+        api.svf_calculator(2, amaxvalue=g["amax"], a=g["dsm"], vegdem=g["vegdsm"], vegdem2=g["vegdsm2"], bush=z,
+                           scale=1.0)
+
+
+def test_core_adapter_solves_a_write_window(api, golden_dir):
+    """SolveRequestView -> dispatch -> HIP backend; outputs are the write-window crop with global origins."""
+    _say("core adapter write window")
+    from solweig_project_amd import met as M
+    from solweig_project_amd.adapter import solve_with_core_adapter
+    from solweig_project_amd.solweig_core import ArrayView, PhysicalWindow, SolveRequestView, TimeCoverage
+    g, d = _golden(golden_dir, "a48")
+    dom = "golden/a48"
+    req = SolveRequestView(logical_domain_id=dom, rows=48, cols=48, origin_x_m=0.0, origin_y_m=0.0,
+                           pixel_size_m=1.0 / float(g["scale"]), read_window=PhysicalWindow(0, 48, 0, 48),
+                           write_window=PhysicalWindow(6, 40, 3, 45), time=TimeCoverage(0, 24, 24),
+                           requested_variables=("utci", "tmrt", "shadow"), profile_id="mi355x_v1", device="cuda",
+                           dtype_plan={"building_dsm": "<f4", "dem": "<f4", "veg_canopy": "<f4", "walls": "<f4",
+                                       "wall_aspect": "<f4", "landcover": "<f4"})
+    names = {"building_dsm": "dsm", "dem": "dem", "veg_canopy": "cdsm", "walls": "walls", "wall_aspect": "aspect",
+             "landcover": "landcover"}
+    planes = {k: ArrayView.from_numpy(np.ascontiguousarray(g[v]), logical_domain_id=dom, freeze=True)
+              for k, v in names.items()}
+    sun = M.SunTable(d["altitude"][0], d["azimuth"][0], d["zen"][0], d["jday"][0], d["dectime"], d["altmax"][0])
+    views = {}
+    out = solve_with_core_adapter(req, planes, met=d["met"], sun=sun, latitude=float(d["latitude"]),
+                                  output_views=views)
+    assert set(out) == {"utci", "tmrt", "shadow"}
+    sl = (slice(None), slice(6, 40), slice(3, 45))
+    assert np.array_equal(out["shadow"], d["o_shadow"][sl])
+    assert np.abs(out["tmrt"] - d["o_tmrt"][sl]).max() <= 0.01
+    m = ~np.isnan(d["o_utci"][sl])
+    assert np.abs(out["utci"][m] - d["o_utci"][sl][m]).max() <= 0.01
+    assert views["tmrt"].global_origin == (0, 6, 3) and views["tmrt"].frozen
+    assert not out["tmrt"].flags.writeable
