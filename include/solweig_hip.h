@@ -45,6 +45,30 @@ extern "C" {
  * buildings), 1 vegsh (1 = not blocked by vegetation), 2 vbsh == 1,
  * 3 vbsh == 2 (vbsh == 0 when neither bit is set). */
 
+/* Per-patch constants of one step, laid out patch-major so the flux kernel
+ * fetches a patch's record with one wide scalar load. Products are float32
+ * in the reference's operation order; the kernel adds them where the
+ * patch's boolean factors are 1. */
+typedef struct sw_patch_k {
+    float lv, lumchi, angl, ster;   /* Perez luminance, lumChi, incidence cosine, steradian */
+    float k_sh;                      /* (shaded_k * ster) * cos_alt: Kref_veg, Kref_sh */
+    float k_sun;                     /* (sunlit_k * ster) * cos_alt: Kref_sun */
+    float yi_k;                      /* shortwave shaded_or_sunlit offset */
+    float sun_lt;                    /* x < sun_lt  <=> atan(x)*180/pi < alt  (sunlit) */
+    float sun_gt;                    /* x > sun_gt  <=> atan(x)*180/pi > alt  (shaded) */
+    float lsky_down, lsky_side;      /* sky longwave per patch */
+    float lv_side, lv_down;          /* (veg_l * ster) * cos_alt | sin_alt */
+    float cos_alt, sin_alt;
+    int32_t l_sun;                   /* longwave sun-facing window active */
+} sw_patch_k_t;
+
+typedef struct sw_patch_l {
+    float yi_l;                      /* longwave shaded_or_sunlit offset */
+    float ls_side, ls_down;          /* (sunlit_l * ster) * cos_alt | sin_alt */
+    float lh_side, lh_down;          /* (shaded_l * ster) * cos_alt | sin_alt */
+    float pad[3];
+} sw_patch_l_t;
+
 /* Per-time-step constants (host-computed scalar chain), one per step in
  * device memory so a captured day replays without host work. */
 typedef struct sw_step {
@@ -60,13 +84,9 @@ typedef struct sw_step {
     float sunlit_k, shaded_k, sunlit_l, shaded_l, veg_l;
     float tgwall, lwall;
     float tg_class[SW_NCLASS];
-    /* per patch */
-    float lv[SW_NPATCH], lumchi[SW_NPATCH], yi_k[SW_NPATCH], yi_l[SW_NPATCH];
-    float lsky_down[SW_NPATCH], lsky_side[SW_NPATCH];
-    float alt[SW_NPATCH], ster[SW_NPATCH], cos_alt[SW_NPATCH], sin_alt[SW_NPATCH], angl[SW_NPATCH];
-    float sun_lt[SW_NPATCH];  /* x < sun_lt  <=> atan(x)*180/pi < alt  (patch sunlit) */
-    float sun_gt[SW_NPATCH];  /* x > sun_gt  <=> atan(x)*180/pi > alt  (patch shaded) */
-    int32_t l_sun[SW_NPATCH]; /* longwave: sun-facing patch window active */
+    int32_t pad0[3];
+    sw_patch_k_t pk[SW_NPATCH];
+    sw_patch_l_t pl[SW_NPATCH];
 } sw_step_t;
 
 /* Step-invariant model constants and walk tables. */
@@ -214,8 +234,10 @@ int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
  * state: 6 planes (Tgmap1, E, S, W, N, TgOut1), read and updated.
  * out: planes selected by out_mask bits: 1 tmrt, 2 utci, 4 shadow,
  * 8 kdown, 16 kup, 32 ldown, 64 lup, written at out + k * n for the k-th
- * selected plane in that bit order. walk_halo must equal model->walk_halo
- * (it sizes the shared-memory tile of walk sources). */
+ * selected plane in that bit order. walk_halo sizes the shared-memory tile
+ * of walk sources and must be at least model->walk_halo for a day step;
+ * -1 launches without the tile (night steps: less shared memory, more
+ * waves). A day step launched with a smaller halo writes NaN outputs. */
 int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg,
                  const float *tan_asvf, const float *tan_svfalfa, const uint32_t *walkface,
                  const float *buildings, const uint8_t *cls, const sw_step_t *steps,

@@ -26,6 +26,20 @@ _c_i32 = ctypes.c_int32
 _c_f = ctypes.c_float
 
 
+class PatchK(ctypes.Structure):
+    """Mirror of ``sw_patch_k_t``."""
+
+    _fields_ = [(k, _c_f) for k in ("lv", "lumchi", "angl", "ster", "k_sh", "k_sun", "yi_k", "sun_lt", "sun_gt",
+                                     "lsky_down", "lsky_side", "lv_side", "lv_down", "cos_alt", "sin_alt")] + [
+        ("l_sun", _c_i32)]
+
+
+class PatchL(ctypes.Structure):
+    """Mirror of ``sw_patch_l_t``."""
+
+    _fields_ = [(k, _c_f) for k in ("yi_l", "ls_side", "ls_down", "lh_side", "lh_down")] + [("pad", _c_f * 3)]
+
+
 class StepStruct(ctypes.Structure):
     """Mirror of ``sw_step_t``."""
 
@@ -39,12 +53,8 @@ class StepStruct(ctypes.Structure):
         ("sunlit_k", _c_f), ("shaded_k", _c_f), ("sunlit_l", _c_f), ("shaded_l", _c_f), ("veg_l", _c_f),
         ("tgwall", _c_f), ("lwall", _c_f),
         ("tg_class", _c_f * NCLASS),
-        ("lv", _c_f * NPATCH), ("lumchi", _c_f * NPATCH), ("yi_k", _c_f * NPATCH), ("yi_l", _c_f * NPATCH),
-        ("lsky_down", _c_f * NPATCH), ("lsky_side", _c_f * NPATCH),
-        ("alt", _c_f * NPATCH), ("ster", _c_f * NPATCH), ("cos_alt", _c_f * NPATCH),
-        ("sin_alt", _c_f * NPATCH), ("angl", _c_f * NPATCH),
-        ("sun_lt", _c_f * NPATCH), ("sun_gt", _c_f * NPATCH),
-        ("l_sun", _c_i32 * NPATCH),
+        ("pad0", _c_i32 * 3),
+        ("pk", PatchK * NPATCH), ("pl", PatchL * NPATCH),
     ]
 
 

@@ -688,15 +688,127 @@ __global__ void k_utci(const float *__restrict__ ta, const float *__restrict__ r
 // ---------------------------------------------------------------------------
 // flux step
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mask_bits(const uint32_t *msh, const uint32_t *mvs, const uint32_t *mv1,
-                                          const uint32_t *mv2, int p, float *sh, float *vs, float *vb) {
-    const int w = p >> 5;
-    const uint32_t bit = 1u << (p & 31);
-    *sh = (msh[w] & bit) ? 1.f : 0.f;
-    *vs = (mvs[w] & bit) ? 1.f : 0.f;
-    *vb = (mv1[w] & bit) ? 1.f : ((mv2[w] & bit) ? 2.f : 0.f);
+// Pass A over the 153 patches (Kside_veg_v2022a patch sums, solweig.py:564-832,
+// and the first define_patch_characteristics pass, solweig.py:1439-1648).
+struct PassA {
+    float aniLum, KsideD, Kref_sun, Kref_sh, Kref_veg;
+    float Ldown_sky, Ldown_veg, Ldown_sun, Ldown_sh;
+    float Lside_sky, Lside_veg, Lside_sun, Lside_sh;
+};
+
+template <bool DAY>
+__device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, const uint32_t *msh,
+                                             const uint32_t *mvs, const uint32_t *mv1, const uint32_t *mv2,
+                                             float hsvf, PassA &o) {
+    float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
+    float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f;
+    float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f;
+    const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        // this word's bits, shifted down one patch per iteration
+        uint32_t bsh = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
+        uint32_t bvs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
+        uint32_t bv1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
+        uint32_t bv2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
+        const int pend = min(w * 32 + 32, SW_NPATCH);
+#pragma unroll 4
+        for (int p = w * 32; p < pend; ++p) {
+            const bool sh = bsh & 1u, vs = bvs & 1u, vb1 = bv1 & 1u, vb2 = bv2 & 1u;
+            bsh >>= 1; bvs >>= 1; bv1 >>= 1; bv2 >>= 1;
+            const bool tvs = !vs || !(vb1 || vb2);   // vegsh == 0 or vbsh == 0
+            const bool tsh = !sh && vb1;             // (1 - sh) * vbsh == 1
+            const bool tsky = sh && vs;
+            const sw_patch_k_t q = sp->pk[p];
+            const sw_patch_l_t u = sp->pl[p];
+            if (DAY) {
+                const float diffsh = vs ? (sh ? 1.f : 0.f) : (sh ? d10 : d00);
+                aniLum = aniLum + diffsh * q.lv;
+                KsideD = KsideD + diffsh * q.lumchi * q.angl * q.ster;
+                Kref_veg = Kref_veg + (tvs ? q.k_sh : 0.f);
+                const float x = hsvf + q.yi_k;
+                Kref_sun = Kref_sun + ((tsh && x < q.sun_lt) ? q.k_sun : 0.f);
+                Kref_sh = Kref_sh + ((tsh && x > q.sun_gt) ? q.k_sh : 0.f);
+            }
+            Ldown_sky = Ldown_sky + (tsky ? q.lsky_down : 0.f);
+            Lside_sky = Lside_sky + (tsky ? q.lsky_side : 0.f);
+            Lside_veg = Lside_veg + (tvs ? q.lv_side : 0.f);
+            Ldown_veg = Ldown_veg + (tvs ? q.lv_down : 0.f);
+            // sun-facing window: sunlit / shaded by the threshold test; elsewhere every
+            // shaded wall patch counts as shaded and none as sunlit
+            const float xl = hsvf + u.yi_l;
+            const bool ls = q.l_sun != 0;
+            const bool sunlit = tsh && ls && (xl < q.sun_lt);
+            const bool shaded = tsh && (!ls || (xl > q.sun_gt));
+            Lside_sun = Lside_sun + (sunlit ? u.ls_side : 0.f);
+            Lside_sh = Lside_sh + (shaded ? u.lh_side : 0.f);
+            Ldown_sun = Ldown_sun + (sunlit ? u.ls_down : 0.f);
+            Ldown_sh = Ldown_sh + (shaded ? u.lh_down : 0.f);
+        }
+    }
+    o.aniLum = aniLum; o.KsideD = KsideD; o.Kref_sun = Kref_sun; o.Kref_sh = Kref_sh; o.Kref_veg = Kref_veg;
+    o.Ldown_sky = Ldown_sky; o.Ldown_veg = Ldown_veg; o.Ldown_sun = Ldown_sun; o.Ldown_sh = Ldown_sh;
+    o.Lside_sky = Lside_sky; o.Lside_veg = Lside_veg; o.Lside_sun = Lside_sun; o.Lside_sh = Lside_sh;
 }
 
+// Ground-view walk of one azimuth (sunonsurface_2018a, solweig.py:136-288).
+// Booleans are kept as integers; the sums that add a constant (walls seen,
+// their longwave and albedo terms) are kept as counts k and read from the
+// sequential-sum tables s_tlw[k] / s_tal[k] afterwards, which is the same
+// float32 value the step-by-step additions produce. A source that left
+// the grid keeps the last in-grid temporaries (stale border); INTERIOR
+// tiles have every source in the grid.
+struct WalkAcc {
+    int kb, kw;                 // walls seen (bub), walls-or-blocked seen (bubw)
+    float wsh, wlupsh, walbsh, walbnosh;
+};
+
+template <bool INTERIOR>
+__device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, const float4 *__restrict__ s_dec,
+                                             const sw_model_t *__restrict__ md, int j, int nsec, int nfirst,
+                                             int lbase, int lw, int r, int c, int rows, int cols, int fb,
+                                             WalkAcc &acc, WalkAcc &snap) {
+    int f = fb;
+    int bub = 0, bubw = 0;
+    float ts = 0.f, tl = 0.f, ta = 0.f, tn = 0.f;
+    int tb = 0, tw = 0;
+    bool live = true;
+    acc.kb = 0; acc.kw = 0; acc.wsh = 0.f; acc.wlupsh = 0.f; acc.walbsh = 0.f; acc.walbnosh = 0.f;
+    const int32_t *wdx = md->walk_dx + j * SW_WALK_MAX, *wdy = md->walk_dy + j * SW_WALK_MAX;
+#define SW_WALK_STEP(S)                                                                      \
+    {                                                                                        \
+        const int ddx = wdx[S], ddy = wdy[S];                                                \
+        if (!INTERIOR)                                                                       \
+            live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols); \
+        if (INTERIOR || live) {                                                              \
+            const uint2 e = s_src[lbase + ddx * lw + ddy];                                   \
+            const uint32_t fl = e.y;                                                         \
+            const float4 d = s_dec[(fl & 3u) | ((fl >> 2) & 0x1Cu)];                         \
+            tb = (int)((fl >> 3) & 1u);                                                      \
+            tw = (int)((fl >> 2) & 1u);                                                      \
+            ts = d.x; ta = d.y; tn = d.z;                                                    \
+            tl = __uint_as_float(e.x);                                                       \
+        }                                                                                    \
+        f &= tb;                                                                             \
+        acc.wsh += f ? ts : 0.f;                                                             \
+        acc.wlupsh += f ? tl : 0.f;                                                          \
+        acc.walbsh += f ? ta : 0.f;                                                          \
+        acc.walbnosh += f ? tn : 0.f;                                                        \
+        bub |= tw & f;                                                                       \
+        acc.kb += bub;                                                                       \
+        bubw |= f ^ 1;                                                                       \
+        acc.kw += bubw;                                                                      \
+    }
+    int s = 0;
+    for (; s < nfirst; ++s) SW_WALK_STEP(s)
+    snap = acc;
+#pragma unroll 2
+    for (; s < nsec; ++s) SW_WALK_STEP(s)
+#undef SW_WALK_STEP
+}
+
+// WALK = false instantiates a night-only kernel (no walk tile, no shortwave
+// code, fewer registers); it refuses a day step with NaN outputs.
+template <bool WALK>
 __global__ void __launch_bounds__(256)
 k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
@@ -707,6 +819,9 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             int rows, int cols) {
     extern __shared__ uint2 s_src[];
     __shared__ float s_alb[SW_NCLASS];
+    __shared__ float4 s_dec[32];                  // walk source decode: {shadow, alb * shadow, alb}
+    __shared__ float s_tlw[SW_WALK_MAX + 1];      // k-fold sequential sums of lwall
+    __shared__ float s_tal[SW_WALK_MAX + 1];      // k-fold sequential sums of albedo_b
     const sw_step_t *sp = steps + step;
     const sw_model_t *md = model;
     const int tiles_x = (cols + FLUX_TX - 1) / FLUX_TX;
@@ -719,10 +834,13 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     const bool valid = (r < rows) && (c < cols);
     const long n = (long)rows * cols;
     const long t = valid ? (long)r * cols + c : 0;
-    const bool day = sp->is_day != 0;
+    const bool day = WALK && sp->is_day != 0;
     const int lw = FLUX_TX + 2 * halo, lh = FLUX_TY + 2 * halo;
+    const float omp = sp->one_m_psi;
     if (threadIdx.x < SW_NCLASS) s_alb[threadIdx.x] = md->alb_class[threadIdx.x];
-    if (day) {
+    // halo < walk_halo (incl. the tile-less launch, halo = -1) refuses a day step below
+    const bool staged = day && halo >= md->walk_halo;
+    if (staged) {
         // stage the walk sources of the tile + halo (out-of-grid cells are
         // never read: the walk tracks liveness on global coordinates)
         for (int i = threadIdx.x; i < lw * lh; i += blockDim.x) {
@@ -732,10 +850,27 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             if ((unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols) v = src[(long)gr * cols + gc];
             s_src[i] = v;
         }
+        if (threadIdx.x < 32) {
+            const int shb = threadIdx.x & 1, vsb = (threadIdx.x >> 1) & 1, k = threadIdx.x >> 2;
+            const float al = md->alb_class[k];
+            const float shd = (float)shb - (1.f - (float)vsb) * omp;
+            s_dec[threadIdx.x] = make_float4(shd, al * shd, al, 0.f);
+        }
+        if (threadIdx.x == 32) {
+            float a1 = 0.f, a2 = 0.f;
+            const float lwall = sp->lwall, albb = md->albedo_b;
+            s_tlw[0] = 0.f; s_tal[0] = 0.f;
+            for (int k = 1; k <= SW_WALK_MAX; ++k) {
+                a1 = a1 + lwall;
+                a2 = a2 + albb;
+                s_tlw[k] = a1;
+                s_tal[k] = a2;
+            }
+        }
     }
     __syncthreads();
     if (!valid) return;
-    if (md->walk_halo > halo) {
+    if ((!WALK && sp->is_day) || (day && !staged)) {
         // the shared tile is smaller than the walk reach: refuse loudly
         int k = 0;
         for (int b = 1; b <= 64; b <<= 1)
@@ -747,17 +882,12 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     const float se_t = md->sbc * md->emis_class[k_t];
     float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f, Kside = 0.f;
     float Lup, LupE, LupS, LupW, LupN, shadow = 0.f;
-    uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
-    load_masks(masks, n, t, msh, mvs, mv1, mv2);
-    const float hsvf = tan_asvf[t];
-    float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
     float F_sh = 0.f, galb = 0.f, gnosh = 0.f, galbE = 0.f, gnoshE = 0.f, galbS = 0.f, gnoshS = 0.f,
           galbW = 0.f, gnoshW = 0.f, galbN = 0.f, gnoshN = 0.f;
     if (day) {
         const uint2 me = src[t];
         const uint32_t fl_t = me.y;
         const float lup_t = __uint_as_float(me.x);
-        const float omp = sp->one_m_psi;
         shadow = (float)(fl_t & 1u) - (1.f - (float)((fl_t >> 1) & 1u)) * omp;
         const float alb_t = s_alb[k_t];
         // ---- ground view factors (gvf_2018a): 18 azimuths x nsec steps ----
@@ -765,56 +895,25 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         float gLW = 0.f, gAW = 0.f, gNW = 0.f, gLN = 0.f, gAN = 0.f, gNN = 0.f;
         const int nsec = md->nsec;
         const float first = md->first, second = md->second;
+        int nfirst = 0;
+        while (nfirst < nsec && (float)(nfirst + 1) <= first) ++nfirst;
         const float fp1 = first + 1.f, sp1 = second + 1.f;
-        const float lwall = sp->lwall, albb = md->albedo_b;
         const uint32_t wf = walkface[t];
         const float nb = b_t * -1.f + 1.f;
+        const int fb = (b_t == 1.f) ? 1 : 0;     // buildings is 0/1
         const int lbase = (ty + halo) * lw + (tx + halo);
+        const int wh = md->walk_halo;
+        const bool interior = (r0 >= wh) && (r0 + FLUX_TY + wh <= rows) && (c0 >= wh) && (c0 + FLUX_TX + wh <= cols);
         for (int j = 0; j < SW_WALK_AZ; ++j) {
-            float tb = 0.f, ts = 0.f, tl = 0.f, ta_ = 0.f, tn = 0.f, tw = 0.f;
-            float f = b_t, bub = 0.f, bubw = 0.f;
-            float wsh = 0.f, wwall = 0.f, wlupsh = 0.f, wlwall = 0.f, walbsh = 0.f, walbwall = 0.f,
-                  walbnosh = 0.f, walbwallnosh = 0.f;
-            float f_wwall = 0.f, f_wsh = 0.f, f_wlwall = 0.f, f_wlupsh = 0.f, f_walbwall = 0.f,
-                  f_walbsh = 0.f, f_walbwallnosh = 0.f, f_walbnosh = 0.f;
-            bool live = true;
-            for (int s = 0; s < nsec; ++s) {
-                const int ddx = md->walk_dx[j * SW_WALK_MAX + s];
-                const int ddy = md->walk_dy[j * SW_WALK_MAX + s];
-                // a source that left the grid stays out (offsets grow monotonically);
-                // the temporaries then keep the last in-grid values (stale border)
-                live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols);
-                if (live) {
-                    const uint2 e = s_src[lbase + ddx * lw + ddy];
-                    const uint32_t fl = e.y;
-                    const float shd = (float)(fl & 1u) - (1.f - (float)((fl >> 1) & 1u)) * omp;
-                    const float al = s_alb[(fl >> 4) & 7u];
-                    tb = (fl & 8u) ? 1.f : 0.f;
-                    ts = shd;
-                    tl = __uint_as_float(e.x);
-                    ta_ = al * shd;
-                    tn = al;
-                    tw = (fl & 4u) ? 1.f : 0.f;
-                }
-                f = fminf(f, tb);
-                wsh = wsh + ts * f;
-                wlupsh = wlupsh + tl * f;
-                walbsh = walbsh + ta_ * f;
-                walbnosh = walbnosh + tn * f;
-                const float tempb = tw * f;
-                const float tempbwall = f * -1.f + 1.f;
-                bub = ((tempb + bub) > 0.f) ? 1.f : 0.f;
-                bubw = ((tempbwall + bubw) > 0.f) ? 1.f : 0.f;
-                wlwall = wlwall + bub * lwall;
-                walbwall = walbwall + bub * albb;
-                wwall = wwall + bub;
-                walbwallnosh = walbwallnosh + bubw * albb;
-                if ((float)(s + 1) <= first) {
-                    f_wwall = wwall; f_wsh = wsh; f_wlwall = wlwall; f_wlupsh = wlupsh;
-                    f_walbwall = walbwall; f_walbsh = walbsh; f_walbwallnosh = walbwallnosh;
-                    f_walbnosh = walbnosh;
-                }
-            }
+            WalkAcc A, F;
+            if (interior)
+                walk_azimuth<true>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
+            else
+                walk_azimuth<false>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
+            const float wwall = (float)A.kb, wlwall = s_tlw[A.kb], walbwall = s_tal[A.kb];
+            const float walbwallnosh = s_tal[A.kw];
+            const float f_wwall = (float)F.kb, f_wlwall = s_tlw[F.kb], f_walbwall = s_tal[F.kb];
+            const float f_walbwallnosh = s_tal[F.kw];
             // keep == 1  <=>  wwall == second and facesh == 0 (bit j of walkface)
             const bool keep1 = (wwall == second) && ((wf >> j) & 1u);
             const float wsi1 = (f_wwall > 0.f) ? 1.f : 0.f;
@@ -823,12 +922,12 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             const float wi2 = (walbwallnosh > 0.f) ? 1.f : 0.f;
             const float lw2 = keep1 ? 0.f : wlwall;
             const float aw2 = keep1 ? 0.f : walbwall;
-            const float gl1 = ((f_wlwall + f_wlupsh) / fp1) * wsi1 + (f_wlupsh) / first * (wsi1 * -1.f + 1.f);
-            const float gl2 = ((lw2 + wlupsh) / sp1) * wsi2 + (wlupsh) / second * (wsi2 * -1.f + 1.f);
-            const float ga1 = ((f_walbwall + f_walbsh) / fp1) * wsi1 + (f_walbsh) / first * (wsi1 * -1.f + 1.f);
-            const float ga2 = ((aw2 + walbsh) / sp1) * wsi2 + (walbsh) / second * (wsi2 * -1.f + 1.f);
-            const float gn1 = ((f_walbwallnosh + f_walbnosh) / fp1) * wi1 + (f_walbnosh) / first * (wi1 * -1.f + 1.f);
-            const float gn2 = ((walbwallnosh + walbnosh) / second) * wi2 + (walbnosh) / second * (wi2 * -1.f + 1.f);
+            const float gl1 = ((f_wlwall + F.wlupsh) / fp1) * wsi1 + (F.wlupsh) / first * (wsi1 * -1.f + 1.f);
+            const float gl2 = ((lw2 + A.wlupsh) / sp1) * wsi2 + (A.wlupsh) / second * (wsi2 * -1.f + 1.f);
+            const float ga1 = ((f_walbwall + F.walbsh) / fp1) * wsi1 + (F.walbsh) / first * (wsi1 * -1.f + 1.f);
+            const float ga2 = ((aw2 + A.walbsh) / sp1) * wsi2 + (A.walbsh) / second * (wsi2 * -1.f + 1.f);
+            const float gn1 = ((f_walbwallnosh + F.walbnosh) / fp1) * wi1 + (F.walbnosh) / first * (wi1 * -1.f + 1.f);
+            const float gn2 = ((walbwallnosh + A.walbnosh) / second) * wi2 + (A.walbnosh) / second * (wi2 * -1.f + 1.f);
             float gLj = (gl1 * 0.5f + gl2 * 0.4f) / 0.9f;
             gLj = gLj + lup_t * nb;
             float gAj = (ga1 * 0.5f + ga2 * 0.4f) / 0.9f;
@@ -893,58 +992,23 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         LupE = LupS = LupW = LupN = Lup;
     }
     // ---- pass A over the 153 patches: diffuse sky + Kside (day) and the
-    // first define_patch_characteristics pass (every step); each accumulator
-    // keeps the reference's patch order ----
-    float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f, Ldown_ref = 0.f;
-    float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f, Lside_ref = 0.f;
-    const float veg_l = sp->veg_l, sunlit_l = sp->sunlit_l, shaded_l = sp->shaded_l;
-    const float sunlit_k = sp->sunlit_k, shaded_k = sp->shaded_k;
-    for (int w = 0; w < SW_MASK_WORDS; ++w) {
-        // this word's bits, shifted down one patch per iteration
-        uint32_t bsh = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
-        uint32_t bvs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
-        uint32_t bv1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
-        uint32_t bv2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
-        const int pend = min(w * 32 + 32, SW_NPATCH);
-#pragma unroll 4
-        for (int p = w * 32; p < pend; ++p) {
-            const float sh = (bsh & 1u) ? 1.f : 0.f;
-            const float vs = (bvs & 1u) ? 1.f : 0.f;
-            const float vb = (bv1 & 1u) ? 1.f : ((bv2 & 1u) ? 2.f : 0.f);
-            bsh >>= 1; bvs >>= 1; bv1 >>= 1; bv2 >>= 1;
-            const float ster = sp->ster[p], calt = sp->cos_alt[p], salt = sp->sin_alt[p];
-            const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
-            const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
-            if (day) {
-                const float diffsh = sh - (1.f - vs) * 0.97f;
-                aniLum = aniLum + diffsh * sp->lv[p];
-                KsideD = KsideD + diffsh * sp->lumchi[p] * sp->angl[p] * ster;
-                Kref_veg = Kref_veg + shaded_k * tvs * ster * calt;
-                const float x = hsvf + sp->yi_k[p];
-                const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
-                const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
-                Kref_sun = Kref_sun + sunlit_k * sunlit * tsh * ster * calt;
-                Kref_sh = Kref_sh + shaded_k * shaded * tsh * ster * calt;
-            }
-            const float tsky = (sh == 1.f && vs == 1.f) ? 1.f : 0.f;
-            Ldown_sky = Ldown_sky + tsky * sp->lsky_down[p];
-            Lside_sky = Lside_sky + tsky * sp->lsky_side[p];
-            Lside_veg = Lside_veg + veg_l * ster * calt * tvs;
-            Ldown_veg = Ldown_veg + veg_l * ster * salt * tvs;
-            if (sp->l_sun[p]) {
-                const float x = hsvf + sp->yi_l[p];
-                const float sunlit = (x < sp->sun_lt[p]) ? 1.f : 0.f;
-                const float shaded = (x > sp->sun_gt[p]) ? 1.f : 0.f;
-                Lside_sun = Lside_sun + sunlit_l * sunlit * ster * calt * tsh;
-                Lside_sh = Lside_sh + shaded_l * shaded * ster * calt * tsh;
-                Ldown_sun = Ldown_sun + sunlit_l * sunlit * ster * salt * tsh;
-                Ldown_sh = Ldown_sh + shaded_l * shaded * ster * salt * tsh;
-            } else {
-                Lside_sh = Lside_sh + shaded_l * ster * calt * tsh;
-                Ldown_sh = Ldown_sh + shaded_l * ster * salt * tsh;
-            }
-        }
-    }
+    // first define_patch_characteristics pass (every step). Products of
+    // step and patch constants come precomputed (k_sh, k_sun, lv_*, ls_*,
+    // lh_*) and are added where the patch's boolean factors are 1, which is
+    // the value the full product takes (adding +0 elsewhere changes no sum
+    // that starts at +0 and only grows); each accumulator keeps the
+    // reference's patch order ----
+    uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
+    load_masks(masks, n, t, msh, mvs, mv1, mv2);
+    const float hsvf = tan_asvf[t];
+    PassA pa;
+    if (day) patch_pass_a<true>(sp, msh, mvs, mv1, mv2, hsvf, pa);
+    else patch_pass_a<false>(sp, msh, mvs, mv1, mv2, hsvf, pa);
+    const float aniLum = pa.aniLum, KsideD = pa.KsideD, Kref_sun = pa.Kref_sun, Kref_sh = pa.Kref_sh,
+                Kref_veg = pa.Kref_veg;
+    const float Ldown_sky = pa.Ldown_sky, Ldown_veg = pa.Ldown_veg, Ldown_sun = pa.Ldown_sun, Ldown_sh = pa.Ldown_sh;
+    const float Lside_sky = pa.Lside_sky, Lside_veg = pa.Lside_veg, Lside_sun = pa.Lside_sun, Lside_sh = pa.Lside_sh;
+    float Ldown_ref = 0.f, Lside_ref = 0.f;
     if (day) {
         const float dRad = aniLum * sp->radD;
         const float sb = 1.f - svfbuveg[t];
@@ -973,8 +1037,9 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         for (int p = w * 32; p < pend; ++p) {
             const float trf = (tr & 1u) ? 1.f : 0.f;
             tr >>= 1;
-            Lside_ref = Lside_ref + refl * sp->ster[p] * sp->cos_alt[p] * trf;
-            Ldown_ref = Ldown_ref + refl * sp->ster[p] * sp->sin_alt[p] * trf;
+            const sw_patch_k_t q = sp->pk[p];
+            Lside_ref = Lside_ref + refl * q.ster * q.cos_alt * trf;
+            Ldown_ref = Ldown_ref + refl * q.ster * q.sin_alt * trf;
         }
     }
     const float Lside = Lside_sky + Lside_veg + Lside_sh + Lside_sun + Lside_ref;
@@ -1257,16 +1322,23 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
     if (!masks || !src || !svfbuveg || !tan_asvf || !tan_svfalfa || !walkface || !buildings || !cls || !steps ||
         !model || !state || !out)
         return set_err(SW_E_ARG, "sw_flux_step: NULL pointer");
-    if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_flux_step: walk halo out of range");
+    if (walk_halo < -1 || walk_halo > 48) return set_err(SW_E_ARG, "sw_flux_step: walk halo out of range");
     const int tiles = ((cols + FLUX_TX - 1) / FLUX_TX) * ((rows + FLUX_TY - 1) / FLUX_TY);
     const int grid = ((tiles + 7) / 8) * 8;
-    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t lds = walk_halo < 0 ? 0
+                                     : (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
     if (lds > 65536) {
-        rc = hip_check(hipFuncSetAttribute((const void *)k_flux_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+        rc = hip_check(hipFuncSetAttribute((const void *)k_flux_step<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds), "hipFuncSetAttribute");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_flux_step, dim3(grid), dim3(256), lds, S(stream), masks, (const uint2 *)src,
+    if (walk_halo < 0) {
+        hipLaunchKernelGGL(k_flux_step<false>, dim3(grid), dim3(256), 0, S(stream), masks, (const uint2 *)src,
+                           svfbuveg, tan_asvf, tan_svfalfa, walkface, buildings, cls, steps, model, state, out,
+                           out_mask, step, walk_halo, rows, cols);
+        return hip_check(hipGetLastError(), "k_flux_step<night>");
+    }
+    hipLaunchKernelGGL(k_flux_step<true>, dim3(grid), dim3(256), lds, S(stream), masks, (const uint2 *)src,
                        svfbuveg, tan_asvf, tan_svfalfa, walkface, buildings, cls, steps, model, state, out,
                        out_mask, step, walk_halo, rows, cols);
     return hip_check(hipGetLastError(), "k_flux_step");

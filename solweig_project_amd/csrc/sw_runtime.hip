@@ -249,7 +249,8 @@ int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t
         }
         rc = sw_flux_step(tile->masks, tile->src, tile->svfbuveg, tile->tan_asvf, tile->tan_svfalfa,
                           tile->walkface, tile->buildings, tile->cls, steps, model, tile->state,
-                          out + (long)(i - t0) * stride, out_mask, i, walk_halo, rows, cols, stream);
+                          out + (long)(i - t0) * stride, out_mask, i, is_day[i - t0] ? walk_halo : -1, rows,
+                          cols, stream);
         if (rc) return rc;
     }
     return SW_OK;

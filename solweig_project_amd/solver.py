@@ -174,7 +174,6 @@ STEP_DTYPE = np.dtype(L.StepStruct)
 _SCALARS = ("ta", "ta_mat", "rh", "va", "tak4", "radI", "radD", "radG", "sinalt", "cosalt_k",
             "tan_zen", "one_m_psi", "tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
             "shaded_l", "veg_l", "tgwall", "lwall")
-_PATCH_ROWS = ("lv", "lumchi", "yi_k", "yi_l", "lsky_down", "lsky_side")
 
 
 def step_records(params, sun_offs) -> np.ndarray:
@@ -193,16 +192,26 @@ def step_records(params, sun_offs) -> np.ndarray:
         r["tw_first"] = int(sp.tw_first)
         r["tw_update"] = int(sp.tw_update)
         r["tg_class"] = sp.tg_class
-        for k in _PATCH_ROWS:
-            r[k] = getattr(sp, k)
         g = sp.geom
         if id(g) not in geo_memo:
             lt, gt = patch_thresholds(np.asarray(g.alt, F32))
-            geo_memo[id(g)] = dict(alt=g.alt, ster=g.ster, cos_alt=g.cos_alt, sin_alt=g.sin_alt,
-                                   angl=g.angl_inc, sun_lt=lt, sun_gt=gt)
-        for k, v in geo_memo[id(g)].items():
-            r[k] = v
-        r["l_sun"] = sp.l_sun_side
+            geo_memo[id(g)] = (lt, gt)
+        lt, gt = geo_memo[id(g)]
+        ster, calt, salt = (np.asarray(x, F32) for x in (g.ster, g.cos_alt, g.sin_alt))
+        pk, pl = r["pk"], r["pl"]
+        pk["lv"], pk["lumchi"], pk["angl"], pk["ster"] = sp.lv, sp.lumchi, g.angl_inc, ster
+        # step x patch products in float32, in the flux kernel's operation order
+        pk["k_sh"] = (F32(r["shaded_k"]) * ster) * calt
+        pk["k_sun"] = (F32(r["sunlit_k"]) * ster) * calt
+        pk["yi_k"], pk["sun_lt"], pk["sun_gt"] = sp.yi_k, lt, gt
+        pk["lsky_down"], pk["lsky_side"] = sp.lsky_down, sp.lsky_side
+        vbase = F32(r["veg_l"]) * ster
+        pk["lv_side"], pk["lv_down"] = vbase * calt, vbase * salt
+        pk["cos_alt"], pk["sin_alt"], pk["l_sun"] = calt, salt, sp.l_sun_side
+        sbase, hbase = F32(r["sunlit_l"]) * ster, F32(r["shaded_l"]) * ster
+        pl["yi_l"] = sp.yi_l
+        pl["ls_side"], pl["ls_down"] = sbase * calt, sbase * salt
+        pl["lh_side"], pl["lh_down"] = hbase * calt, hbase * salt
     return rec
 
 
@@ -516,7 +525,8 @@ class HipSolver:
             with self.timer.span("flux_day" if plan.is_day[i] else "flux_night", stream):
                 L.call("sw_flux_step", P(ds.masks), P(ds.src), P(ds.svfbuveg), P(ds.tan_asvf), P(ds.tan_sa),
                        P(ds.walkface), P(ds.buildings), P(ds.cls), P(dplan["steps"]), P(dplan["model"]),
-                       P(ds.state), base + i * stride, mask, i, plan.walk_halo, ds.rows, ds.cols, sp)
+                       P(ds.state), base + i * stride, mask, i, plan.walk_halo if plan.is_day[i] else -1,
+                       ds.rows, ds.cols, sp)
         return out
 
     def output_names(self, outputs):
