@@ -33,6 +33,7 @@ extern "C" {
 #define SW_WALK_AZ 18          /* ground-view-factor search azimuths */
 #define SW_WALK_MAX 64         /* max walk steps per azimuth */
 #define SW_NCLASS 8            /* land-cover class codes 0..7 */
+#define SW_SVF_PAD 16          /* border cells of the march's padded height buffer */
 
 /* Error codes */
 #define SW_OK 0
@@ -170,12 +171,15 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
                const float *landcover, void *hpack, float *vegdsm2, float *buildings, uint8_t *cls,
                float *aspect_rad, uint32_t *stats, int rows, int cols, void *stream);
 /* Patch tables: tab[offs[p] .. offs[p+1]) holds int4 {dx, dy, dz bits,
- * dx * cols + dy} for every geometrically valid step of patch p; the
- * library applies the reference's amax stop test with amax from stats,
- * writing the executed step count of each patch to counts[153]
- * (caller-owned workspace). */
-int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts,
-                 uint32_t *masks, const uint32_t *stats, int rows, int cols, void *stream);
+ * dx * (cols + 2 * SW_SVF_PAD) + dy} for every geometrically valid step of
+ * patch p (consecutive steps move at most one cell); the library applies
+ * the reference's amax stop test with amax from stats, writing the executed
+ * step count of each patch to counts[153]. hpad is caller-owned workspace of
+ * (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 3 floats; stats[3] is
+ * rewritten (1 when some height is negative, which selects the
+ * bounds-checked march). */
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
+                 uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream);
 /* Fold the masks into the 16 fields [svf, E, S, W, N, veg, vegE.., aveg, avegE.., total]
  * (shadow.py:545-613), in the reference's per-annulus summation order. */
 int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso,
@@ -246,10 +250,10 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
 
 /* ---- whole stages ---------------------------------------------------------
  * SVF sweep = sw_svf_march + sw_svf_fold (svf_calculator, shadow.py:409-625). */
-int sw_svf_sweep(const void *hpack, const uint32_t *stats, const int32_t *offs, const int32_t *tab,
+int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
                  const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
-                 const int32_t *nann, const int32_t *dirm, int32_t *counts, uint32_t *masks, float *fields,
-                 int rows, int cols, void *stream);
+                 const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                 float *fields, int rows, int cols, void *stream);
 /* Day loop over steps [t0, t1) (the time loop of run_utci_window,
  * utci_process.py:917-1049): per step the sun march when is_day[i - t0]
  * (host array) and the flux step; outputs of step i at

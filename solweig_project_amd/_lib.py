@@ -21,6 +21,7 @@ NFIELDS = 16
 WALK_AZ = 18
 WALK_MAX = 64
 NCLASS = 8
+SVF_PAD = 16
 
 _c_i32 = ctypes.c_int32
 _c_f = ctypes.c_float
@@ -119,11 +120,11 @@ SIGNATURES = {
     "sw_graph_end": (_I, [_VP, _VP]),
     "sw_graph_launch": (_I, [_VP, _VP]),
     "sw_graph_destroy": (_I, [_VP]),
-    "sw_svf_sweep": (_I, [_VP] * 13 + [_I, _I, _VP]),
+    "sw_svf_sweep": (_I, [_VP] * 14 + [_I, _I, _VP]),
     "sw_solve_day": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP]),
     "sw_pack_heights": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_compose": (_I, [_VP] * 11 + [_I, _I, _VP]),
-    "sw_svf_march": (_I, [_VP] * 6 + [_I, _I, _VP]),
+    "sw_svf_march": (_I, [_VP] * 7 + [_I, _I, _VP]),
     "sw_svf_fold": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_unpack_masks": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_pack_masks": (_I, [_VP] * 5 + [_I, _I, _VP]),

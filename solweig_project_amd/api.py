@@ -151,6 +151,7 @@ def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=No
     hpack = torch.empty((n, 4), dtype=torch.float32, device=dev)
     stats = torch.empty(4, dtype=torch.int32, device=dev)
     counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
+    hpad = torch.empty(((rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD), 3), dtype=torch.float32, device=dev)
     masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
     fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
     sp = _stream_ptr(None)
@@ -160,7 +161,7 @@ def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=No
     offs, tab = solver.svf_tables(rows, cols, float(scale))
     c = solver._fold_consts
     L.call("sw_svf_sweep", P(hpack), P(stats), P(offs), P(tab), P(v2), P(c["w_iso"]), P(c["w_aniso"]),
-           P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(counts), P(masks), P(fields), rows, cols, sp)
+           P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(counts), P(hpad), P(masks), P(fields), rows, cols, sp)
     if int(stats[2].item()) & 1:
         raise UnsupportedInput("non-finite height in a, vegdem or vegdem2")
     return SvfBundle(rows, cols, masks, fields, solver)

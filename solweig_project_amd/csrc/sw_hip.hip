@@ -159,145 +159,246 @@ k_compose(const float *__restrict__ dsm, const float *__restrict__ dem, const fl
     }
 }
 
+// per-patch executed step count: the march runs while the previous step had
+// amax >= dz, so the first step whose dz exceeds amax is the last one run;
+// one wave per patch
+__global__ void __launch_bounds__(64)
+k_svf_counts(const int4 *__restrict__ tab, const int32_t *__restrict__ offs, uint32_t *__restrict__ stats,
+             int32_t *__restrict__ counts) {
+    const int p = blockIdx.x;
+    const float amax = funkey(stats[0]);
+    const int o = offs[p], len = offs[p + 1] - o;
+    int c = len;
+    for (int base = 0; base < len; base += 64) {
+        const int s = base + (int)threadIdx.x;
+        const bool stop = (s < len) && !(amax >= __int_as_float(tab[o + s].z));
+        const unsigned long long m = __ballot(stop);
+        if (m) {
+            c = base + __ffsll((long long)m);   // 1-based index of the first stop step = count
+            break;
+        }
+    }
+    if (threadIdx.x == 0) {
+        counts[p] = c;
+        if (p == 0) stats[3] = 0u;   // negative-height flag, set by k_pad_heights
+    }
+}
+
+// Heights padded by SW_SVF_PAD cells on every side for the march, 12 bytes
+// per cell {dsm, vegdsm, vegdsm2}; pad cells hold {0, 0, -0}. For a
+// non-negative target and dz >= 0 a zero-height source has the same effect
+// as the reference's zero-filled out-of-grid temporary, and the sign bit of
+// vegdsm2 marks "outside" for the exact out-of-grid exit (grid cells carry
+// +0 there: -0 and +0 act alike in every use). Every table step moves at
+// most one cell and a lane stops after its first outside step at s >= 1, so
+// its reads reach at most 1 + SVF_CHUNK cells past the grid, inside the
+// SW_SVF_PAD border. stats[3] gets bit 1 when any
+// height is negative (the march then checks bounds).
+__global__ void __launch_bounds__(256)
+k_pad_heights(const float4 *__restrict__ hp, float *__restrict__ hpad, uint32_t *__restrict__ stats, int rows,
+              int cols) {
+    __shared__ uint32_t s_neg;
+    if (threadIdx.x == 0) s_neg = 0u;
+    __syncthreads();
+    const int W = cols + 2 * SW_SVF_PAD, H = rows + 2 * SW_SVF_PAD;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (long)W * H) {
+        const int pr = (int)(i / W), pc = (int)(i - (long)pr * W);
+        const int r = pr - SW_SVF_PAD, c = pc - SW_SVF_PAD;
+        float x = 0.f, y = 0.f, z = -0.f;
+        if ((unsigned)r < (unsigned)rows && (unsigned)c < (unsigned)cols) {
+            const float4 v = hp[(long)r * cols + c];
+            x = v.x;
+            y = v.y;
+            z = v.z + 0.f;   // -0 -> +0, every other value unchanged
+            if (x < 0.f || y < 0.f || z < 0.f) s_neg = 1u;
+        }
+        hpad[3 * i] = x;
+        hpad[3 * i + 1] = y;
+        hpad[3 * i + 2] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_neg) atomicOr(&stats[3], 1u);
+}
+
+struct H3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ H3 ld3(const float *__restrict__ hp, long i) {
+    const float *q = hp + 3 * i;
+    H3 h;
+    h.x = q[0];
+    h.y = q[1];
+    h.z = q[2];
+    return h;
+}
+
+__device__ __forceinline__ bool outside(const H3 &h) { return (__float_as_uint(h.z) >> 31) != 0u; }
+
 // ---------------------------------------------------------------------------
 // SVF patch march. Semantics of shadow(): out-of-bounds sources read an
 // exact 0, the first step carries the vegetation condition and the
 // target-local trunk gate and clears the accumulator, and a step runs only
 // while the previous step had amax >= dz (per-patch counts computed on the
-// device by k_svf_counts). Tables: int4 {dx, dy, dz bits, dx * cols + dy}.
+// device by k_svf_counts). Tables: int4 {dx, dy, dz bits, dx * W + dy} with
+// W = cols + 2 * SW_SVF_PAD, the padded row pitch.
+//
+// State per lane: f (running max of a - dz), sh = f > a, vs in {0, 1}
+// (vegsh of the reference, never negative), vbb = "vbshvegsh sum > 0".
+// Exact early exits (every later step leaves the outputs unchanged):
+// sh && !vs; for a >= 0 after the first step, a source outside the grid
+// (all later ones are outside too) or hmax - dz <= a (no later source rises
+// above the target).
 // ---------------------------------------------------------------------------
-__global__ void k_svf_counts(const int4 *__restrict__ tab, const int32_t *__restrict__ offs,
-                             const uint32_t *__restrict__ stats, int32_t *__restrict__ counts) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= SW_NPATCH) return;
-    const float amax = funkey(stats[0]);
-    const int o = offs[p], len = offs[p + 1] - o;
-    int c = len;
-    for (int s = 0; s < len; ++s) {
-        if (!(amax >= __int_as_float(tab[o + s].z))) {
-            c = s + 1;
-            break;
-        }
-    }
-    counts[p] = c;
-}
-
-struct SvfState {
+struct SvfLane {
     float f;
-    int sh, vs, vb;
+    bool sh, vs, vbb;
 };
 
-// one march step; returns whether the source was inside the grid
-__device__ __forceinline__ bool svf_gather(const float4 *__restrict__ hp, long t, int r, int c, int rows, int cols,
-                                           int4 e, float *ha, float *hv, float *hv2, bool want) {
-    const bool inb = ((unsigned)(r + e.x) < (unsigned)rows) && ((unsigned)(c + e.y) < (unsigned)cols);
-    *ha = 0.f; *hv = 0.f; *hv2 = 0.f;
-    if (inb && want) {
-        const float4 h = hp[t + e.w];
-        *ha = h.x; *hv = h.y; *hv2 = h.z;
-    }
-    return inb;
-}
-
-__device__ __forceinline__ void svf_update(SvfState &st, bool inb, float ha, float hv, float hv2, float dzs,
-                                           float at, float veg2t, bool first) {
-    float tmp = 0.f, tv = 0.f, tv2 = 0.f;
-    if (inb) {
-        tv = hv - dzs;
-        tv2 = hv2 - dzs;
-        tmp = ha - dzs;
-    }
+__device__ __forceinline__ void svf_first(SvfLane &st, float at, float veg2t, float ha, float hv, float hv2,
+                                          float dz) {
+    const float tmp = ha - dz, tv = hv - dz, tv2 = hv2 - dz;
     st.f = fmaxf(st.f, tmp);
-    st.sh = (st.f > at) ? 1 : 0;
-    const int v2 = (tv > at ? 1 : 0) - (tv2 > at ? 1 : 0);
-    st.vs = st.vs > v2 ? st.vs : v2;
-    if (st.vs * st.sh > 0) st.vs = 0;
-    st.vb = st.vs + st.vb;
-    if (first) {
-        float fv = tv - tmp;
-        if (fv <= 0.f) fv = 1000.f;
-        if (fv < dzs) st.vs = 1;
-        st.vs = st.vs * ((veg2t > at) ? 1 : 0);
-        st.vb = 0;
+    st.sh = st.f > at;
+    st.vs = (tv > at) && !(tv2 > at) && !st.sh;
+    float fv = tv - tmp;
+    if (fv <= 0.f) fv = 1000.f;
+    if (fv < dz) st.vs = true;
+    st.vs = st.vs && (veg2t > at);
+    st.vbb = false;
+}
+
+__device__ __forceinline__ void svf_next(SvfLane &st, float at, float ha, float hv, float hv2, float dz) {
+    const float tmp = ha - dz, tv = hv - dz, tv2 = hv2 - dz;
+    st.f = fmaxf(st.f, tmp);
+    st.sh = st.f > at;
+    st.vs = (st.vs || ((tv > at) && !(tv2 > at))) && !st.sh;
+    st.vbb = st.vbb || st.vs;
+}
+
+__device__ __forceinline__ uint32_t svf_out(const SvfLane &st) {
+    const uint32_t out_sh = st.sh ? 0u : 1u;
+    const uint32_t out_vs = st.vs ? 0u : 1u;
+    const uint32_t out_vb = 1u - (st.vbb ? 1u : 0u) + (st.vs ? 1u : 0u);
+    return out_sh | (out_vs << 1) | (out_vb << 2);
+}
+
+// fast path: every height >= 0 and dz >= 0, sources read from the padded
+// buffer; chunks of SVF_CHUNK steps keep that many gathers in flight per lane
+#define SVF_CHUNK 8
+static_assert(SVF_CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
+__device__ __forceinline__ uint32_t march_svf_fast(const float *__restrict__ hpad, long tp, float at,
+                                                   float veg2t, bool valid, const int4 *__restrict__ T,
+                                                   int count, float hmax) {
+    SvfLane st;
+    st.f = at; st.sh = false; st.vs = false; st.vbb = false;
+    bool active = valid && count > 0;
+    if (active) {
+        const int4 e = T[0];
+        const H3 h = ld3(hpad, tp + e.w);
+        svf_first(st, at, veg2t, h.x, h.y, h.z, __int_as_float(e.z));
+        if (st.sh && !st.vs) active = false;
     }
-}
-
-// exact early exits (state is a fixed point of every later step):
-// sh == 1 && vs == 0, or (after the first step, for a non-negative target)
-// every later source is out of the grid or below the target.
-__device__ __forceinline__ bool svf_exit(const SvfState &st, int s, bool exit_ok, bool inb, float hmax, float dzs,
-                                         float at) {
-    if (st.sh == 1 && st.vs == 0) return true;
-    return s >= 1 && exit_ok && (!inb || (hmax - dzs) <= at);
-}
-
-__device__ __forceinline__ uint32_t march_svf_pixel(const float4 *__restrict__ hp, int rows, int cols, long t,
-                                                    int r, int c, float at, float veg2t, bool valid,
-                                                    const int4 *__restrict__ T, int count, float hmax) {
-    SvfState st;
-    st.f = at; st.sh = 0; st.vs = 0; st.vb = 0;
-    bool active = valid;
-    const bool exit_ok = (at >= 0.f);
-    int s = 0;
-    // chunks of four steps: four independent gathers in flight per lane
-    for (; s + 4 <= count; s += 4) {
+    int s = 1;
+    for (; s + SVF_CHUNK <= count; s += SVF_CHUNK) {
         if (!wave_any(active)) break;
-        const int4 e0 = T[s], e1 = T[s + 1], e2 = T[s + 2], e3 = T[s + 3];
-        float a0, v0, w0, a1, v1, w1, a2, v2, w2, a3, v3, w3;
-        const bool i0 = svf_gather(hp, t, r, c, rows, cols, e0, &a0, &v0, &w0, active);
-        const bool i1 = svf_gather(hp, t, r, c, rows, cols, e1, &a1, &v1, &w1, active);
-        const bool i2 = svf_gather(hp, t, r, c, rows, cols, e2, &a2, &v2, &w2, active);
-        const bool i3 = svf_gather(hp, t, r, c, rows, cols, e3, &a3, &v3, &w3, active);
+        int4 e[SVF_CHUNK];
+        H3 h[SVF_CHUNK];
+#pragma unroll
+        for (int k = 0; k < SVF_CHUNK; ++k) e[k] = T[s + k];
         if (active) {
-            const float d0 = __int_as_float(e0.z), d1 = __int_as_float(e1.z), d2 = __int_as_float(e2.z),
-                        d3 = __int_as_float(e3.z);
-            bool ex;
-            svf_update(st, i0, a0, v0, w0, d0, at, veg2t, s == 0);
-            ex = svf_exit(st, s, exit_ok, i0, hmax, d0, at);
-            svf_update(st, i1, a1, v1, w1, d1, at, veg2t, false);
-            ex = ex || svf_exit(st, s + 1, exit_ok, i1, hmax, d1, at);
-            svf_update(st, i2, a2, v2, w2, d2, at, veg2t, false);
-            ex = ex || svf_exit(st, s + 2, exit_ok, i2, hmax, d2, at);
-            svf_update(st, i3, a3, v3, w3, d3, at, veg2t, false);
-            ex = ex || svf_exit(st, s + 3, exit_ok, i3, hmax, d3, at);
+#pragma unroll
+            for (int k = 0; k < SVF_CHUNK; ++k) h[k] = ld3(hpad, tp + e[k].w);
+            bool ex = false;
+#pragma unroll
+            for (int k = 0; k < SVF_CHUNK; ++k) {
+                const float d = __int_as_float(e[k].z);
+                svf_next(st, at, h[k].x, h[k].y, h[k].z, d);
+                ex = ex || (st.sh && !st.vs) || outside(h[k]) || (hmax - d <= at);
+            }
             if (ex) active = false;
         }
     }
     for (; s < count; ++s) {
         if (!wave_any(active)) break;
         const int4 e = T[s];
-        float ha, hv, hv2;
-        const bool inb = svf_gather(hp, t, r, c, rows, cols, e, &ha, &hv, &hv2, active);
         if (active) {
-            const float dzs = __int_as_float(e.z);
-            svf_update(st, inb, ha, hv, hv2, dzs, at, veg2t, s == 0);
-            if (svf_exit(st, s, exit_ok, inb, hmax, dzs, at)) active = false;
+            const H3 h = ld3(hpad, tp + e.w);
+            const float dz = __int_as_float(e.z);
+            svf_next(st, at, h.x, h.y, h.z, dz);
+            if ((st.sh && !st.vs) || outside(h) || (hmax - dz <= at)) active = false;
         }
     }
-    const int out_sh = 1 - st.sh;
-    int b = st.vb > 0 ? 1 : st.vb;
-    b = b - st.vs;
-    const int vsf = st.vs > 0 ? 1 : st.vs;
-    const int out_vs = 1 - vsf;
-    const int out_vb = 1 - b;
-    return (uint32_t)out_sh | ((uint32_t)out_vs << 1) | ((uint32_t)out_vb << 2);
+    return svf_out(st);
 }
 
-// grid: x = pixel tiles of 256 (XCD-remapped), y = 32-patch word group.
+// checked path (some height is negative): out-of-grid sources read exact
+// zeros as in the reference; the out-of-grid exit applies to a >= 0 only
+__device__ __forceinline__ uint32_t march_svf_checked(const float *__restrict__ hpad, long tp, int r, int c,
+                                                      int rows, int cols, float at, float veg2t, bool valid,
+                                                      const int4 *__restrict__ T, int count, float hmax) {
+    SvfLane st;
+    st.f = at; st.sh = false; st.vs = false; st.vbb = false;
+    bool active = valid;
+    const bool exit_ok = at >= 0.f;
+    for (int s = 0; s < count; ++s) {
+        if (!wave_any(active)) break;
+        const int4 e = T[s];
+        if (active) {
+            const bool inb = ((unsigned)(r + e.x) < (unsigned)rows) && ((unsigned)(c + e.y) < (unsigned)cols);
+            const float dz = __int_as_float(e.z);
+            float tmp = 0.f, tv = 0.f, tv2 = 0.f;
+            if (inb) {
+                const H3 h = ld3(hpad, tp + e.w);
+                tmp = h.x - dz;
+                tv = h.y - dz;
+                tv2 = h.z - dz;
+            }
+            // the update on (tmp, tv, tv2) directly: a source value of 0 is not h - dz
+            st.f = fmaxf(st.f, tmp);
+            st.sh = st.f > at;
+            if (s == 0) {
+                st.vs = (tv > at) && !(tv2 > at) && !st.sh;
+                float fv = tv - tmp;
+                if (fv <= 0.f) fv = 1000.f;
+                if (fv < dz) st.vs = true;
+                st.vs = st.vs && (veg2t > at);
+                st.vbb = false;
+                if (st.sh && !st.vs) active = false;
+            } else {
+                st.vs = (st.vs || ((tv > at) && !(tv2 > at))) && !st.sh;
+                st.vbb = st.vbb || st.vs;
+                if ((st.sh && !st.vs) || (exit_ok && (!inb || (hmax - dz) <= at))) active = false;
+            }
+        }
+    }
+    return svf_out(st);
+}
+
+// grid: x = 16 x 16 pixel tiles (XCD-remapped; each wave marches an 8 x 8
+// square, which keeps more lanes busy than a row segment because exit
+// steps are spatially correlated), y = 32-patch word group.
+#define SVF_TILE 16
 __global__ void __launch_bounds__(256)
-k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
+k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
             const int32_t *__restrict__ counts, uint32_t *__restrict__ masks,
             const uint32_t *__restrict__ stats, int rows, int cols) {
     const long n = (long)rows * cols;
     const float hmax = funkey(stats[1]);
+    const bool checked = stats[3] != 0u;
+    const int tiles_x = (cols + SVF_TILE - 1) / SVF_TILE;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const long t = (long)tile * 256 + threadIdx.x;
-    const bool valid = (t < n);
-    const int r = valid ? (int)(t / cols) : 0;
-    const int c = valid ? (int)(t - (long)r * cols) : 0;
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int r = (tile / tiles_x) * SVF_TILE + (wv >> 1) * 8 + (ln >> 3);
+    const int c = (tile % tiles_x) * SVF_TILE + (wv & 1) * 8 + (ln & 7);
+    const bool valid = (r < rows) && (c < cols);
+    const int W = cols + 2 * SW_SVF_PAD;
+    const long tp = valid ? (long)(r + SW_SVF_PAD) * W + (c + SW_SVF_PAD) : 0;
+    const long t = valid ? (long)r * cols + c : 0;
     float at = 0.f, veg2t = 0.f;
     if (valid) {
-        const float4 h = hp[t];
+        const H3 h = ld3(hpad, tp);
         at = h.x;
         veg2t = h.z;
     }
@@ -306,8 +407,12 @@ k_svf_march(const float4 *__restrict__ hp, const int32_t *__restrict__ offs, con
     const int p1 = min(p0 + 32, SW_NPATCH);
     uint32_t bsh = 0, bvs = 0, bv1 = 0, bv2 = 0;
     for (int p = p0; p < p1; ++p) {
-        const uint32_t m = march_svf_pixel(hp, rows, cols, valid ? t : 0, r, c, at, veg2t, valid, tab + offs[p],
-                                           counts[p], hmax);
+        // the zero-height pad stands for the reference's zero source only when
+        // dz >= 0; float32 tan(90 deg) makes the zenith patch's dz negative
+        const bool chk = checked || (counts[p] > 0 && __int_as_float(tab[offs[p]].z) < 0.f);
+        const uint32_t m = chk
+            ? march_svf_checked(hpad, tp, r, c, rows, cols, at, veg2t, valid, tab + offs[p], counts[p], hmax)
+            : march_svf_fast(hpad, tp, at, veg2t, valid, tab + offs[p], counts[p], hmax);
         const uint32_t bit = 1u << (p - p0);
         if (m & 1u) bsh |= bit;
         if (m & 2u) bvs |= bit;
@@ -1184,18 +1289,24 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
     return hip_check(hipGetLastError(), "k_compose");
 }
 
-int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, uint32_t *masks,
-                 const uint32_t *stats, int rows, int cols, void *stream) {
+int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
+                 uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream) {
     int rc = check_shape(rows, cols, "sw_svf_march");
     if (rc) return rc;
-    if (!hpack || !offs || !tab || !counts || !masks || !stats)
+    if (!hpack || !offs || !tab || !counts || !hpad || !masks || !stats)
         return set_err(SW_E_ARG, "sw_svf_march: NULL pointer");
-    const long n = (long)rows * cols;
-    hipLaunchKernelGGL(k_svf_counts, dim3(1), dim3(256), 0, S(stream), (const int4 *)tab, offs, stats, counts);
+    const long padded = (long)(rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD);
+    hipLaunchKernelGGL(k_svf_counts, dim3(SW_NPATCH), dim3(64), 0, S(stream), (const int4 *)tab, offs, stats,
+                       counts);
     rc = hip_check(hipGetLastError(), "k_svf_counts");
     if (rc) return rc;
-    dim3 grid(grid_xcd(n), SW_MASK_WORDS);
-    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float4 *)hpack, offs, (const int4 *)tab,
+    hipLaunchKernelGGL(k_pad_heights, dim3(grid_for(padded, 256)), dim3(256), 0, S(stream), (const float4 *)hpack,
+                       (float *)hpad, stats, rows, cols);
+    rc = hip_check(hipGetLastError(), "k_pad_heights");
+    if (rc) return rc;
+    const long tiles = (long)((rows + SVF_TILE - 1) / SVF_TILE) * ((cols + SVF_TILE - 1) / SVF_TILE);
+    dim3 grid((unsigned)(((tiles + 7) / 8) * 8), SW_MASK_WORDS);
+    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float *)hpad, offs, (const int4 *)tab,
                        counts, masks, stats, rows, cols);
     return hip_check(hipGetLastError(), "k_svf_march");
 }

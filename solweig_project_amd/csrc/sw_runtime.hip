@@ -216,11 +216,11 @@ int sw_graph_destroy(void *graph_exec) {
     return swi_hip_check(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy");
 }
 
-int sw_svf_sweep(const void *hpack, const uint32_t *stats, const int32_t *offs, const int32_t *tab,
+int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
                  const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
-                 const int32_t *nann, const int32_t *dirm, int32_t *counts, uint32_t *masks, float *fields,
-                 int rows, int cols, void *stream) {
-    int rc = sw_svf_march(hpack, offs, tab, counts, masks, stats, rows, cols, stream);
+                 const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                 float *fields, int rows, int cols, void *stream) {
+    int rc = sw_svf_march(hpack, offs, tab, counts, hpad, masks, stats, rows, cols, stream);
     if (rc) return rc;
     return sw_svf_fold(masks, vegdsm2, w_iso, w_aniso, ring, nann, dirm, fields, rows, cols, stream);
 }

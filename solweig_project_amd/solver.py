@@ -335,6 +335,8 @@ class DeviceScene:
         self.aspect = torch.empty(n, dtype=torch.float32, device=dev)
         self.stats = torch.empty(4, dtype=torch.int32, device=dev)
         self.counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
+        self.hpad = torch.empty(((rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD), 3), dtype=torch.float32,
+                                device=dev)
         self.masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
         self.fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
         self.svfbuveg = torch.empty(n, dtype=torch.float32, device=dev)
@@ -444,7 +446,14 @@ class HipSolver:
         key = (rows, cols, scale)
         if key not in self._tables:
             offs, dx, dy, dz = SV.svf_tables(self.vault, scale, None, rows, cols)
-            tab = np.stack([dx, dy, dz.view(np.int32), dx * cols + dy], axis=1).astype(np.int32)
+            # the padded-buffer march needs steps of at most one cell
+            for p in range(L.NPATCH):
+                a, b = int(offs[p]), int(offs[p + 1])
+                if b > a and (max(abs(int(dx[a])), abs(int(dy[a]))) > 1 or (b - a > 1 and max(
+                        np.abs(np.diff(dx[a:b])).max(), np.abs(np.diff(dy[a:b])).max()) > 1)):
+                    raise UnsupportedInput(f"patch {p}: march step longer than one cell")
+            pitch = cols + 2 * L.SVF_PAD
+            tab = np.stack([dx, dy, dz.view(np.int32), dx * pitch + dy], axis=1).astype(np.int32)
             self._tables[key] = (torch.from_numpy(np.ascontiguousarray(offs)).to(self.device),
                                  torch.from_numpy(np.ascontiguousarray(tab)).to(self.device))
         return self._tables[key]
@@ -464,8 +473,8 @@ class HipSolver:
         self.compose(ds, stream)
         offs, tab = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
         with self.timer.span("svf_march", stream):
-            L.call("sw_svf_march", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.masks), P(ds.stats),
-                   ds.rows, ds.cols, sp)
+            L.call("sw_svf_march", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.hpad), P(ds.masks),
+                   P(ds.stats), ds.rows, ds.cols, sp)
         c = self._fold_consts
         with self.timer.span("svf_fold", stream):
             L.call("sw_svf_fold", P(ds.masks), P(ds.vegdsm2), P(c["w_iso"]), P(c["w_aniso"]), P(c["ring"]),

@@ -164,3 +164,26 @@ def test_repeat_runs_bitwise_identical(solver, c1_case):
     assert np.array_equal(f1, f2)
     for k in o1:
         assert np.array_equal(o1[k], o2[k], equal_nan=True), k
+
+
+def test_svf_with_negative_heights_matches_oracle(solver):
+    """Ground below zero selects the bounds-checked march (out-of-grid sources read exact zeros)."""
+    from oracle import driver as D
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import compose_scene
+    _say("svf negative heights vs oracle")
+    site = synth.make_site(72, 56, seed=44)
+    off = np.float32(3.0)
+    dsm, dem = site.dsm - off, site.dem - off
+    scene = compose_scene(dsm, dem, site.cdsm, site.walls, site.aspect, site.landcover, scale=site.scale)
+    ds = solver.upload(scene)
+    solver.svf(ds)
+    cubes = [c.cpu().numpy() for c in solver.unpack_masks(ds)]
+    torch.cuda.synchronize()
+    assert int(ds.stats[3].item()) == 1
+    sc = D.compose(dsm, dem, site.cdsm, site.scale)
+    fields, shm, vm, bm = D.svf(sc)
+    for got, ref in zip(cubes, (shm, vm, bm)):
+        assert np.array_equal(got, ref.reshape(153, 72, 56).transpose(1, 2, 0))
+    got_f = ds.fields.reshape(16, 72, 56).cpu().numpy()
+    assert np.abs(got_f - fields).max() <= 1e-5

@@ -131,8 +131,8 @@ def test_svf_sweep_one_call_matches_split(L, day_case):
     c = solver._fold_consts
     P = L.ptr
     L.call("sw_svf_sweep", P(ds.hpack), P(ds.stats), P(offs), P(tab), P(ds.vegdsm2), P(c["w_iso"]),
-           P(c["w_aniso"]), P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(ds.counts), P(ds.masks), P(ds.fields),
-           ds.rows, ds.cols, torch.cuda.current_stream().cuda_stream)
+           P(c["w_aniso"]), P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(ds.counts), P(ds.hpad), P(ds.masks),
+           P(ds.fields), ds.rows, ds.cols, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert torch.equal(ds.masks, ref_masks)
     assert torch.equal(ds.fields, ref_fields)
