@@ -171,11 +171,13 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
                const float *landcover, void *hpack, float *vegdsm2, float *buildings, uint8_t *cls,
                float *aspect_rad, uint32_t *stats, int rows, int cols, void *stream);
 /* Patch tables: tab[offs[p] .. offs[p+1]) holds int4 {dx, dy, dz bits,
- * dx * (cols + 2 * SW_SVF_PAD) + dy} for every geometrically valid step of
+ * 12 * (dx * (cols + 2 * SW_SVF_PAD) + dy)} (byte offset in the padded
+ * buffer) for every geometrically valid step of
  * patch p (consecutive steps move at most one cell); the library applies
  * the reference's amax stop test with amax from stats, writing the executed
  * step count of each patch to counts[153]. hpad is caller-owned workspace of
- * (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 3 floats; stats[3] is
+ * (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 3 floats (below 2 GiB);
+ * stats[3] is
  * rewritten (1 when some height is negative, which selects the
  * bounds-checked march). */
 int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,

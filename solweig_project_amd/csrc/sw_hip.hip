@@ -225,12 +225,16 @@ struct H3 {
     float x, y, z;
 };
 
-__device__ __forceinline__ H3 ld3(const float *__restrict__ hp, long i) {
-    const float *q = hp + 3 * i;
+typedef unsigned int sw_u3 __attribute__((ext_vector_type(3)));
+
+// one padded cell through a buffer resource: range-checked by the hardware
+// (an offset past the buffer reads zeros instead of faulting)
+__device__ __forceinline__ H3 ld3(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off) {
+    const sw_u3 v = __builtin_amdgcn_raw_buffer_load_b96(rs, byte_off, 0, 0);
     H3 h;
-    h.x = q[0];
-    h.y = q[1];
-    h.z = q[2];
+    h.x = __uint_as_float(v.x);
+    h.y = __uint_as_float(v.y);
+    h.z = __uint_as_float(v.z);
     return h;
 }
 
@@ -241,8 +245,8 @@ __device__ __forceinline__ bool outside(const H3 &h) { return (__float_as_uint(h
 // exact 0, the first step carries the vegetation condition and the
 // target-local trunk gate and clears the accumulator, and a step runs only
 // while the previous step had amax >= dz (per-patch counts computed on the
-// device by k_svf_counts). Tables: int4 {dx, dy, dz bits, dx * W + dy} with
-// W = cols + 2 * SW_SVF_PAD, the padded row pitch.
+// device by k_svf_counts). Tables: int4 {dx, dy, dz bits, byte offset
+// 12 * (dx * W + dy)} with W = cols + 2 * SW_SVF_PAD, the padded row pitch.
 //
 // State per lane: f (running max of a - dz), sh = f > a, vs in {0, 1}
 // (vegsh of the reference, never negative), vbb = "vbshvegsh sum > 0".
@@ -252,29 +256,29 @@ __device__ __forceinline__ bool outside(const H3 &h) { return (__float_as_uint(h
 // above the target).
 // ---------------------------------------------------------------------------
 struct SvfLane {
-    float f;
     bool sh, vs, vbb;
 };
 
+// f = max(a, tmp_1 .. tmp_s) exceeds a exactly when some tmp_i does, so the
+// running maximum reduces to an OR of comparisons
+// (bitwise & | on the booleans: no short-circuit branches in the hot loop)
 __device__ __forceinline__ void svf_first(SvfLane &st, float at, float veg2t, float ha, float hv, float hv2,
                                           float dz) {
     const float tmp = ha - dz, tv = hv - dz, tv2 = hv2 - dz;
-    st.f = fmaxf(st.f, tmp);
-    st.sh = st.f > at;
-    st.vs = (tv > at) && !(tv2 > at) && !st.sh;
+    st.sh = tmp > at;
+    st.vs = (tv > at) & !(tv2 > at) & !st.sh;
     float fv = tv - tmp;
     if (fv <= 0.f) fv = 1000.f;
-    if (fv < dz) st.vs = true;
-    st.vs = st.vs && (veg2t > at);
+    st.vs = st.vs | (fv < dz);
+    st.vs = st.vs & (veg2t > at);
     st.vbb = false;
 }
 
 __device__ __forceinline__ void svf_next(SvfLane &st, float at, float ha, float hv, float hv2, float dz) {
     const float tmp = ha - dz, tv = hv - dz, tv2 = hv2 - dz;
-    st.f = fmaxf(st.f, tmp);
-    st.sh = st.f > at;
-    st.vs = (st.vs || ((tv > at) && !(tv2 > at))) && !st.sh;
-    st.vbb = st.vbb || st.vs;
+    st.sh = st.sh | (tmp > at);
+    st.vs = (st.vs | ((tv > at) & !(tv2 > at))) & !st.sh;
+    st.vbb = st.vbb | st.vs;
 }
 
 __device__ __forceinline__ uint32_t svf_out(const SvfLane &st) {
@@ -285,49 +289,50 @@ __device__ __forceinline__ uint32_t svf_out(const SvfLane &st) {
 }
 
 // fast path: every height >= 0 and dz >= 0, sources read from the padded
-// buffer; chunks of SVF_CHUNK steps keep that many gathers in flight per lane
+// buffer; chunks of SVF_CHUNK steps keep that many gathers in flight per
+// lane. The exit tests run on the chunk's last step: each exit condition,
+// once true, holds for every later step (the fixed point sh && !vegsh, a
+// ray that left the grid stays out, dz grows along the patch), so testing
+// the last step of a chunk finds the exit the per-step test would have
+// found in that chunk, and the steps run past it change no output.
 #define SVF_CHUNK 8
 static_assert(SVF_CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
-__device__ __forceinline__ uint32_t march_svf_fast(const float *__restrict__ hpad, long tp, float at,
+__device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, uint32_t tpb, float at,
                                                    float veg2t, bool valid, const int4 *__restrict__ T,
                                                    int count, float hmax) {
     SvfLane st;
-    st.f = at; st.sh = false; st.vs = false; st.vbb = false;
+    st.sh = false; st.vs = false; st.vbb = false;
     bool active = valid && count > 0;
     if (active) {
         const int4 e = T[0];
-        const H3 h = ld3(hpad, tp + e.w);
+        const H3 h = ld3(rs, tpb + (uint32_t)e.w);
         svf_first(st, at, veg2t, h.x, h.y, h.z, __int_as_float(e.z));
-        if (st.sh && !st.vs) active = false;
+        active = !(st.sh & !st.vs);
     }
     int s = 1;
     for (; s + SVF_CHUNK <= count; s += SVF_CHUNK) {
         if (!wave_any(active)) break;
         int4 e[SVF_CHUNK];
-        H3 h[SVF_CHUNK];
 #pragma unroll
         for (int k = 0; k < SVF_CHUNK; ++k) e[k] = T[s + k];
         if (active) {
+            H3 h[SVF_CHUNK];
 #pragma unroll
-            for (int k = 0; k < SVF_CHUNK; ++k) h[k] = ld3(hpad, tp + e[k].w);
-            bool ex = false;
+            for (int k = 0; k < SVF_CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
 #pragma unroll
-            for (int k = 0; k < SVF_CHUNK; ++k) {
-                const float d = __int_as_float(e[k].z);
-                svf_next(st, at, h[k].x, h[k].y, h[k].z, d);
-                ex = ex || (st.sh && !st.vs) || outside(h[k]) || (hmax - d <= at);
-            }
-            if (ex) active = false;
+            for (int k = 0; k < SVF_CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
+            const float dl = __int_as_float(e[SVF_CHUNK - 1].z);
+            active = !((st.sh & !st.vs) | outside(h[SVF_CHUNK - 1]) | (hmax - dl <= at));
         }
     }
     for (; s < count; ++s) {
         if (!wave_any(active)) break;
         const int4 e = T[s];
         if (active) {
-            const H3 h = ld3(hpad, tp + e.w);
+            const H3 h = ld3(rs, tpb + (uint32_t)e.w);
             const float dz = __int_as_float(e.z);
             svf_next(st, at, h.x, h.y, h.z, dz);
-            if ((st.sh && !st.vs) || outside(h) || (hmax - dz <= at)) active = false;
+            active = !((st.sh & !st.vs) | outside(h) | (hmax - dz <= at));
         }
     }
     return svf_out(st);
@@ -335,11 +340,11 @@ __device__ __forceinline__ uint32_t march_svf_fast(const float *__restrict__ hpa
 
 // checked path (some height is negative): out-of-grid sources read exact
 // zeros as in the reference; the out-of-grid exit applies to a >= 0 only
-__device__ __forceinline__ uint32_t march_svf_checked(const float *__restrict__ hpad, long tp, int r, int c,
+__device__ __forceinline__ uint32_t march_svf_checked(__amdgpu_buffer_rsrc_t rs, uint32_t tpb, int r, int c,
                                                       int rows, int cols, float at, float veg2t, bool valid,
                                                       const int4 *__restrict__ T, int count, float hmax) {
     SvfLane st;
-    st.f = at; st.sh = false; st.vs = false; st.vbb = false;
+    st.sh = false; st.vs = false; st.vbb = false;
     bool active = valid;
     const bool exit_ok = at >= 0.f;
     for (int s = 0; s < count; ++s) {
@@ -350,15 +355,14 @@ __device__ __forceinline__ uint32_t march_svf_checked(const float *__restrict__ 
             const float dz = __int_as_float(e.z);
             float tmp = 0.f, tv = 0.f, tv2 = 0.f;
             if (inb) {
-                const H3 h = ld3(hpad, tp + e.w);
+                const H3 h = ld3(rs, tpb + (uint32_t)e.w);
                 tmp = h.x - dz;
                 tv = h.y - dz;
                 tv2 = h.z - dz;
             }
             // the update on (tmp, tv, tv2) directly: a source value of 0 is not h - dz
-            st.f = fmaxf(st.f, tmp);
-            st.sh = st.f > at;
             if (s == 0) {
+                st.sh = tmp > at;
                 st.vs = (tv > at) && !(tv2 > at) && !st.sh;
                 float fv = tv - tmp;
                 if (fv <= 0.f) fv = 1000.f;
@@ -367,6 +371,7 @@ __device__ __forceinline__ uint32_t march_svf_checked(const float *__restrict__ 
                 st.vbb = false;
                 if (st.sh && !st.vs) active = false;
             } else {
+                st.sh = st.sh || (tmp > at);
                 st.vs = (st.vs || ((tv > at) && !(tv2 > at))) && !st.sh;
                 st.vbb = st.vbb || st.vs;
                 if ((st.sh && !st.vs) || (exit_ok && (!inb || (hmax - dz) <= at))) active = false;
@@ -394,11 +399,14 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
     const int c = (tile % tiles_x) * SVF_TILE + (wv & 1) * 8 + (ln & 7);
     const bool valid = (r < rows) && (c < cols);
     const int W = cols + 2 * SW_SVF_PAD;
-    const long tp = valid ? (long)(r + SW_SVF_PAD) * W + (c + SW_SVF_PAD) : 0;
+    const uint32_t nbytes = (uint32_t)((long)(rows + 2 * SW_SVF_PAD) * W * 12);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)hpad, (short)0, (int)nbytes,
+                                                                          0x00020000);
+    const uint32_t tpb = valid ? (uint32_t)(((long)(r + SW_SVF_PAD) * W + (c + SW_SVF_PAD)) * 12) : 0u;
     const long t = valid ? (long)r * cols + c : 0;
     float at = 0.f, veg2t = 0.f;
     if (valid) {
-        const H3 h = ld3(hpad, tp);
+        const H3 h = ld3(rs, tpb);
         at = h.x;
         veg2t = h.z;
     }
@@ -411,8 +419,8 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         // dz >= 0; float32 tan(90 deg) makes the zenith patch's dz negative
         const bool chk = checked || (counts[p] > 0 && __int_as_float(tab[offs[p]].z) < 0.f);
         const uint32_t m = chk
-            ? march_svf_checked(hpad, tp, r, c, rows, cols, at, veg2t, valid, tab + offs[p], counts[p], hmax)
-            : march_svf_fast(hpad, tp, at, veg2t, valid, tab + offs[p], counts[p], hmax);
+            ? march_svf_checked(rs, tpb, r, c, rows, cols, at, veg2t, valid, tab + offs[p], counts[p], hmax)
+            : march_svf_fast(rs, tpb, at, veg2t, valid, tab + offs[p], counts[p], hmax);
         const uint32_t bit = 1u << (p - p0);
         if (m & 1u) bsh |= bit;
         if (m & 2u) bvs |= bit;
@@ -1296,6 +1304,7 @@ int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int
     if (!hpack || !offs || !tab || !counts || !hpad || !masks || !stats)
         return set_err(SW_E_ARG, "sw_svf_march: NULL pointer");
     const long padded = (long)(rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD);
+    if (padded * 12 >= (1L << 31)) return set_err(SW_E_ARG, "sw_svf_march: tile too large for the padded buffer");
     hipLaunchKernelGGL(k_svf_counts, dim3(SW_NPATCH), dim3(64), 0, S(stream), (const int4 *)tab, offs, stats,
                        counts);
     rc = hip_check(hipGetLastError(), "k_svf_counts");

@@ -453,7 +453,7 @@ class HipSolver:
                         np.abs(np.diff(dx[a:b])).max(), np.abs(np.diff(dy[a:b])).max()) > 1)):
                     raise UnsupportedInput(f"patch {p}: march step longer than one cell")
             pitch = cols + 2 * L.SVF_PAD
-            tab = np.stack([dx, dy, dz.view(np.int32), dx * pitch + dy], axis=1).astype(np.int32)
+            tab = np.stack([dx, dy, dz.view(np.int32), (dx * pitch + dy) * 12], axis=1).astype(np.int32)
             self._tables[key] = (torch.from_numpy(np.ascontiguousarray(offs)).to(self.device),
                                  torch.from_numpy(np.ascontiguousarray(tab)).to(self.device))
         return self._tables[key]
