@@ -700,9 +700,9 @@ __device__ __forceinline__ float pow4(float x) {
     return (float)(d2 * d2);
 }
 
-// Walk-source tile in LDS: a 64 x 4 pixel tile plus the walk halo.
-#define FLUX_TX 64
-#define FLUX_TY 4
+// Walk-source tile in LDS: a 16 x 16 pixel tile plus the walk halo.
+#define FLUX_TX 16
+#define FLUX_TY 16
 
 __global__ void __launch_bounds__(256)
 k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
@@ -818,44 +818,53 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
     float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f;
     const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
     for (int w = 0; w < SW_MASK_WORDS; ++w) {
-        // this word's bits, shifted down one patch per iteration
-        uint32_t bsh = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
-        uint32_t bvs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
-        uint32_t bv1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
-        uint32_t bv2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
+        const uint32_t ash = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
+        const uint32_t avs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
+        const uint32_t av1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
+        const uint32_t av2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
+        // per-word patch booleans: tvs = vegsh == 0 or vbsh == 0; tsh = (1 - sh) * vbsh == 1;
+        // tsky = sh and vegsh
+        uint32_t btvs = ~avs | ~(av1 | av2), btsh = ~ash & av1, btsky = ash & avs;
+        uint32_t bsh = ash, bvs = avs;
         const int pend = min(w * 32 + 32, SW_NPATCH);
 #pragma unroll 4
         for (int p = w * 32; p < pend; ++p) {
-            const bool sh = bsh & 1u, vs = bvs & 1u, vb1 = bv1 & 1u, vb2 = bv2 & 1u;
-            bsh >>= 1; bvs >>= 1; bv1 >>= 1; bv2 >>= 1;
-            const bool tvs = !vs || !(vb1 || vb2);   // vegsh == 0 or vbsh == 0
-            const bool tsh = !sh && vb1;             // (1 - sh) * vbsh == 1
-            const bool tsky = sh && vs;
+            const float tvs = (float)(btvs & 1u), tsky = (float)(btsky & 1u);
+            const bool tsh = btsh & 1u;
+            const bool sh = bsh & 1u, vs = bvs & 1u;
+            btvs >>= 1; btsh >>= 1; btsky >>= 1; bsh >>= 1; bvs >>= 1;
             const sw_patch_k_t q = sp->pk[p];
             const sw_patch_l_t u = sp->pl[p];
+            // sum + c * b with b in {0, 1}: fma(b, c, sum) rounds once, like the reference
             if (DAY) {
                 const float diffsh = vs ? (sh ? 1.f : 0.f) : (sh ? d10 : d00);
                 aniLum = aniLum + diffsh * q.lv;
                 KsideD = KsideD + diffsh * q.lumchi * q.angl * q.ster;
-                Kref_veg = Kref_veg + (tvs ? q.k_sh : 0.f);
-                const float x = hsvf + q.yi_k;
-                Kref_sun = Kref_sun + ((tsh && x < q.sun_lt) ? q.k_sun : 0.f);
-                Kref_sh = Kref_sh + ((tsh && x > q.sun_gt) ? q.k_sh : 0.f);
+                Kref_veg = __builtin_fmaf(tvs, q.k_sh, Kref_veg);
             }
-            Ldown_sky = Ldown_sky + (tsky ? q.lsky_down : 0.f);
-            Lside_sky = Lside_sky + (tsky ? q.lsky_side : 0.f);
-            Lside_veg = Lside_veg + (tvs ? q.lv_side : 0.f);
-            Ldown_veg = Ldown_veg + (tvs ? q.lv_down : 0.f);
-            // sun-facing window: sunlit / shaded by the threshold test; elsewhere every
-            // shaded wall patch counts as shaded and none as sunlit
-            const float xl = hsvf + u.yi_l;
-            const bool ls = q.l_sun != 0;
-            const bool sunlit = tsh && ls && (xl < q.sun_lt);
-            const bool shaded = tsh && (!ls || (xl > q.sun_gt));
-            Lside_sun = Lside_sun + (sunlit ? u.ls_side : 0.f);
-            Lside_sh = Lside_sh + (shaded ? u.lh_side : 0.f);
-            Ldown_sun = Ldown_sun + (sunlit ? u.ls_down : 0.f);
-            Ldown_sh = Ldown_sh + (shaded ? u.lh_down : 0.f);
+            Ldown_sky = __builtin_fmaf(tsky, q.lsky_down, Ldown_sky);
+            Lside_sky = __builtin_fmaf(tsky, q.lsky_side, Lside_sky);
+            Lside_veg = __builtin_fmaf(tvs, q.lv_side, Lside_veg);
+            Ldown_veg = __builtin_fmaf(tvs, q.lv_down, Ldown_veg);
+            // wall-patch terms need tsh; a wave whose lanes all have tsh == 0 adds
+            // only zeros there and skips them
+            if (wave_any(tsh)) {
+                if (DAY) {
+                    const float x = hsvf + q.yi_k;
+                    Kref_sun = __builtin_fmaf((tsh && x < q.sun_lt) ? 1.f : 0.f, q.k_sun, Kref_sun);
+                    Kref_sh = __builtin_fmaf((tsh && x > q.sun_gt) ? 1.f : 0.f, q.k_sh, Kref_sh);
+                }
+                // sun-facing window: sunlit / shaded by the threshold test; elsewhere every
+                // shaded wall patch counts as shaded and none as sunlit
+                const float xl = hsvf + u.yi_l;
+                const bool ls = q.l_sun != 0;
+                const float sunlit = (tsh && ls && (xl < q.sun_lt)) ? 1.f : 0.f;
+                const float shaded = (tsh && (!ls || (xl > q.sun_gt))) ? 1.f : 0.f;
+                Lside_sun = __builtin_fmaf(sunlit, u.ls_side, Lside_sun);
+                Lside_sh = __builtin_fmaf(shaded, u.lh_side, Lside_sh);
+                Ldown_sun = __builtin_fmaf(sunlit, u.ls_down, Ldown_sun);
+                Ldown_sh = __builtin_fmaf(shaded, u.lh_down, Ldown_sh);
+            }
         }
     }
     o.aniLum = aniLum; o.KsideD = KsideD; o.Kref_sun = Kref_sun; o.Kref_sh = Kref_sh; o.Kref_veg = Kref_veg;
@@ -864,28 +873,32 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
 }
 
 // Ground-view walk of one azimuth (sunonsurface_2018a, solweig.py:136-288).
-// Booleans are kept as integers; the sums that add a constant (walls seen,
-// their longwave and albedo terms) are kept as counts k and read from the
-// sequential-sum tables s_tlw[k] / s_tal[k] afterwards, which is the same
-// float32 value the step-by-step additions produce. A source that left
-// the grid keeps the last in-grid temporaries (stale border); INTERIOR
-// tiles have every source in the grid.
+// Every boolean of the walk is a float 0 or 1, as in the reference: the
+// running minimum of building flags is a product, the "seen" flags are
+// maxima. Sums that add a constant (walls seen, their longwave and albedo
+// terms) are kept as counts k and read from the sequential-sum tables
+// s_tlw[k] / s_tal[k] afterwards, which is the same float32 value the
+// step-by-step additions produce. The other sums use fma(flag, value, sum)
+// with flag exactly 0 or 1: the product is exact, so the single rounding
+// equals the reference's "sum + value * flag". A source that left the grid
+// keeps the last in-grid temporaries (stale border); INTERIOR tiles have
+// every source in the grid. Once every lane of the wave has met a building
+// (flag product 0) the remaining steps only add to the counts, in closed
+// form.
 struct WalkAcc {
-    int kb, kw;                 // walls seen (bub), walls-or-blocked seen (bubw)
-    float wsh, wlupsh, walbsh, walbnosh;
+    float kb, kw;               // walls seen (bub), walls-or-blocked seen (bubw): exact small integers
+    float wlupsh, walbsh, walbnosh;
 };
 
 template <bool INTERIOR>
 __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, const float4 *__restrict__ s_dec,
                                              const sw_model_t *__restrict__ md, int j, int nsec, int nfirst,
-                                             int lbase, int lw, int r, int c, int rows, int cols, int fb,
+                                             int lbase, int lw, int r, int c, int rows, int cols, float fb,
                                              WalkAcc &acc, WalkAcc &snap) {
-    int f = fb;
-    int bub = 0, bubw = 0;
-    float ts = 0.f, tl = 0.f, ta = 0.f, tn = 0.f;
-    int tb = 0, tw = 0;
+    float f = fb, bub = 0.f, bubw = 0.f;
+    float tl = 0.f, ta = 0.f, tn = 0.f, tb = 0.f, tw = 0.f;
     bool live = true;
-    acc.kb = 0; acc.kw = 0; acc.wsh = 0.f; acc.wlupsh = 0.f; acc.walbsh = 0.f; acc.walbnosh = 0.f;
+    acc.kb = 0.f; acc.kw = 0.f; acc.wlupsh = 0.f; acc.walbsh = 0.f; acc.walbnosh = 0.f;
     const int32_t *wdx = md->walk_dx + j * SW_WALK_MAX, *wdy = md->walk_dy + j * SW_WALK_MAX;
 #define SW_WALK_STEP(S)                                                                      \
     {                                                                                        \
@@ -894,29 +907,33 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
             live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols); \
         if (INTERIOR || live) {                                                              \
             const uint2 e = s_src[lbase + ddx * lw + ddy];                                   \
-            const uint32_t fl = e.y;                                                         \
-            const float4 d = s_dec[(fl & 3u) | ((fl >> 2) & 0x1Cu)];                         \
-            tb = (int)((fl >> 3) & 1u);                                                      \
-            tw = (int)((fl >> 2) & 1u);                                                      \
-            ts = d.x; ta = d.y; tn = d.z;                                                    \
+            const float4 d = s_dec[e.y & 0x7Fu];                                             \
+            ta = d.x; tn = d.y; tb = d.z; tw = d.w;                                          \
             tl = __uint_as_float(e.x);                                                       \
         }                                                                                    \
-        f &= tb;                                                                             \
-        acc.wsh += f ? ts : 0.f;                                                             \
-        acc.wlupsh += f ? tl : 0.f;                                                          \
-        acc.walbsh += f ? ta : 0.f;                                                          \
-        acc.walbnosh += f ? tn : 0.f;                                                        \
-        bub |= tw & f;                                                                       \
+        f = f * tb;                                                                          \
+        acc.wlupsh = __builtin_fmaf(f, tl, acc.wlupsh);                                      \
+        acc.walbsh = __builtin_fmaf(f, ta, acc.walbsh);                                      \
+        acc.walbnosh = __builtin_fmaf(f, tn, acc.walbnosh);                                  \
+        bub = fmaxf(bub, tw * f);                                                            \
         acc.kb += bub;                                                                       \
-        bubw |= f ^ 1;                                                                       \
+        bubw = fmaxf(bubw, 1.f - f);                                                         \
         acc.kw += bubw;                                                                      \
     }
     int s = 0;
     for (; s < nfirst; ++s) SW_WALK_STEP(s)
     snap = acc;
 #pragma unroll 2
-    for (; s < nsec; ++s) SW_WALK_STEP(s)
+    for (; s < nsec; ++s) {
+        if (!wave_any(f != 0.f)) break;
+        SW_WALK_STEP(s)
+    }
 #undef SW_WALK_STEP
+    // every lane has f == 0 here (or the walk is complete): bubw is 1 and
+    // bub stays, for the nsec - s steps left
+    const float rest = (float)(nsec - s);
+    acc.kb += bub * rest;
+    acc.kw += rest;
 }
 
 // WALK = false instantiates a night-only kernel (no walk tile, no shortwave
@@ -932,7 +949,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             int rows, int cols) {
     extern __shared__ uint2 s_src[];
     __shared__ float s_alb[SW_NCLASS];
-    __shared__ float4 s_dec[32];                  // walk source decode: {shadow, alb * shadow, alb}
+    __shared__ float4 s_dec[128];                 // walk source decode by flag bits 0-6: {alb * shadow, alb, ground, sunwall}
     __shared__ float s_tlw[SW_WALK_MAX + 1];      // k-fold sequential sums of lwall
     __shared__ float s_tal[SW_WALK_MAX + 1];      // k-fold sequential sums of albedo_b
     const sw_step_t *sp = steps + step;
@@ -942,7 +959,9 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= tiles_x * tiles_y) return;  // whole block idle (uniform)
     const int r0 = (tile / tiles_x) * FLUX_TY, c0 = (tile % tiles_x) * FLUX_TX;
-    const int tx = threadIdx.x % FLUX_TX, ty = threadIdx.x / FLUX_TX;
+    // each wave covers an 8 x 8 square of the 16 x 16 tile
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int ty = (wv >> 1) * 8 + (ln >> 3), tx = (wv & 1) * 8 + (ln & 7);
     const int r = r0 + ty, c = c0 + tx;
     const bool valid = (r < rows) && (c < cols);
     const long n = (long)rows * cols;
@@ -963,13 +982,14 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             if ((unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols) v = src[(long)gr * cols + gc];
             s_src[i] = v;
         }
-        if (threadIdx.x < 32) {
-            const int shb = threadIdx.x & 1, vsb = (threadIdx.x >> 1) & 1, k = threadIdx.x >> 2;
+        if (threadIdx.x < 128) {
+            const int shb = threadIdx.x & 1, vsb = (threadIdx.x >> 1) & 1, k = (threadIdx.x >> 4) & 7;
             const float al = md->alb_class[k];
             const float shd = (float)shb - (1.f - (float)vsb) * omp;
-            s_dec[threadIdx.x] = make_float4(shd, al * shd, al, 0.f);
+            s_dec[threadIdx.x] = make_float4(al * shd, al, (float)((threadIdx.x >> 3) & 1),
+                                             (float)((threadIdx.x >> 2) & 1));
         }
-        if (threadIdx.x == 32) {
+        if (threadIdx.x == 128) {
             float a1 = 0.f, a2 = 0.f;
             const float lwall = sp->lwall, albb = md->albedo_b;
             s_tlw[0] = 0.f; s_tal[0] = 0.f;
@@ -1013,7 +1033,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         const float fp1 = first + 1.f, sp1 = second + 1.f;
         const uint32_t wf = walkface[t];
         const float nb = b_t * -1.f + 1.f;
-        const int fb = (b_t == 1.f) ? 1 : 0;     // buildings is 0/1
+        const float fb = b_t;                    // buildings is 0/1
         const int lbase = (ty + halo) * lw + (tx + halo);
         const int wh = md->walk_halo;
         const bool interior = (r0 >= wh) && (r0 + FLUX_TY + wh <= rows) && (c0 >= wh) && (c0 + FLUX_TX + wh <= cols);
@@ -1023,10 +1043,11 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
                 walk_azimuth<true>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
             else
                 walk_azimuth<false>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
-            const float wwall = (float)A.kb, wlwall = s_tlw[A.kb], walbwall = s_tal[A.kb];
-            const float walbwallnosh = s_tal[A.kw];
-            const float f_wwall = (float)F.kb, f_wlwall = s_tlw[F.kb], f_walbwall = s_tal[F.kb];
-            const float f_walbwallnosh = s_tal[F.kw];
+            const int kb = (int)A.kb, kw = (int)A.kw, fkb = (int)F.kb, fkw = (int)F.kw;
+            const float wwall = A.kb, wlwall = s_tlw[kb], walbwall = s_tal[kb];
+            const float walbwallnosh = s_tal[kw];
+            const float f_wwall = F.kb, f_wlwall = s_tlw[fkb], f_walbwall = s_tal[fkb];
+            const float f_walbwallnosh = s_tal[fkw];
             // keep == 1  <=>  wwall == second and facesh == 0 (bit j of walkface)
             const bool keep1 = (wwall == second) && ((wf >> j) & 1u);
             const float wsi1 = (f_wwall > 0.f) ? 1.f : 0.f;
