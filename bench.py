@@ -119,7 +119,7 @@ def main():
         solver.solve_device(ds, met, sun, site.latitude, out=out)
     torch.cuda.synchronize()
     ds.check_flags()
-    timer = KernelTimer()
+    timer = KernelTimer(reserve=args.steps * (6 + 3 * T))
     solver.timer = timer
     log(f"timing {args.steps} steps")
     solver.plan_ms.clear()

@@ -264,6 +264,13 @@ int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const 
 int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
                  const int32_t *sun_tab, const int32_t *is_day, int t0, int t1, float *out, int out_mask,
                  int walk_halo, int reset_state, void *stream);
+/* sw_solve_day recording events (created with sw_event_create) around the
+ * launches of step i: events[3 * (i - t0) + 0] before the sun march (or the
+ * flux step of a night), + 1 between sun march and flux step, + 2 after the
+ * flux step; for per-kernel timing without host work between launches. */
+int sw_solve_day_timed(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                       const int32_t *sun_tab, const int32_t *is_day, void **events, int t0, int t1, float *out,
+                       int out_mask, int walk_halo, int reset_state, void *stream);
 
 /* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,

@@ -122,6 +122,7 @@ SIGNATURES = {
     "sw_graph_destroy": (_I, [_VP]),
     "sw_svf_sweep": (_I, [_VP] * 14 + [_I, _I, _VP]),
     "sw_solve_day": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP]),
+    "sw_solve_day_timed": (_I, [_VP] * 6 + [_I, _I, _VP, _I, _I, _I, _VP]),
     "sw_pack_heights": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP]),
     "sw_compose": (_I, [_VP] * 11 + [_I, _I, _VP]),
     "sw_svf_march": (_I, [_VP] * 7 + [_I, _I, _VP]),

@@ -447,6 +447,32 @@ __device__ __forceinline__ void load_masks(const uint32_t *__restrict__ masks, l
     }
 }
 
+// one patch of the fold with NA annuli (fully unrolled, weights in SGPRs)
+template <int NA>
+__device__ __forceinline__ void fold_patch(float sh, float vs, float vb, const float *__restrict__ wi,
+                                           const float *__restrict__ wa, int dm, float &s0, float &v0, float &b0,
+                                           float &s1, float &v1, float &b1, float &s2, float &v2, float &b2,
+                                           float &s3, float &v3, float &b3, float &s4, float &v4, float &b4) {
+    float W[NA], A[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) { W[k] = wi[k]; A[k] = wa[k]; }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        s0 = __builtin_fmaf(sh, W[k], s0); v0 = __builtin_fmaf(vs, W[k], v0); b0 = __builtin_fmaf(vb, W[k], b0);
+    }
+#define SW_FOLD_DIR(BIT, S, V, B)                                                                   \
+    if (dm & BIT) {                                                                              \
+        _Pragma("unroll") for (int k = 0; k < NA; ++k) {                                          \
+            S = __builtin_fmaf(sh, A[k], S); V = __builtin_fmaf(vs, A[k], V); B = __builtin_fmaf(vb, A[k], B); \
+        }                                                                                        \
+    }
+    SW_FOLD_DIR(1, s1, v1, b1)
+    SW_FOLD_DIR(2, s2, v2, b2)
+    SW_FOLD_DIR(4, s3, v3, b3)
+    SW_FOLD_DIR(8, s4, v4, b4)
+#undef SW_FOLD_DIR
+}
+
 // ---------------------------------------------------------------------------
 // SVF fold, literal per-annulus order (bit-exact with the reference's sums)
 // ---------------------------------------------------------------------------
@@ -462,27 +488,28 @@ k_svf_fold(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
     float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f;
+    // sh, vegsh are 0/1 and vbsh is 0/1/2, so every weight x mask product is
+    // exact and fma(mask, weight, sum) rounds once, like "sum + weight * mask".
+    // Each sum keeps the reference's (patch, annulus) order; the direction
+    // windows are wave-uniform per patch, so each runs as its own loop.
     for (int p = 0; p < SW_NPATCH; ++p) {
         const int w = p >> 5;
         const uint32_t bit = 1u << (p & 31);
         const float sh = (msh[w] & bit) ? 1.f : 0.f;
         const float vs = (mvs[w] & bit) ? 1.f : 0.f;
         const float vb = (mv1[w] & bit) ? 1.f : ((mv2[w] & bit) ? 2.f : 0.f);
-        const int i = ring[p], dm = dirm[p], na = nann[i];
-        for (int k = 0; k < na; ++k) {
-            const float wi = w_iso[i * 12 + k], wa = w_aniso[i * 12 + k];
-            s0 = s0 + wi * sh;
-            const float wsh = wa * sh;
-            if (dm & 1) s1 = s1 + wsh;
-            if (dm & 2) s2 = s2 + wsh;
-            if (dm & 4) s3 = s3 + wsh;
-            if (dm & 8) s4 = s4 + wsh;
-            v0 = v0 + wi * vs;
-            b0 = b0 + wi * vb;
-            if (dm & 1) { v1 = v1 + wa * vs; b1 = b1 + wa * vb; }
-            if (dm & 2) { v2 = v2 + wa * vs; b2 = b2 + wa * vb; }
-            if (dm & 4) { v3 = v3 + wa * vs; b3 = b3 + wa * vb; }
-            if (dm & 8) { v4 = v4 + wa * vs; b4 = b4 + wa * vb; }
+        const int i = ring[p], dm = dirm[p];
+        const float *wi = w_iso + i * 12, *wa = w_aniso + i * 12;
+        if (nann[i] == 12) {
+            fold_patch<12>(sh, vs, vb, wi, wa, dm, s0, v0, b0, s1, v1, b1, s2, v2, b2, s3, v3, b3, s4, v4, b4);
+        } else {
+            for (int k = 0; k < nann[i]; ++k) {
+                s0 = __builtin_fmaf(sh, wi[k], s0); v0 = __builtin_fmaf(vs, wi[k], v0); b0 = __builtin_fmaf(vb, wi[k], b0);
+            }
+            if (dm & 1) for (int k = 0; k < nann[i]; ++k) { s1 = __builtin_fmaf(sh, wa[k], s1); v1 = __builtin_fmaf(vs, wa[k], v1); b1 = __builtin_fmaf(vb, wa[k], b1); }
+            if (dm & 2) for (int k = 0; k < nann[i]; ++k) { s2 = __builtin_fmaf(sh, wa[k], s2); v2 = __builtin_fmaf(vs, wa[k], v2); b2 = __builtin_fmaf(vb, wa[k], b2); }
+            if (dm & 4) for (int k = 0; k < nann[i]; ++k) { s3 = __builtin_fmaf(sh, wa[k], s3); v3 = __builtin_fmaf(vs, wa[k], v3); b3 = __builtin_fmaf(vb, wa[k], b3); }
+            if (dm & 8) for (int k = 0; k < nann[i]; ++k) { s4 = __builtin_fmaf(sh, wa[k], s4); v4 = __builtin_fmaf(vs, wa[k], v4); b4 = __builtin_fmaf(vb, wa[k], b4); }
         }
     }
     const float cst = 3.0459e-004f;

@@ -225,9 +225,9 @@ int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const 
     return sw_svf_fold(masks, vegdsm2, w_iso, w_aniso, ring, nann, dirm, fields, rows, cols, stream);
 }
 
-int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model, const int32_t *sun_tab,
-                 const int32_t *is_day, int t0, int t1, float *out, int out_mask, int walk_halo, int reset_state,
-                 void *stream) {
+static int solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                     const int32_t *sun_tab, const int32_t *is_day, int t0, int t1, float *out, int out_mask,
+                     int walk_halo, int reset_state, void **events, void *stream) {
     if (!tile || !steps || !model || !sun_tab || !is_day || !out)
         return swi_set_err(SW_E_ARG, "sw_solve_day: NULL pointer");
     int rc = swi_check_shape(tile->rows, tile->cols, "sw_solve_day");
@@ -242,9 +242,18 @@ int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t
         if (rc) return rc;
     }
     for (int i = t0; i < t1; ++i) {
+        void **ev = events ? events + 3 * (i - t0) : nullptr;
+        if (ev) {
+            rc = swi_hip_check(hipEventRecord((hipEvent_t)ev[0], swi_stream(stream)), "hipEventRecord");
+            if (rc) return rc;
+        }
         if (is_day[i - t0]) {
             rc = sw_sun_step(tile->hpack, tile->walls, tile->aspect_rad, tile->cls, tile->buildings, steps, model,
                              sun_tab, tile->stats, tile->src, i, rows, cols, stream);
+            if (rc) return rc;
+        }
+        if (ev) {
+            rc = swi_hip_check(hipEventRecord((hipEvent_t)ev[1], swi_stream(stream)), "hipEventRecord");
             if (rc) return rc;
         }
         rc = sw_flux_step(tile->masks, tile->src, tile->svfbuveg, tile->tan_asvf, tile->tan_svfalfa,
@@ -252,8 +261,27 @@ int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t
                           out + (long)(i - t0) * stride, out_mask, i, is_day[i - t0] ? walk_halo : -1, rows,
                           cols, stream);
         if (rc) return rc;
+        if (ev) {
+            rc = swi_hip_check(hipEventRecord((hipEvent_t)ev[2], swi_stream(stream)), "hipEventRecord");
+            if (rc) return rc;
+        }
     }
     return SW_OK;
+}
+
+int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model, const int32_t *sun_tab,
+                 const int32_t *is_day, int t0, int t1, float *out, int out_mask, int walk_halo, int reset_state,
+                 void *stream) {
+    return solve_day(tile, steps, model, sun_tab, is_day, t0, t1, out, out_mask, walk_halo, reset_state, nullptr,
+                     stream);
+}
+
+int sw_solve_day_timed(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                       const int32_t *sun_tab, const int32_t *is_day, void **events, int t0, int t1, float *out,
+                       int out_mask, int walk_halo, int reset_state, void *stream) {
+    if (!events) return swi_set_err(SW_E_ARG, "sw_solve_day_timed: NULL events");
+    return solve_day(tile, steps, model, sun_tab, is_day, t0, t1, out, out_mask, walk_halo, reset_state, events,
+                     stream);
 }
 
 }  // extern "C"

@@ -384,34 +384,72 @@ def upload_inputs(tile: TileInputs, device):
 
 
 class KernelTimer:
-    """Records HIP events around each launch on the launch stream."""
+    """Per-launch kernel times from HIP events recorded on the launch stream.
 
-    def __init__(self):
-        self.records = []
+    Events come from a pool created up front (reserve), so timing adds no
+    event creation between launches; a day loop records its events inside
+    the library (sw_solve_day_timed).
+    """
+
+    def __init__(self, reserve: int = 0):
+        self.records = []          # (name, event_before, event_after)
+        self._pool = []
+        self._next = 0
+        self.reserve(reserve)
+
+    def reserve(self, n: int):
+        while len(self._pool) < n:
+            e = ctypes.c_void_p()
+            L.call("sw_event_create", ctypes.byref(e))
+            self._pool.append(e.value)
+
+    def _take(self, k: int):
+        self.reserve(self._next + k)
+        evs = self._pool[self._next:self._next + k]
+        self._next += k
+        return evs
 
     def span(self, name, stream=None):
         timer = self
+        sp = _stream_ptr(stream)
 
         class _Span:
             def __enter__(self_):
-                self_.e0 = torch.cuda.Event(enable_timing=True)
-                self_.e1 = torch.cuda.Event(enable_timing=True)
-                self_.e0.record(stream)
+                self_.e0, self_.e1 = timer._take(2)
+                L.call("sw_event_record", self_.e0, sp)
                 return self_
 
             def __exit__(self_, *exc):
-                self_.e1.record(stream)
+                L.call("sw_event_record", self_.e1, sp)
                 timer.records.append((name, self_.e0, self_.e1))
                 return False
         return _Span()
 
+    def day_events(self, is_day):
+        """ctypes array of 3 events per step for sw_solve_day_timed, registered for the summary."""
+        evs = self._take(3 * len(is_day))
+        for i, d in enumerate(is_day):
+            e0, e1, e2 = evs[3 * i:3 * i + 3]
+            if d:
+                self.records.append(("sun_step", e0, e1))
+            self.records.append(("flux_day" if d else "flux_night", e1, e2))
+        return (ctypes.c_void_p * len(evs))(*evs)
+
     def summary(self):
         """{name: (launches, total_ms)} (call after synchronising)."""
         out = {}
+        ms = ctypes.c_float()
         for name, e0, e1 in self.records:
+            L.call("sw_event_elapsed", e0, e1, ctypes.byref(ms))
             n, t = out.get(name, (0, 0.0))
-            out[name] = (n + 1, t + e0.elapsed_time(e1))
+            out[name] = (n + 1, t + float(ms.value))
         return out
+
+    def close(self):
+        for e in self._pool:
+            L.call("sw_event_destroy", e)
+        self._pool = []
+        self._next = 0
 
 
 class _NoTimer:
@@ -436,6 +474,7 @@ class HipSolver:
         self._tables = {}
         self.timer = _NoTimer()
         self.plan_ms: list[float] = []      # host planning time of each solve
+        self._staging = None
 
     # -- SVF ---------------------------------------------------------------
     def upload(self, tile: TileInputs) -> DeviceScene:
@@ -488,14 +527,38 @@ class HipSolver:
         return cubes
 
     # -- day ---------------------------------------------------------------
-    def upload_plan(self, plan: DayPlan):
-        """Copy the day's tables to HBM from pinned staging, without a host sync."""
-        dev = self.device
+    def upload_plan(self, plan: DayPlan, stream=None):
+        """Copy the day's tables to HBM from persistent pinned staging, without a host sync.
 
-        def up(a):
-            return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
-
-        return dict(steps=up(plan.steps), sun_tab=up(plan.sun_tab), model=up(plan.model.copy()))
+        The staging buffer is rewritten only after the event of its previous
+        copy has completed; the device buffers are reused in stream order
+        (the copy runs after every kernel queued before it on the stream)."""
+        parts = [np.ascontiguousarray(plan.steps).view(np.uint8),
+                 np.ascontiguousarray(plan.sun_tab).view(np.uint8).reshape(-1),
+                 np.ascontiguousarray(plan.model).view(np.uint8).reshape(-1)]
+        sizes = [p.nbytes for p in parts]
+        offs = np.concatenate([[0], np.cumsum([(z + 255) // 256 * 256 for z in sizes])]).astype(int)
+        total = int(offs[-1])
+        st = self._staging
+        if st is None or st["host"].numel() < total:
+            if st is not None:
+                st["event"].synchronize()
+            cap = max(total, 1 << 20)
+            st = dict(host=torch.empty(cap, dtype=torch.uint8, pin_memory=True),
+                      dev=torch.empty(cap, dtype=torch.uint8, device=self.device), event=torch.cuda.Event())
+            st["event"].record()
+            self._staging = st
+        st["event"].synchronize()          # the previous copy out of the staging buffer has run
+        host = st["host"].numpy()
+        for p, o in zip(parts, offs):
+            host[o:o + p.nbytes] = p
+        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        with torch.cuda.stream(s):
+            st["dev"][:total].copy_(st["host"][:total], non_blocking=True)
+            st["event"].record(s)
+        d = st["dev"]
+        return dict(steps=d[offs[0]:offs[0] + sizes[0]], sun_tab=d[offs[1]:offs[1] + sizes[1]],
+                    model=d[offs[2]:offs[2] + sizes[2]])
 
     def tile_static(self, ds: DeviceScene, dplan, stream=None):
         P = L.ptr
@@ -503,7 +566,7 @@ class HipSolver:
                P(ds.tan_asvf), P(ds.tan_sa), P(ds.walkface), ds.rows, ds.cols, _stream_ptr(stream))
 
     def run_day(self, ds: DeviceScene, plan: DayPlan, dplan, outputs=("tmrt", "utci", "shadow"),
-                out=None, stream=None, reset_state=True):
+                out=None, stream=None, reset_state=True, per_launch=False):
         """Enqueue the time loop; returns the device output tensor (T, k, rows, cols)."""
         mask = 0
         for name in outputs:
@@ -516,11 +579,16 @@ class HipSolver:
         P = L.ptr
         if reset_state:
             ds.state.zero_()
-        if isinstance(self.timer, _NoTimer):
+        is_day = (ctypes.c_int32 * T)(*[int(x) for x in plan.is_day])
+        if not per_launch:
             # one library call enqueues the whole day
-            is_day = (ctypes.c_int32 * T)(*[int(x) for x in plan.is_day])
-            L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
-                   P(dplan["sun_tab"]), is_day, 0, T, P(out), mask, plan.walk_halo, 0, sp)
+            if isinstance(self.timer, KernelTimer):
+                evs = self.timer.day_events(plan.is_day)
+                L.call("sw_solve_day_timed", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
+                       P(dplan["sun_tab"]), is_day, evs, 0, T, P(out), mask, plan.walk_halo, 0, sp)
+            else:
+                L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
+                       P(dplan["sun_tab"]), is_day, 0, T, P(out), mask, plan.walk_halo, 0, sp)
             return out
         # per-launch path (kernel timing spans around each launch)
         stride = k * ds.n * 4

@@ -139,16 +139,32 @@ def test_svf_sweep_one_call_matches_split(L, day_case):
 
 
 def _per_launch_day(day_case, ds, plan, dplan):
-    from solweig_project_amd.solver import KernelTimer
     solver = day_case["solver"]
-    old = solver.timer
-    solver.timer = KernelTimer()        # forces the per-launch path
-    try:
-        out = solver.run_day(ds, plan, dplan, ("tmrt", "utci", "shadow", "kdown", "lup"))
-    finally:
-        solver.timer = old
+    out = solver.run_day(ds, plan, dplan, ("tmrt", "utci", "shadow", "kdown", "lup"), per_launch=True)
     torch.cuda.synchronize()
     return out.clone()
+
+
+def test_timed_day_matches_and_reports_every_launch(L, day_case):
+    _say("sw_solve_day_timed")
+    from solweig_project_amd.solver import KernelTimer
+    solver = day_case["solver"]
+    ds, plan, dplan = _prepared(day_case)
+    ref = _per_launch_day(day_case, ds, plan, dplan)
+    old = solver.timer
+    solver.timer = KernelTimer(reserve=3 * plan.n_steps)
+    try:
+        out = solver.run_day(ds, plan, dplan, ("tmrt", "utci", "shadow", "kdown", "lup"))
+        torch.cuda.synchronize()
+        summ = solver.timer.summary()
+    finally:
+        solver.timer.close()
+        solver.timer = old
+    assert torch.equal(torch.nan_to_num(out, nan=-1e30), torch.nan_to_num(ref, nan=-1e30))
+    nday = sum(plan.is_day)
+    assert summ["sun_step"][0] == nday and summ["flux_day"][0] == nday
+    assert summ["flux_night"][0] == plan.n_steps - nday
+    assert all(t > 0 for _, t in summ.values())
 
 
 def test_solve_day_one_call_matches_per_launch(L, day_case):
