@@ -5,9 +5,10 @@ One "step" is one full solve of the 1000 x 1000 x 24 h synthetic tile
 (scene compose, 153-patch SVF sweep, 24 hourly steps of sun march, ground
 view factors, flux assembly, Tmrt and UTCI) with the input rasters already
 resident in HBM and the outputs (16 SVF fields, per-step Tmrt, UTCI and
-shadow) left in HBM. The host scalar chain of the day (clearness index,
-Perez luminance, tables) runs inside the timed region, overlapped with the
-SVF sweep. Prints one JSON line.
+shadow) left in HBM. The host scalar chain of each solve's day (clearness
+index, Perez luminance, sun tables, step records) is computed inside the
+timed region, in a planner process one solve ahead of the GPU. Prints one
+JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=1000)
     ap.add_argument("--cols", type=int, default=1000)
@@ -49,7 +50,8 @@ def log(msg):
 
 
 PROFILE_KEYS = {"svf_march": "k_svf_march", "svf_fold": "k_svf_fold", "compose": "k_compose",
-                "flux_day": "k_flux_step [day]", "flux_night": "k_flux_step [night]", "sun_step": "k_sun_step"}
+                "flux_day": "k_flux_step<true> (day)", "flux_night": "k_flux_step<false> (night)",
+                "sun_step": "k_sun_step"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_full_solve_1000x1000x24.md"
 
 
@@ -95,6 +97,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
               "design-only (tiles with read-only halos, no collectives)", file=sys.stderr)
         sys.exit(2)
+    from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
+    pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used
     import numpy as np
     import torch
     from solweig_project_amd import skyvault as SV
@@ -114,9 +118,10 @@ def main():
     inputs = upload_inputs(tile, dev)
     ds = DeviceScene(tile, dev, inputs)
     out = torch.empty((T, 3, rows, cols), dtype=torch.float32, device=dev)
+    job = Job(met, sun, site.latitude)
     log("warmup")
-    for _ in range(args.warmup):
-        solver.solve_device(ds, met, sun, site.latitude, out=out)
+    pipeline.warm(ds.tile, job)
+    solve_many(solver, ds, [job] * max(args.warmup, 1), pipeline, out=out)
     torch.cuda.synchronize()
     ds.check_flags()
     timer = KernelTimer(reserve=args.steps * (6 + 3 * T))
@@ -125,12 +130,16 @@ def main():
     solver.plan_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solver.solve_device(ds, met, sun, site.latitude, out=out)
+    # every solve plans its own day (in the planner process, one solve ahead)
+    solve_many(solver, ds, [job] * args.steps, pipeline, out=out)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solver.timer = _NoTimer()
-    plan_ms = sum(solver.plan_ms) / max(len(solver.plan_ms), 1)
+    marks = np.array(getattr(solver, "pipeline_marks", [[0, 0, 0, 0]])) * 1e3
+    log("host ms per solve (mean): svf enqueue %.2f, wait for plan %.2f, upload %.2f, day enqueue %.2f"
+        % tuple(marks.mean(axis=0)))
+    pipeline.close()
+    plan_ms = sum(solver.plan_ms) / max(len(solver.plan_ms), 1)   # per solve, in the planner process
     ms = (t1 - t0) / args.steps * 1e3
     value = rows * cols * T / (ms / 1e3) / 1e6
     ksum = timer.summary()

@@ -217,3 +217,32 @@ def test_solve_day_rejects_bad_arguments(L, day_case):
     with pytest.raises(L.HipError, match="out_mask"):
         L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
                P(dplan["sun_tab"]), is_day, 0, 1, P(ds.state), 0, plan.walk_halo, 0, None)
+
+
+class _InlinePlanner:
+    """solve_many's planner interface, planning in this process (no child process after GPU init)."""
+
+    class _Done:
+        def __init__(self, v):
+            self.v = v
+
+        def get(self, timeout=None):
+            return self.v
+
+    def submit(self, tile, job):
+        from solweig_project_amd.solver import plan_day
+        return self._Done(plan_day(tile, job.met, job.sun, job.latitude, time_start=job.time_start,
+                                   time_stop=job.time_stop))
+
+
+def test_pipelined_solves_match_solve_device(L, day_case):
+    _say("solve_many vs solve_device")
+    from solweig_project_amd.pipeline import Job, solve_many
+    solver, tile = day_case["solver"], day_case["tile"]
+    ds = solver.upload(tile)
+    _, ref = solver.solve_device(ds, day_case["met"], day_case["sun"], day_case["lat"])
+    ref = ref.clone()
+    job = Job(day_case["met"], day_case["sun"], day_case["lat"])
+    out = solve_many(solver, ds, [job, job], _InlinePlanner())
+    torch.cuda.synchronize()
+    assert torch.equal(torch.nan_to_num(out, nan=-1e30), torch.nan_to_num(ref, nan=-1e30))
