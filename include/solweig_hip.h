@@ -228,7 +228,8 @@ int sw_tile_static(const float *fields, const float *walls, const float *aspect_
 
 /* Sun march + wall tail + ground-view-factor walk sources for one day step
  * (shadowingfunction_wallheight_23, solweig.py:1079-1224; sunonsurface
- * prologue, solweig.py:121-134). src: 2 x uint32 per pixel {Lup bits, flags}.
+ * prologue, solweig.py:121-134). src: 2 x uint32 per pixel {Lup bits, flags}
+ * with flag bits 4..10 = {sh, vegsh, sunwall, ground, land-cover class x3}.
  * sun_tab: int4 {dx, dy, dz bits, dzprev bits} per march step. */
 int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
                 const uint8_t *cls, const float *buildings, const sw_step_t *steps,

@@ -784,8 +784,10 @@ k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
     const float lup = se * pow4(x) - se * sp->tak4;
     const float q = wallsun / w * b;
     const uint32_t sunwall = (q == 1.f) ? 1u : 0u;
-    const uint32_t flags = (uint32_t)shv | ((uint32_t)vso << 1) | (sunwall << 2) |
-                           ((b == 1.f ? 1u : 0u) << 3) | ((uint32_t)k << 4);
+    // flag bits 4-10 {sh, vegsh, sunwall, ground, class x3}: the walk reads
+    // (flags & 0x7F0) as the byte offset of its 16-byte decode entry
+    const uint32_t flags = ((uint32_t)shv | ((uint32_t)vso << 1) | (sunwall << 2) |
+                            ((b == 1.f ? 1u : 0u) << 3) | ((uint32_t)k << 4)) << 4;
     src[t] = make_uint2(__float_as_uint(lup), flags);
 }
 
@@ -901,8 +903,9 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
 
 // Ground-view walk of one azimuth (sunonsurface_2018a, solweig.py:136-288).
 // Every boolean of the walk is a float 0 or 1, as in the reference: the
-// running minimum of building flags is a product, the "seen" flags are
-// maxima. Sums that add a constant (walls seen, their longwave and albedo
+// running minimum of building flags is a product f, the "wall seen" flag is
+// a maximum, and "wall or building seen" (bubw) equals 1 - f because f never
+// rises again, so its count is the step count minus the sum of f. Sums that add a constant (walls seen, their longwave and albedo
 // terms) are kept as counts k and read from the sequential-sum tables
 // s_tlw[k] / s_tal[k] afterwards, which is the same float32 value the
 // step-by-step additions produce. The other sums use fma(flag, value, sum)
@@ -913,7 +916,8 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
 // (flag product 0) the remaining steps only add to the counts, in closed
 // form.
 struct WalkAcc {
-    float kb, kw;               // walls seen (bub), walls-or-blocked seen (bubw): exact small integers
+    float kb;                   // steps with a wall seen (bub): an exact small integer
+    float fs;                   // sum of the flag product f; steps with bubw = steps - fs
     float wlupsh, walbsh, walbnosh;
 };
 
@@ -922,10 +926,10 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
                                              const sw_model_t *__restrict__ md, int j, int nsec, int nfirst,
                                              int lbase, int lw, int r, int c, int rows, int cols, float fb,
                                              WalkAcc &acc, WalkAcc &snap) {
-    float f = fb, bub = 0.f, bubw = 0.f;
+    float f = fb, bub = 0.f;
     float tl = 0.f, ta = 0.f, tn = 0.f, tb = 0.f, tw = 0.f;
     bool live = true;
-    acc.kb = 0.f; acc.kw = 0.f; acc.wlupsh = 0.f; acc.walbsh = 0.f; acc.walbnosh = 0.f;
+    acc.kb = 0.f; acc.fs = 0.f; acc.wlupsh = 0.f; acc.walbsh = 0.f; acc.walbnosh = 0.f;
     const int32_t *wdx = md->walk_dx + j * SW_WALK_MAX, *wdy = md->walk_dy + j * SW_WALK_MAX;
 #define SW_WALK_STEP(S)                                                                      \
     {                                                                                        \
@@ -934,7 +938,7 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
             live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols); \
         if (INTERIOR || live) {                                                              \
             const uint2 e = s_src[lbase + ddx * lw + ddy];                                   \
-            const float4 d = s_dec[e.y & 0x7Fu];                                             \
+            const float4 d = *(const float4 *)((const char *)s_dec + (e.y & 0x7F0u));        \
             ta = d.x; tn = d.y; tb = d.z; tw = d.w;                                          \
             tl = __uint_as_float(e.x);                                                       \
         }                                                                                    \
@@ -944,8 +948,7 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
         acc.walbnosh = __builtin_fmaf(f, tn, acc.walbnosh);                                  \
         bub = fmaxf(bub, tw * f);                                                            \
         acc.kb += bub;                                                                       \
-        bubw = fmaxf(bubw, 1.f - f);                                                         \
-        acc.kw += bubw;                                                                      \
+        acc.fs += f;                                                                         \
     }
     int s = 0;
     for (; s < nfirst; ++s) SW_WALK_STEP(s)
@@ -956,11 +959,9 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
         SW_WALK_STEP(s)
     }
 #undef SW_WALK_STEP
-    // every lane has f == 0 here (or the walk is complete): bubw is 1 and
-    // bub stays, for the nsec - s steps left
-    const float rest = (float)(nsec - s);
-    acc.kb += bub * rest;
-    acc.kw += rest;
+    // every lane has f == 0 here (or the walk is complete): bub stays and f
+    // adds nothing for the nsec - s steps left
+    acc.kb += bub * (float)(nsec - s);
 }
 
 // WALK = false instantiates a night-only kernel (no walk tile, no shortwave
@@ -1048,7 +1049,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
         const uint2 me = src[t];
         const uint32_t fl_t = me.y;
         const float lup_t = __uint_as_float(me.x);
-        shadow = (float)(fl_t & 1u) - (1.f - (float)((fl_t >> 1) & 1u)) * omp;
+        shadow = (float)((fl_t >> 4) & 1u) - (1.f - (float)((fl_t >> 5) & 1u)) * omp;
         const float alb_t = s_alb[k_t];
         // ---- ground view factors (gvf_2018a): 18 azimuths x nsec steps ----
         float gL = 0.f, gA = 0.f, gN = 0.f, gLE = 0.f, gAE = 0.f, gNE = 0.f, gLS = 0.f, gAS = 0.f, gNS = 0.f;
@@ -1070,7 +1071,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
                 walk_azimuth<true>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
             else
                 walk_azimuth<false>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
-            const int kb = (int)A.kb, kw = (int)A.kw, fkb = (int)F.kb, fkw = (int)F.kw;
+            const int kb = (int)A.kb, kw = nsec - (int)A.fs, fkb = (int)F.kb, fkw = nfirst - (int)F.fs;
             const float wwall = A.kb, wlwall = s_tlw[kb], walbwall = s_tal[kb];
             const float walbwallnosh = s_tal[kw];
             const float f_wwall = F.kb, f_wlwall = s_tlw[fkb], f_walbwall = s_tal[fkb];
