@@ -196,6 +196,25 @@ def main():
                torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         log(f"copy calibration: k_pack_heights read {12 * m} B, wrote {16 * m} B")
+    # tile inputs on the device: wall heights + wall aspect of the same DSM (not part of the metric)
+    from solweig_project_amd import walls as WG
+    dsm_dev = torch.from_numpy(site.dsm).to(dev)
+    WG.walls_and_aspect(dsm_dev, tile.scale)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = 5
+    wa_ms = [0.0, 0.0]
+    for _ in range(reps):
+        ev[0].record()
+        w_dev = WG.findwalls(dsm_dev)
+        ev[1].record()
+        WG.filter1Goodwin_as_aspect_v3(w_dev, tile.scale, dsm_dev)
+        ev[2].record()
+        torch.cuda.synchronize()
+        wa_ms[0] += ev[0].elapsed_time(ev[1]) / reps
+        wa_ms[1] += ev[1].elapsed_time(ev[2]) / reps
+    walls_info = {"find_walls_ms": round(wa_ms[0], 4), "wall_aspect_ms": round(wa_ms[1], 4),
+                  "wall_cells": int((w_dev > 0).sum().item()), "filter_size": WG.filter_size(tile.scale)}
+    log(f"walls/aspect on the device: {wa_ms[0]:.3f} + {wa_ms[1]:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         if args.cpu_rows:
@@ -206,6 +225,11 @@ def main():
             csite, csun, cmet = site, sun, met
         log(f"CPU oracle baseline on {csite.dsm.shape}")
         cpu = cpu_baseline(csite, csun, cmet)
+        # walls/aspect CPU leg: the numpy oracle (vectorised over wall cells) on the same DSM
+        from oracle import walls_aspect as OW
+        c0 = time.perf_counter()
+        OW.wall_aspect(OW.findwalls(site.dsm, 3.0), tile.scale, site.dsm)
+        walls_info["cpu_oracle_ms"] = round((time.perf_counter() - c0) * 1e3, 1)
     line = {
         "metric": "Mpixel-steps/s (Tmrt+UTCI, 1000²×24h)",
         "value": round(value, 3),
@@ -228,6 +252,7 @@ def main():
         "value_host_to_host": round(host_value, 3),
         "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
         "host_plan_ms": round(plan_ms, 3),
+        "walls_aspect": walls_info,
     }
     print(json.dumps(line), flush=True)
 

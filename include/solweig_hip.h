@@ -277,6 +277,44 @@ int sw_solve_day_timed(const sw_tile_t *tile, const sw_step_t *steps, const sw_m
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,
             int64_t n, void *stream);
 
+/* ---- walls and wall aspect (tile inputs) -----------------------------------
+ * Wall heights: replaces findwalls(dem_array, walllimit)
+ * (solweig_gpu/walls_aspect.py:27-58). Per cell: maximum of the 4
+ * neighbours (NaN if any is NaN) minus the cell, formed in double; values
+ * below walllimit become 0; the one-cell border is 0. Written as float32,
+ * the dtype of the wall raster the reference writes (walls_aspect.py:214-228). */
+int sw_find_walls(const float *dsm, float *walls, int rows, int cols, double walllimit, void *stream);
+
+/* Rotated line/building filters of the wall-aspect method, one set per
+ * rotation h = 0..179 (walls_aspect.py:121-142, built with
+ * scipy.ndimage.rotate by the host). For each h: count[h][0] offsets of the
+ * rotated line filter (cells equal to 1), count[h][1] / count[h][2] offsets of
+ * the building filter cells equal to 1 / 2, in row-major window order;
+ * off[h][l][k] = (dy, dx) from the window centre. size is the filter size
+ * the table was built for; counts are at most SW_WALL_LIST_CAP. */
+#define SW_WALL_ROTATIONS 180
+#define SW_WALL_LIST_CAP 128
+typedef struct {
+    int32_t size;
+    int32_t count[SW_WALL_ROTATIONS][3];
+    int8_t off[SW_WALL_ROTATIONS][3][SW_WALL_LIST_CAP][2];
+} sw_wall_filters_t;
+
+/* Filter size for a scale in pixels per metre (walls_aspect.py:114-118):
+ * floor((scale + 1e-10) * 9), 3 if that is <= 2, made odd by +1 unless 9. */
+int sw_wall_filter_size(double scale);
+
+/* Wall aspect in degrees: replaces filter1Goodwin_as_aspect_v3(walls, scale, a)
+ * (walls_aspect.py:99-166). filters is a device copy of the table for
+ * sw_wall_filter_size(scale); a table of another size gives NaN everywhere.
+ * Wall cells (walls > 0) of the window interior take the rotation with the
+ * largest line count (first maximum kept), 270 - h, minus 180 when the
+ * building half 1 has the larger float32 DSM sum (np.sum order); cells
+ * still at 0 take the DSM gradient aspect (get_ders, :78-97). rows and
+ * cols must be at least 2 (np.gradient's minimum). */
+int sw_wall_aspect(const float *walls, const float *dsm, const sw_wall_filters_t *filters, float *aspect,
+                   int rows, int cols, double scale, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,20 @@ class TileStruct(ctypes.Structure):
     ]
 
 
+WALL_ROTATIONS = 180
+WALL_LIST_CAP = 128
+
+
+class WallFiltersStruct(ctypes.Structure):
+    """Mirror of ``sw_wall_filters_t`` (rotated wall-aspect filters)."""
+
+    _fields_ = [
+        ("size", _c_i32),
+        ("count", (_c_i32 * 3) * WALL_ROTATIONS),
+        ("off", (((ctypes.c_int8 * 2) * WALL_LIST_CAP) * 3) * WALL_ROTATIONS),
+    ]
+
+
 #: exported symbols and their (restype, argtypes)
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
@@ -137,6 +151,9 @@ SIGNATURES = {
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
+    "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),
+    "sw_wall_filter_size": (_I, [ctypes.c_double]),
+    "sw_wall_aspect": (_I, [_VP] * 4 + [_I, _I, ctypes.c_double, _VP]),
 }
 
 

@@ -191,6 +191,43 @@ def _place_trees(rng, rows, cols, cover, building):
     return cdsm
 
 
+def mixed_dsm(rows: int, cols: int, seed: int = 7, *, cover: float = 0.25, slope: float = 0.05) -> np.ndarray:
+    """DSM with walls of every orientation: rotated rectangles and round towers on sloped ground.
+
+    Ground rises ``slope`` m per pixel along a random direction; roofs are
+    flat at U(4, 30) m above the ground under the footprint centre, with
+    fractional heights, so the wall aspect and its gradient fallback both see
+    non-trivial float32 sums.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ii, jj = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    th = rng.uniform(0.0, 2.0 * np.pi)
+    ground = slope * (np.cos(th) * ii + np.sin(th) * jj)
+    ground -= ground.min()
+    dsm = ground.astype(F32)
+    roof = np.zeros((rows, cols), bool)
+    guard = 0
+    while roof.sum() < cover * rows * cols and guard < 1000:
+        guard += 1
+        ci, cj = rng.uniform(0, rows), rng.uniform(0, cols)
+        if rng.random() < 0.35:
+            r = rng.uniform(2.5, max(3.0, min(rows, cols) / 6))
+            fp = (ii - ci) ** 2 + (jj - cj) ** 2 <= r * r
+        else:
+            a = rng.uniform(0.0, np.pi)
+            hl, hw = rng.uniform(2.5, max(3.0, rows / 4)), rng.uniform(2.5, max(3.0, cols / 5))
+            u = (ii - ci) * np.cos(a) + (jj - cj) * np.sin(a)
+            v = -(ii - ci) * np.sin(a) + (jj - cj) * np.cos(a)
+            fp = (np.abs(u) <= hl) & (np.abs(v) <= hw)
+        if not fp.any():
+            continue
+        base = ground[min(rows - 1, int(ci)), min(cols - 1, int(cj))]
+        top = F32(base + rng.uniform(4.0, 30.0))
+        dsm[fp] = np.maximum(dsm[fp], top)
+        roof |= fp
+    return dsm
+
+
 def make_site(rows: int, cols: int, seed: int = 20261203, *, pixel_size: float = 1.0,
               building_cover: float = 0.30, canopy_cover: float = 0.05,
               with_aspect: bool = True, latitude: float = 30.0,

@@ -287,6 +287,75 @@ def gen_sun(sp):
     print("[golden] sun tables", len(cases))
 
 
+WALL_SCENES = {
+    # name: (kind, rows, cols, seed, pixel_size); kind "site" = make_site roofs, "mixed" = mixed_dsm
+    "site48": ("site", 48, 48, 11, 1.0),
+    "mixed64": ("mixed", 64, 64, 3, 1.0),
+    "mixed40x56_px2": ("mixed", 40, 56, 4, 2.0),
+    "mixed72_px05": ("mixed", 72, 72, 5, 0.5),
+    "mixed24x20_px4": ("mixed", 24, 20, 6, 4.0),
+    "nan36": ("mixed", 36, 36, 8, 1.0),
+    "tiny8": ("mixed", 8, 8, 9, 1.0),
+    "strip2x5": ("mixed", 2, 5, 10, 1.0),
+    # random heights: wall cells everywhere, winning rotations at every angle
+    "noise40": ("noise", 40, 40, 12, 1.0),
+    "noise24_px4": ("noise", 24, 24, 13, 4.0),
+    # rough roofs at 800 m mirrored about an alley cross: the two building
+    # halves hold the same heights in opposite order, so only float32
+    # summation in np.sum's order decides the side
+    "mirror41": ("mirror", 41, 41, 14, 1.0),
+    # two height levels 3.5 m apart: side sums close enough that the cell the
+    # reference removes from the building filter at h = 30 / 150 decides them
+    "blocks48_px4": ("blocks", 48, 48, 28, 4.0),
+}
+
+
+def _wall_scene_dsm(name):
+    from solweig_project_amd import synth
+    kind, rows, cols, seed, px = WALL_SCENES[name]
+    if kind == "site":
+        dsm = synth.make_site(rows, cols, seed=seed, pixel_size=px, with_aspect=False).dsm
+    elif kind == "noise":
+        dsm = np.random.default_rng(seed).uniform(0.0, 12.0, (rows, cols)).astype(np.float32)
+    elif kind == "blocks":
+        rng = np.random.default_rng(seed)
+        dsm = (rng.integers(0, 2, (rows, cols)) * 3.5 + rng.uniform(0, 0.5, (rows, cols))).astype(np.float32)
+    elif kind == "mirror":
+        rng = np.random.default_rng(seed)
+        q = (800.0 + rng.uniform(0.0, 2.0, (rows // 2 + 1, cols // 2 + 1))).astype(np.float32)
+        dsm = np.concatenate([q, q[:, -2::-1]], axis=1)[:, :cols]
+        dsm = np.concatenate([dsm, dsm[-2::-1, :]], axis=0)[:rows, :]
+        dsm[rows // 2, :] = 790.0
+        dsm[:, cols // 2] = 790.0
+    else:
+        dsm = synth.mixed_dsm(rows, cols, seed=seed, cover=0.3 if rows > 8 else 0.5)
+    dsm = np.asarray(dsm, np.float32).copy()
+    if name == "nan36":
+        dsm[10, 12] = np.nan
+        dsm[20:22, 30] = np.nan
+    return dsm, px
+
+
+def gen_walls():
+    """Wall heights and aspect from the reference's walls_aspect functions."""
+    import warnings
+    import solweig_gpu.walls_aspect as wa  # noqa: E402 -- needs the osgeo stand-in
+    out = {}
+    for name in WALL_SCENES:
+        dsm, px = _wall_scene_dsm(name)
+        scale = 1 / px
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            walls = wa.findwalls(dsm, wa.walllimit)
+            aspect = wa.filter1Goodwin_as_aspect_v3(walls, scale, dsm)
+        out[f"{name}_dsm"] = dsm
+        out[f"{name}_scale"] = np.float64(scale)
+        out[f"{name}_walls"] = walls
+        out[f"{name}_aspect"] = aspect
+        print("[golden] walls", name, dsm.shape, int((walls > 0).sum()), "wall cells", flush=True)
+    np.savez_compressed(HERE / "walls_aspect.npz", **out)
+
+
 def main():
     if not REF.is_dir():
         raise SystemExit("reference checkout not present; golden vectors are generated where it is")
@@ -299,6 +368,8 @@ def main():
         gen_utci(cu)
     if not only or "sun" in only:
         gen_sun(sp)
+    if not only or "walls" in only:
+        gen_walls()
     for name in SCENES:
         if not only or name in only:
             gen_scene(name, shadow_mod, solweig_mod, up, sp)

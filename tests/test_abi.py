@@ -60,16 +60,17 @@ def test_library_loads_and_reports_version():
 def test_struct_layout_matches_c(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "solweig_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(sw_step_t), sizeof(sw_model_t),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(sw_step_t), sizeof(sw_model_t),'
                    ' offsetof(sw_step_t, pk), offsetof(sw_step_t, pl), offsetof(sw_model_t, walk_hi),'
-                   ' sizeof(sw_tile_t), offsetof(sw_tile_t, state));'
+                   ' sizeof(sw_tile_t), offsetof(sw_tile_t, state), sizeof(sw_wall_filters_t),'
+                   ' offsetof(sw_wall_filters_t, off));'
                    'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
     vals = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     assert vals == [ctypes.sizeof(L.StepStruct), ctypes.sizeof(L.ModelStruct), L.StepStruct.pk.offset,
                     L.StepStruct.pl.offset, L.ModelStruct.walk_hi.offset, ctypes.sizeof(L.TileStruct),
-                    L.TileStruct.state.offset]
+                    L.TileStruct.state.offset, ctypes.sizeof(L.WallFiltersStruct), L.WallFiltersStruct.off.offset]
 
 
 def test_loader_fails_loudly_without_library(tmp_path):
