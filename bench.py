@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="side of the CPU-oracle tile (0: the benchmark tile itself)")
+    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+                    help="c2: one 1000x1000 tile (default); c3: batched tiles 300/700/1200/2000 squared plus "
+                         "64 edit scenarios of a 500x500 base")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run k_pack_heights over 16 Mi pixels (12 B read + 16 B written per pixel) "
                          "as the known-bytes reference for rocprof FETCH_SIZE / WRITE_SIZE")
@@ -91,12 +94,139 @@ def cpu_baseline(site, sun, met):
                                     f"{dt:.1f} s wall")
 
 
+def c3_inputs(seed, dev):
+    """Tiles 300/700/1200/2000 squared (seeds +10..+13) and 64 edit scenarios of a 500 x 500 base
+    (seed +20); walls and wall aspect of every DSM computed on the device."""
+    from solweig_project_amd import synth
+    from solweig_project_amd import walls as WG
+    from solweig_project_amd.solver import TileInputs
+
+    def with_walls(base, dsm, cdsm, lc):
+        w, a = WG.walls_and_aspect(dsm, base.scale)
+        return TileInputs.from_site(synth.site_with(base, dsm, cdsm, lc, walls=w, aspect=a))
+    tiles = []
+    for k, n in enumerate((300, 700, 1200, 2000)):
+        s = synth.make_site(n, n, seed=seed + 10 + k, with_aspect=False)
+        tiles.append(with_walls(s, s.dsm, s.cdsm, s.landcover))
+    base = synth.make_site(500, 500, seed=seed + 20, with_aspect=False)
+    edits = [with_walls(base, d, c, lc) for d, c, lc, _ in synth.edit_scenarios(base, 64, seed=seed + 21)]
+    return tiles, edits, base
+
+
+def run_c3(args):
+    """Batched exact solves: every group runs its SVF sweeps, then one sw_solve_batch day loop."""
+    import numpy as np
+    import torch
+    from solweig_project_amd import _lib as L
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import DayBatch, HipSolver
+
+    dev = torch.device("cuda:0")
+    log("generating C3 inputs (walls/aspect on the device)")
+    tiles, edits, base = c3_inputs(args.seed, dev)
+    sun = sun_for_site(base.latitude, base.longitude, base.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=args.seed + 1)
+    T = met.shape[0]
+    solver = HipSolver(dev)
+    groups = {"tiles": [solver.upload(t) for t in tiles], "edits": [solver.upload(t) for t in edits]}
+    px = {g: sum(ds.n for ds in sc) for g, sc in groups.items()}
+
+    def solve_group(scenes):
+        for ds in scenes:
+            solver.svf(ds)
+        batch = DayBatch(solver, scenes, met, sun, base.latitude)   # plans while the GPU marches
+        for ds in scenes:
+            solver.tile_static(ds, dict(model=batch.model))
+        batch.run()
+        return batch
+
+    def sequential(scenes, batch):
+        # the same work as one launch per tile and kernel (plans shared with the batch)
+        for ds in scenes:
+            solver.svf(ds)
+            solver.tile_static(ds, dict(model=batch.model))
+        for ds, out in zip(scenes, batch.outs):
+            plan = batch.plans[ds.tile.shape]
+            solver.run_day(ds, plan, batch.dplans[ds.tile.shape], out=out)
+
+    res = {}
+    for g, scenes in groups.items():
+        log(f"{g}: {len(scenes)} items, {px[g] / 1e6:.2f} Mpx")
+        b = solve_group(scenes)
+        torch.cuda.synchronize()
+        for ds in scenes:
+            ds.check_flags()
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b = solve_group(scenes)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        ms = float(np.median(times)) * 1e3
+        # day loop only: eager batched launches vs captured graph replay vs one launch per tile
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        b.run()
+        ev[1].record()
+        torch.cuda.synchronize()
+        day_ms = ev[0].elapsed_time(ev[1])
+        s = torch.cuda.Stream(dev)
+        gph = __import__("ctypes").c_void_p()
+        L.call("sw_graph_begin", s.cuda_stream)
+        rc = L.load().sw_solve_batch(L.ptr(b.items_dev), b.items_host, len(scenes), L.ptr(b.model), b._is_day_c, 0,
+                                     b.n_steps, b.mask, b.walk_halo, 1, s.cuda_stream)
+        L.call("sw_graph_end", s.cuda_stream, __import__("ctypes").byref(gph))
+        if rc:
+            raise RuntimeError(L.load().sw_last_error().decode())
+        L.call("sw_graph_launch", gph, s.cuda_stream)
+        s.synchronize()
+        with torch.cuda.stream(s):
+            ev[0].record(s)
+            L.call("sw_graph_launch", gph, s.cuda_stream)
+            ev[1].record(s)
+        s.synchronize()
+        graph_ms = ev[0].elapsed_time(ev[1])
+        L.call("sw_graph_destroy", gph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sequential(scenes, b)
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3
+        res[g] = {"items": len(scenes), "mpx": round(px[g] / 1e6, 3), "ms": round(ms, 2),
+                  "mpixel_steps_per_s": round(px[g] * T / (ms / 1e3) / 1e6, 1),
+                  "day_batched_ms": round(day_ms, 2), "day_graph_replay_ms": round(graph_ms, 2),
+                  "launches_per_day_batched": int(sum(b.is_day)) * 2 + (T - int(sum(b.is_day))),
+                  "sequential_one_launch_per_tile_ms": round(seq_ms, 2)}
+        log(f"{g}: {ms:.1f} ms per batched solve; day {day_ms:.1f} ms eager, {graph_ms:.1f} ms graph; "
+            f"one launch per tile {seq_ms:.1f} ms")
+    tot_px = sum(px.values())
+    tot_ms = sum(r["ms"] for r in res.values())
+    line = {
+        "metric": "Mpixel-steps/s (C3: batched exact solves, 4 uneven tiles + 64 edit scenarios of 500², 24 h)",
+        "value": round(tot_px * T / (tot_ms / 1e3) / 1e6, 3),
+        "unit": "Mpixel-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(tot_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded city tiles + edit scenarios; inputs resident in HBM, outputs left in HBM)",
+        "config": {"workload": "C3 batch: tiles 300², 700², 1200², 2000² and 64 edit scenarios (1-20 tree or "
+                               "building edits of 10-400 px) of a 500² base, 24 hourly steps, one plan per tile "
+                               "shape, SVF sweep per item, one sun-march and one flux launch per step for all "
+                               "items", "time_steps": T},
+        "groups": res,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
               "design-only (tiles with read-only halos, no collectives)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        run_c3(args)
+        return
     from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
     pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used
     import numpy as np

@@ -116,6 +116,21 @@ typedef struct sw_tile {
     float *state;             /* 6 planes, carried across steps */
 } sw_tile_t;
 
+/* One item of a batched day loop: a tile (or an edit scenario of a tile),
+ * its day plan and its output buffer. Items of one batch share the model
+ * constants and the day/night pattern (same site, day, forcing and pixel
+ * size); step records and sun tables are per item because the sun march
+ * ends at the tile's edge. sun_block0 / flux_block0 are filled by
+ * sw_batch_layout. */
+typedef struct sw_batch_item {
+    sw_tile_t tile;
+    const sw_step_t *steps;   /* the item's step records */
+    const int32_t *sun_tab;   /* the item's sun-march tables */
+    float *out;               /* step i at out + (i - t0) * popcount(out_mask) * rows * cols */
+    int32_t sun_block0;       /* first block of the item in the batched sun-march grid */
+    int32_t flux_block0;      /* first block of the item in the batched flux grid */
+} sw_batch_item_t;
+
 /* ---- runtime ------------------------------------------------------------
  * Mirrors the reference native ABI's runtime surface (sw_api.cu:24-171,
  * sw_residency.cu:53-331): queries, counted device allocation, pinned host
@@ -272,6 +287,19 @@ int sw_solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t
 int sw_solve_day_timed(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
                        const int32_t *sun_tab, const int32_t *is_day, void **events, int t0, int t1, float *out,
                        int out_mask, int walk_halo, int reset_state, void *stream);
+
+/* Batched day loop (the reference solves tiles one after another,
+ * utci_process.py:1199-1663; edit scenarios one request at a time,
+ * solweig_gpu/incremental/worker.py:1588-1595). sw_batch_layout assigns each
+ * host item its block ranges and returns the grid totals. sw_solve_batch
+ * then runs steps [t0, t1) of every item with one sun-march launch per day
+ * step and one flux launch per step over all items: items_dev is a device
+ * copy of items_host (the same nitems items, after sw_batch_layout). Every
+ * item's outputs equal those of sw_solve_day on that item alone. */
+int sw_batch_layout(sw_batch_item_t *items, int nitems, int32_t *sun_blocks, int32_t *flux_blocks);
+int sw_solve_batch(const sw_batch_item_t *items_dev, const sw_batch_item_t *items_host, int nitems,
+                   const sw_model_t *model, const int32_t *is_day, int t0, int t1, int out_mask, int walk_halo,
+                   int reset_state, void *stream);
 
 /* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,

@@ -86,6 +86,19 @@ class TileStruct(ctypes.Structure):
     ]
 
 
+class BatchItemStruct(ctypes.Structure):
+    """Mirror of ``sw_batch_item_t`` (one tile or scenario of a batched day loop)."""
+
+    _fields_ = [
+        ("tile", TileStruct),
+        ("steps", ctypes.c_void_p),
+        ("sun_tab", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("sun_block0", _c_i32),
+        ("flux_block0", _c_i32),
+    ]
+
+
 WALL_ROTATIONS = 180
 WALL_LIST_CAP = 128
 
@@ -151,6 +164,8 @@ SIGNATURES = {
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
+    "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),
+    "sw_solve_batch": (_I, [_VP, _VP, _I, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),
     "sw_wall_filter_size": (_I, [ctypes.c_double]),
     "sw_wall_aspect": (_I, [_VP] * 4 + [_I, _I, ctypes.c_double, _VP]),

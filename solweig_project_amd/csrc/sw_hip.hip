@@ -15,6 +15,7 @@
 // float. Heights must be finite and non-negative (checked by the host),
 // which makes max/min NaN semantics moot and the early exits exact.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -731,16 +732,16 @@ __device__ __forceinline__ float pow4(float x) {
 #define FLUX_TX 16
 #define FLUX_TY 16
 
-__global__ void __launch_bounds__(256)
-k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
-           const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
-           const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
-           const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
-           const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols) {
+// one 256-pixel block of the sun step; `tile` is the block's index within its tile
+__device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, const float *__restrict__ walls,
+                                              const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
+                                              const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
+                                              const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
+                                              const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step,
+                                              int rows, int cols, int tile) {
     const long n = (long)rows * cols;
     const float amax = funkey(stats[0]);
     const float hmax = funkey(stats[1]);
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const long t = (long)tile * 256 + threadIdx.x;
     const bool valid = (t < n);
     const sw_step_t *sp = steps + step;
@@ -789,6 +790,72 @@ k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
     const uint32_t flags = ((uint32_t)shv | ((uint32_t)vso << 1) | (sunwall << 2) |
                             ((b == 1.f ? 1u : 0u) << 3) | ((uint32_t)k << 4)) << 4;
     src[t] = make_uint2(__float_as_uint(lup), flags);
+}
+
+__global__ void __launch_bounds__(256)
+k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
+           const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
+           const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
+           const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
+           const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols) {
+    sun_step_body(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
+                  xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// The batch kernels read the item table through this mirror of
+// sw_batch_item_t whose pointers are declared in the global address space:
+// a pointer loaded as a plain (flat) pointer would make every access through
+// it a flat load, and the step records would lose their scalar loads.
+#define SW_GLOBAL __attribute__((address_space(1)))
+struct BatchTileDev {
+    int32_t rows, cols;
+    const SW_GLOBAL float4 *hpack;
+    const SW_GLOBAL float *walls, *aspect_rad, *buildings;
+    const SW_GLOBAL uint8_t *cls;
+    const SW_GLOBAL uint32_t *stats, *masks;
+    const SW_GLOBAL float *svfbuveg, *tan_asvf, *tan_svfalfa;
+    const SW_GLOBAL uint32_t *walkface;
+    SW_GLOBAL uint2 *src;
+    SW_GLOBAL float *state;
+};
+struct BatchItemDev {
+    BatchTileDev tile;
+    const SW_GLOBAL sw_step_t *steps;
+    const SW_GLOBAL int4 *sun_tab;
+    SW_GLOBAL float *out;
+    int32_t sun_block0, flux_block0;
+};
+static_assert(sizeof(BatchItemDev) == sizeof(sw_batch_item_t), "batch item mirror");
+static_assert(offsetof(BatchItemDev, steps) == offsetof(sw_batch_item_t, steps), "batch item mirror");
+static_assert(offsetof(BatchItemDev, out) == offsetof(sw_batch_item_t, out), "batch item mirror");
+static_assert(offsetof(BatchItemDev, flux_block0) == offsetof(sw_batch_item_t, flux_block0), "batch item mirror");
+static_assert(offsetof(BatchTileDev, state) == offsetof(sw_tile_t, state), "batch tile mirror");
+
+// Batched launches: one grid covers the blocks of every item, item k owning
+// launch-order tiles [block0_k, block0_k + blocks_k) after the XCD remap.
+// The owner is found by binary search over the items' (ascending) block0.
+template <int FIELD>
+__device__ __forceinline__ int batch_owner(const BatchItemDev *__restrict__ items, int nitems, int g) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int b0 = FIELD == 0 ? items[mid].sun_block0 : items[mid].flux_block0;
+        if (b0 <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_sun_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const sw_model_t *__restrict__ model,
+            int step) {
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= total) return;
+    const BatchItemDev *it = items + batch_owner<0>(items, nitems, g);
+    const BatchTileDev &T = it->tile;
+    sun_step_body((const float4 *)T.hpack, (const float *)T.walls, (const float *)T.aspect_rad,
+                  (const uint8_t *)T.cls, (const float *)T.buildings, (const sw_step_t *)it->steps, model,
+                  (const int4 *)it->sun_tab, (const uint32_t *)T.stats, (uint2 *)T.src, step, T.rows, T.cols,
+                  g - it->sun_block0);
 }
 
 // ---------------------------------------------------------------------------
@@ -966,15 +1033,15 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
 
 // WALK = false instantiates a night-only kernel (no walk tile, no shortwave
 // code, fewer registers); it refuses a day step with NaN outputs.
+// one 16 x 16 tile of the flux step; `tile` is the block's tile index within its tile grid
 template <bool WALK>
-__global__ void __launch_bounds__(256)
-k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
-            const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
-            const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
-            const float *__restrict__ bu, const uint8_t *__restrict__ cls,
-            const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
-            float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int halo,
-            int rows, int cols) {
+__device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
+                                          const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
+                                          const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
+                                          const float *__restrict__ bu, const uint8_t *__restrict__ cls,
+                                          const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
+                                          float *__restrict__ state, float *__restrict__ out, int out_mask, int step,
+                                          int halo, int rows, int cols, int tile) {
     extern __shared__ uint2 s_src[];
     __shared__ float s_alb[SW_NCLASS];
     __shared__ float4 s_dec[128];                 // walk source decode by flag bits 0-6: {alb * shadow, alb, ground, sunwall}
@@ -984,7 +1051,6 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     const sw_model_t *md = model;
     const int tiles_x = (cols + FLUX_TX - 1) / FLUX_TX;
     const int tiles_y = (rows + FLUX_TY - 1) / FLUX_TY;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= tiles_x * tiles_y) return;  // whole block idle (uniform)
     const int r0 = (tile / tiles_x) * FLUX_TY, c0 = (tile % tiles_x) * FLUX_TX;
     // each wave covers an 8 x 8 square of the 16 x 16 tile
@@ -1223,6 +1289,35 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
     if (out_mask & 16) out[(long)(k++) * n + t] = Kup;
     if (out_mask & 32) out[(long)(k++) * n + t] = Ldown;
     if (out_mask & 64) out[(long)(k++) * n + t] = Lup;
+}
+
+template <bool WALK>
+__global__ void __launch_bounds__(256)
+k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
+            const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
+            const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
+            const float *__restrict__ bu, const uint8_t *__restrict__ cls,
+            const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
+            float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int halo,
+            int rows, int cols) {
+    flux_body<WALK>(masks, src, svfbuveg, tan_asvf, tan_sa, walkface, bu, cls, steps, model, state, out, out_mask,
+                    step, halo, rows, cols, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+template <bool WALK>
+__global__ void __launch_bounds__(256)
+k_flux_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const sw_model_t *__restrict__ model,
+             int out_mask, int out_step, int step, int halo) {
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= total) return;
+    const BatchItemDev *it = items + batch_owner<1>(items, nitems, g);
+    const BatchTileDev &T = it->tile;
+    const long n = (long)T.rows * T.cols;
+    SW_GLOBAL float *out = it->out + (long)out_step * __builtin_popcount((unsigned)out_mask) * n;
+    flux_body<WALK>((const uint32_t *)T.masks, (const uint2 *)T.src, (const float *)T.svfbuveg,
+                    (const float *)T.tan_asvf, (const float *)T.tan_svfalfa, (const uint32_t *)T.walkface,
+                    (const float *)T.buildings, (const uint8_t *)T.cls, (const sw_step_t *)it->steps, model,
+                    (float *)T.state, (float *)out, out_mask, step, halo, T.rows, T.cols, g - it->flux_block0);
 }
 
 // walk face-shading bits: bit j set where facesh of walk azimuth j is 0
@@ -1511,6 +1606,98 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
                        svfbuveg, tan_asvf, tan_svfalfa, walkface, buildings, cls, steps, model, state, out,
                        out_mask, step, walk_halo, rows, cols);
     return hip_check(hipGetLastError(), "k_flux_step");
+}
+
+static int item_blocks_sun(const sw_tile_t &t) { return grid_for((long)t.rows * t.cols, 256); }
+static int item_blocks_flux(const sw_tile_t &t) {
+    return ((t.cols + FLUX_TX - 1) / FLUX_TX) * ((t.rows + FLUX_TY - 1) / FLUX_TY);
+}
+
+int sw_batch_layout(sw_batch_item_t *items, int nitems, int32_t *sun_blocks, int32_t *flux_blocks) {
+    if (!items || nitems <= 0 || !sun_blocks || !flux_blocks)
+        return set_err(SW_E_ARG, "sw_batch_layout: NULL pointer or empty batch");
+    long sb = 0, fb = 0;
+    for (int k = 0; k < nitems; ++k) {
+        int rc = check_shape(items[k].tile.rows, items[k].tile.cols, "sw_batch_layout");
+        if (rc) return rc;
+        items[k].sun_block0 = (int32_t)sb;
+        items[k].flux_block0 = (int32_t)fb;
+        sb += item_blocks_sun(items[k].tile);
+        fb += item_blocks_flux(items[k].tile);
+        if (sb > (1L << 30) || fb > (1L << 30)) return set_err(SW_E_ARG, "sw_batch_layout: batch too large");
+    }
+    *sun_blocks = (int32_t)sb;
+    *flux_blocks = (int32_t)fb;
+    return SW_OK;
+}
+
+// Host checks of a batch; returns the grid totals of items_host (laid out by sw_batch_layout).
+static int check_batch(const sw_batch_item_t *items, int nitems, long *sun_total, long *flux_total) {
+    long sb = 0, fb = 0;
+    for (int k = 0; k < nitems; ++k) {
+        const sw_tile_t &t = items[k].tile;
+        int rc = check_shape(t.rows, t.cols, "sw_solve_batch");
+        if (rc) return rc;
+        if (!t.hpack || !t.walls || !t.aspect_rad || !t.buildings || !t.cls || !t.stats || !t.masks || !t.svfbuveg ||
+            !t.tan_asvf || !t.tan_svfalfa || !t.walkface || !t.src || !t.state || !items[k].out || !items[k].steps ||
+            !items[k].sun_tab)
+            return set_err(SW_E_ARG, "sw_solve_batch: NULL pointer in an item");
+        if (items[k].sun_block0 != sb || items[k].flux_block0 != fb)
+            return set_err(SW_E_ARG, "sw_solve_batch: items not laid out by sw_batch_layout");
+        sb += item_blocks_sun(t);
+        fb += item_blocks_flux(t);
+    }
+    *sun_total = sb;
+    *flux_total = fb;
+    return SW_OK;
+}
+
+int sw_solve_batch(const sw_batch_item_t *items_dev, const sw_batch_item_t *items_host, int nitems,
+                   const sw_model_t *model, const int32_t *is_day, int t0, int t1, int out_mask, int walk_halo,
+                   int reset_state, void *stream) {
+    if (!items_dev || !items_host || nitems <= 0 || !model || !is_day)
+        return set_err(SW_E_ARG, "sw_solve_batch: NULL pointer or empty batch");
+    if (t0 < 0 || t1 < t0) return set_err(SW_E_ARG, "sw_solve_batch: invalid step range");
+    if (out_mask <= 0 || out_mask > 127) return set_err(SW_E_ARG, "sw_solve_batch: out_mask selects nothing");
+    if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_solve_batch: walk halo out of range");
+    long sun_total = 0, flux_total = 0;
+    int rc = check_batch(items_host, nitems, &sun_total, &flux_total);
+    if (rc) return rc;
+    if (reset_state) {
+        for (int k = 0; k < nitems; ++k) {
+            const sw_tile_t &t = items_host[k].tile;
+            rc = sw_fill_f32(t.state, 0.f, 6 * (int64_t)t.rows * t.cols, stream);
+            if (rc) return rc;
+        }
+    }
+    const int sun_grid = (int)(((sun_total + 7) / 8) * 8);
+    const int flux_grid = (int)(((flux_total + 7) / 8) * 8);
+    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    if (lds > 65536) {
+        rc = hip_check(hipFuncSetAttribute((const void *)k_flux_batch<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                       "hipFuncSetAttribute");
+        if (rc) return rc;
+    }
+    for (int i = t0; i < t1; ++i) {
+        if (is_day[i - t0]) {
+            hipLaunchKernelGGL(k_sun_batch, dim3(sun_grid), dim3(256), 0, S(stream), (const BatchItemDev *)items_dev, nitems,
+                               (int)sun_total, model, i);
+            rc = hip_check(hipGetLastError(), "k_sun_batch");
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_flux_batch<true>, dim3(flux_grid), dim3(256), lds, S(stream),
+                               (const BatchItemDev *)items_dev, nitems,
+                               (int)flux_total, model, out_mask, i - t0, i, walk_halo);
+            rc = hip_check(hipGetLastError(), "k_flux_batch");
+        } else {
+            hipLaunchKernelGGL(k_flux_batch<false>, dim3(flux_grid), dim3(256), 0, S(stream),
+                               (const BatchItemDev *)items_dev, nitems,
+                               (int)flux_total, model, out_mask, i - t0, i, -1);
+            rc = hip_check(hipGetLastError(), "k_flux_batch<night>");
+        }
+        if (rc) return rc;
+    }
+    return SW_OK;
 }
 
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out, int64_t n,

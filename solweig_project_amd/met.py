@@ -612,6 +612,7 @@ class DayLoop:
             sp.radG = float(np.float32(radG.item()))
             sp.sinalt = float(np.float32(torch.sin(altitude * (np.pi / 180)).item()))
             # sun march table + face-shading window
+            sp.sun_azalt = (azimuth.item(), altitude.item())
             tab = skyvault.sun_table(azimuth.item(), altitude.item(), self.scale, self.amax, self.rows, self.cols)
             sp.table = tab
             sp.facesh_branch, sp.azilow, sp.azihigh = skyvault.sun_facesh_bounds(azimuth.item())

@@ -304,6 +304,33 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
                    params=params)
 
 
+def plan_for_shape(plan: DayPlan, rows: int, cols: int, scale: float) -> DayPlan:
+    """``plan`` for a tile of another shape (same site, day, forcing and pixel size).
+
+    Only the sun-march tables depend on the shape (the march stops at the
+    grid edge); they are rebuilt and the step records' table offsets and
+    counts rewritten. Equal to ``plan_day`` on a tile of that shape.
+    """
+    T = plan.n_steps
+    steps = np.array(plan.steps, dtype=np.uint8, copy=True)
+    words = steps.view(np.int32).reshape(T, -1)
+    off_w = L.StepStruct.sun_off.offset // 4
+    cnt_w = L.StepStruct.sun_count.offset // 4
+    tabs, off = [], 0
+    for i, sp in enumerate(plan.params):
+        if not sp.is_day:
+            continue
+        t = SV.sun_table(sp.sun_azalt[0], sp.sun_azalt[1], scale, None, rows, cols)
+        tab = np.stack([t.dx, t.dy, t.dz.view(np.int32), t.dzprev.view(np.int32)], axis=1).astype(np.int32)
+        words[i, off_w] = off
+        words[i, cnt_w] = tab.shape[0]
+        tabs.append(tab)
+        off += tab.shape[0]
+    sun_tab = np.concatenate(tabs) if tabs else np.zeros((1, 4), np.int32)
+    return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=plan.model, n_steps=T,
+                   walk_halo=plan.walk_halo, is_day=list(plan.is_day), params=plan.params)
+
+
 # ---------------------------------------------------------------------------
 # device solver
 # ---------------------------------------------------------------------------
@@ -629,6 +656,19 @@ class HipSolver:
         out = self.run_day(ds, plan, dplan, outputs, out=out, stream=stream)
         return ds.fields, out
 
+    def solve_batch(self, scenes, met, sun: M.SunTable, latitude: float, outputs=("tmrt", "utci", "shadow"),
+                    outs=None, stream=None) -> "DayBatch":
+        """Enqueue full solves of several device-resident scenes sharing one forcing.
+
+        Per-item SVF sweeps, then one batched day loop (``sw_solve_batch``);
+        returns the :class:`DayBatch` whose ``outs`` hold each item's
+        (T, k, rows, cols) outputs.
+        """
+        batch = DayBatch(self, scenes, met, sun, latitude, outputs, outs)
+        batch.prepare(stream)
+        batch.run(stream)
+        return batch
+
     def solve(self, tile: TileInputs, met, sun: M.SunTable, latitude: float,
               outputs=("tmrt", "utci", "shadow")):
         """Full solve of one tile; returns (fields (16, rows, cols), {name: (T, rows, cols)}, DeviceScene)."""
@@ -640,3 +680,91 @@ class HipSolver:
         host = out.cpu().numpy()
         res = {nm: host[:, j] for j, nm in enumerate(names)}
         return fields.reshape(L.NFIELDS, ds.rows, ds.cols).cpu().numpy(), res, ds
+
+
+# ---------------------------------------------------------------------------
+# batched day loop (many tiles / edit scenarios, one launch per kernel and step)
+# ---------------------------------------------------------------------------
+def _owned_plan(plan: DayPlan, device):
+    """A day plan copied to device buffers of its own (pinned staging, non-blocking)."""
+    out = {}
+    for key, arr in (("steps", plan.steps), ("sun_tab", plan.sun_tab), ("model", plan.model)):
+        raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+        host = torch.from_numpy(raw.copy()).pin_memory()
+        out[key] = host.to(device, non_blocking=True)
+    return out
+
+
+class DayBatch:
+    """Items of one ``sw_solve_batch``: scenes, their per-shape plans and outputs.
+
+    Items share the model constants and day/night pattern (same forcing,
+    sun, latitude, pixel size and land-cover table); each distinct tile
+    shape gets its own plan because the sun march ends at the tile edge.
+    ``items_dev`` is the device copy of the laid-out item table.
+    """
+
+    def __init__(self, solver: "HipSolver", scenes, met, sun: M.SunTable, latitude: float,
+                 outputs=("tmrt", "utci", "shadow"), outs=None):
+        if not scenes:
+            raise ValueError("empty batch")
+        t0 = scenes[0].tile
+        for ds in scenes[1:]:
+            t = ds.tile
+            if t.scale != t0.scale or t.has_landcover != t0.has_landcover or not np.array_equal(
+                    np.asarray(t.lc_table), np.asarray(t0.lc_table)):
+                raise UnsupportedInput("a batch needs one pixel size and one land-cover table")
+        self.solver = solver
+        self.scenes = list(scenes)
+        self.mask = 0
+        for name in outputs:
+            self.mask |= OUT_BITS[name]
+        self.k = bin(self.mask).count("1")
+        self.plans = {}
+        with torch.inference_mode():
+            base = plan_day(t0, met, sun, latitude)       # the scalar chain once
+            self.plans[t0.shape] = base
+            for ds in self.scenes:
+                if ds.tile.shape not in self.plans:
+                    self.plans[ds.tile.shape] = plan_for_shape(base, ds.rows, ds.cols, t0.scale)
+        first = next(iter(self.plans.values()))
+        for p in self.plans.values():
+            if list(p.is_day) != list(first.is_day) or p.walk_halo != first.walk_halo:
+                raise UnsupportedInput("batch items disagree on the day/night pattern")
+        self.n_steps = first.n_steps
+        self.is_day = list(first.is_day)
+        self.walk_halo = first.walk_halo
+        dev = solver.device
+        self.dplans = {shape: _owned_plan(p, dev) for shape, p in self.plans.items()}
+        self.model = self.dplans[t0.shape]["model"]
+        T = self.n_steps
+        if outs is None:
+            outs = [torch.empty((T, self.k, ds.rows, ds.cols), dtype=torch.float32, device=dev) for ds in scenes]
+        self.outs = outs
+        items = (L.BatchItemStruct * len(scenes))()
+        for k, ds in enumerate(self.scenes):
+            dp = self.dplans[ds.tile.shape]
+            items[k].tile = ds.tile_struct()
+            items[k].steps = L.ptr(dp["steps"])
+            items[k].sun_tab = L.ptr(dp["sun_tab"])
+            items[k].out = L.ptr(outs[k])
+        sb, fb = ctypes.c_int32(), ctypes.c_int32()
+        L.call("sw_batch_layout", items, len(scenes), ctypes.byref(sb), ctypes.byref(fb))
+        self.sun_blocks, self.flux_blocks = sb.value, fb.value
+        self.items_host = items
+        raw = np.frombuffer(bytes(items), dtype=np.uint8)
+        self.items_dev = torch.from_numpy(raw.copy()).pin_memory().to(dev, non_blocking=True)
+        self._is_day_c = (ctypes.c_int32 * T)(*[int(x) for x in self.is_day])
+
+    def prepare(self, stream=None):
+        """SVF sweep and per-day static planes of every item (enqueued)."""
+        for ds in self.scenes:
+            self.solver.svf(ds, stream)
+            self.solver.tile_static(ds, dict(model=self.model), stream)
+
+    def run(self, stream=None, reset_state=True):
+        """Enqueue the batched day; outputs land in ``self.outs``."""
+        L.call("sw_solve_batch", L.ptr(self.items_dev), self.items_host, len(self.scenes), L.ptr(self.model),
+               self._is_day_c, 0, self.n_steps, self.mask, self.walk_halo, 1 if reset_state else 0,
+               _stream_ptr(stream))
+        return self.outs

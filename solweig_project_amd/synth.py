@@ -151,7 +151,8 @@ def _place_buildings(rng, rows, cols, cover, dsm):
     lo, hi = 8, 40
     target = cover * rows * cols
     guard = 0
-    while occupied.sum() < target and guard < 100000:
+    count = 0                                   # occupied.sum(), kept incrementally
+    while count < target and guard < 100000:
         guard += 1
         h = int(rng.integers(lo, hi + 1))
         w = int(rng.integers(lo, hi + 1))
@@ -162,6 +163,7 @@ def _place_buildings(rng, rows, cols, cover, dsm):
         height = F32(rng.uniform(5.0, 40.0))
         sl = (slice(r0, r0 + h), slice(c0, c0 + w))
         dsm[sl] = np.maximum(dsm[sl], height)
+        count += h * w - int(occupied[sl].sum())
         occupied[sl] = True
     return occupied
 
@@ -173,7 +175,8 @@ def _place_trees(rng, rows, cols, cover, building):
     if ground.size == 0:
         return cdsm
     guard = 0
-    while (cdsm > 0).sum() < target and guard < 100000:
+    count = 0                                   # (cdsm > 0).sum(), kept incrementally
+    while count < target and guard < 100000:
         guard += 1
         k = int(ground[rng.integers(0, ground.size)])
         ci, cj = divmod(k, cols)
@@ -187,7 +190,9 @@ def _place_trees(rng, rows, cols, cover, building):
         crown = (h * np.exp(-d2 / (2.0 * sigma * sigma))).astype(F32)
         crown[d2 > r * r] = 0
         crown[building[i0:i1, j0:j1]] = 0
+        before = int((cdsm[i0:i1, j0:j1] > 0).sum())
         cdsm[i0:i1, j0:j1] = np.maximum(cdsm[i0:i1, j0:j1], crown)
+        count += int((cdsm[i0:i1, j0:j1] > 0).sum()) - before
     return cdsm
 
 
@@ -300,3 +305,94 @@ def make_met(altitude_deg: np.ndarray, *, year: int = 2021, doy: int = 180,
     for c in (4, 5, 6, 7, 8, 13, 15, 16, 17, 18, 19, 20):
         met[:, c] = -999.0
     return met
+
+
+@dataclass
+class Edit:
+    kind: str                # "add_tree", "remove_tree", "add_building", "remove_building"
+    rows: tuple              # (r0, r1) bounding rows of the footprint
+    cols: tuple              # (c0, c1)
+    pixels: int              # footprint in pixels
+
+
+def edit_scenarios(base: Site, n: int, seed: int = 20261223, *, max_edits: int = 20):
+    """``n`` edited copies of ``base``: each adds or removes 1..``max_edits`` trees or buildings.
+
+    Footprints are 10..400 px: trees are discs of radius 2..11 px with crowns
+    ``h exp(-d^2 / (2 sigma^2))``, h ~ U(3, 15) m, on non-roof cells;
+    buildings are rectangles of 10..400 px at U(5, 40) m (class 2, no canopy
+    under them). Removing a tree clears the canopy in a disc around a canopy
+    cell; removing a building lowers a rectangle around a roof cell to the
+    ground (class 1). Returns ``[(dsm, cdsm, landcover, [Edit, ...]), ...]``
+    as float32 planes; walls and aspect are left to the caller (they follow
+    from the DSM).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows, cols = base.dsm.shape
+    out = []
+    for _ in range(n):
+        dsm = base.dsm.copy()
+        cdsm = base.cdsm.copy()
+        lc = base.landcover.copy()
+        edits = []
+        for _ in range(int(rng.integers(1, max_edits + 1))):
+            kind = ("add_tree", "remove_tree", "add_building", "remove_building")[int(rng.integers(0, 4))]
+            roof = dsm > base.dem + 2.0
+            if kind in ("add_tree", "remove_tree"):
+                r = int(rng.integers(2, 12))
+                if kind == "add_tree":
+                    ci, cj = int(rng.integers(0, rows)), int(rng.integers(0, cols))
+                else:
+                    cells = np.flatnonzero(cdsm.ravel() > 0)
+                    if cells.size == 0:
+                        continue
+                    ci, cj = divmod(int(cells[rng.integers(0, cells.size)]), cols)
+                i0, i1, j0, j1 = max(0, ci - r), min(rows, ci + r + 1), max(0, cj - r), min(cols, cj + r + 1)
+                ii, jj = np.mgrid[i0:i1, j0:j1]
+                d2 = (ii - ci) ** 2 + (jj - cj) ** 2
+                disc = d2 <= r * r
+                if kind == "add_tree":
+                    h = float(rng.uniform(3.0, 15.0))
+                    sigma = r / 2.0
+                    crown = (h * np.exp(-d2 / (2.0 * sigma * sigma))).astype(F32)
+                    crown[~disc | roof[i0:i1, j0:j1]] = 0
+                    cdsm[i0:i1, j0:j1] = np.maximum(cdsm[i0:i1, j0:j1], crown)
+                else:
+                    cdsm[i0:i1, j0:j1][disc] = 0
+                edits.append(Edit(kind, (i0, i1), (j0, j1), int(disc.sum())))
+            else:
+                area = int(rng.integers(10, 401))
+                h = int(rng.integers(max(2, -(-area // 20)), min(20, area // 2) + 1))
+                w = max(1, min(cols, area // h))
+                h = min(h, rows)
+                if kind == "add_building":
+                    r0, c0 = int(rng.integers(0, rows - h + 1)), int(rng.integers(0, cols - w + 1))
+                else:
+                    cells = np.flatnonzero(roof.ravel())
+                    if cells.size == 0:
+                        continue
+                    ci, cj = divmod(int(cells[rng.integers(0, cells.size)]), cols)
+                    r0, c0 = max(0, min(rows - h, ci - h // 2)), max(0, min(cols - w, cj - w // 2))
+                sl = (slice(r0, r0 + h), slice(c0, c0 + w))
+                if kind == "add_building":
+                    dsm[sl] = np.maximum(dsm[sl], base.dem[sl] + F32(rng.uniform(5.0, 40.0)))
+                    lc[sl] = LC_BUILDING
+                    cdsm[sl] = 0
+                else:
+                    dsm[sl] = base.dem[sl]
+                    lc[sl] = np.where(lc[sl] == LC_BUILDING, F32(LC_PAVED), lc[sl])
+                edits.append(Edit(kind, (r0, r0 + h), (c0, c0 + w), h * w))
+        out.append((dsm.astype(F32), cdsm.astype(F32), lc.astype(F32), edits))
+    return out
+
+
+def site_with(base: Site, dsm: np.ndarray, cdsm: np.ndarray, landcover: np.ndarray, walls=None,
+              aspect=None) -> Site:
+    """A copy of ``base`` with new surfaces; walls/aspect recomputed on the CPU unless given."""
+    if walls is None:
+        walls = findwalls(dsm, 3.0)
+    if aspect is None:
+        aspect = wall_aspect(walls, base.scale, dsm)
+    return Site(dsm=dsm, dem=base.dem, cdsm=cdsm, walls=np.asarray(walls, F32), aspect=np.asarray(aspect, F32),
+                landcover=landcover, pixel_size=base.pixel_size, latitude=base.latitude,
+                longitude=base.longitude, utc_offset=base.utc_offset, seed=base.seed)

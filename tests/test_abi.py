@@ -64,13 +64,17 @@ def test_struct_layout_matches_c(tmp_path):
                    ' offsetof(sw_step_t, pk), offsetof(sw_step_t, pl), offsetof(sw_model_t, walk_hi),'
                    ' sizeof(sw_tile_t), offsetof(sw_tile_t, state), sizeof(sw_wall_filters_t),'
                    ' offsetof(sw_wall_filters_t, off));'
+                   'printf("%zu %zu %zu\\n", sizeof(sw_batch_item_t), offsetof(sw_batch_item_t, out),'
+                   ' offsetof(sw_batch_item_t, flux_block0));'
                    'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
     vals = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     assert vals == [ctypes.sizeof(L.StepStruct), ctypes.sizeof(L.ModelStruct), L.StepStruct.pk.offset,
                     L.StepStruct.pl.offset, L.ModelStruct.walk_hi.offset, ctypes.sizeof(L.TileStruct),
-                    L.TileStruct.state.offset, ctypes.sizeof(L.WallFiltersStruct), L.WallFiltersStruct.off.offset]
+                    L.TileStruct.state.offset, ctypes.sizeof(L.WallFiltersStruct), L.WallFiltersStruct.off.offset,
+                    ctypes.sizeof(L.BatchItemStruct), L.BatchItemStruct.out.offset,
+                    L.BatchItemStruct.flux_block0.offset]
 
 
 def test_loader_fails_loudly_without_library(tmp_path):

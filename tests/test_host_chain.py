@@ -129,3 +129,22 @@ def test_planner_process_matches_in_process_plan():
         got = pp.submit(tile, Job(met, sun, site.latitude, 3, 20)).get(timeout=120)
     assert np.array_equal(got.steps, ref.steps) and np.array_equal(got.sun_tab, ref.sun_tab)
     assert np.array_equal(got.model, ref.model) and got.is_day == ref.is_day and got.walk_halo == ref.walk_halo
+
+
+def test_plan_for_another_shape_equals_planning_that_shape():
+    """The batch plans the scalar chain once and rebuilds only the sun tables per tile shape."""
+    from solweig_project_amd import sunpos, synth
+    from solweig_project_amd.solver import TileInputs, plan_day, plan_for_shape
+    site = synth.make_site(40, 32, seed=11)
+    sun = sunpos.day_sun_table(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=12)
+    base = plan_day(TileInputs.from_site(site), met, sun, site.latitude)
+    for rows, cols in ((17, 90), (64, 64), (300, 12)):
+        other = TileInputs.from_site(synth.make_site(rows, cols, seed=13))
+        ref = plan_day(other, met, sun, site.latitude)
+        got = plan_for_shape(base, rows, cols, other.scale)
+        assert np.array_equal(got.steps, ref.steps), (rows, cols)
+        assert np.array_equal(got.sun_tab, ref.sun_tab), (rows, cols)
+        assert np.array_equal(got.model, ref.model) and got.is_day == ref.is_day
+    # the shape matters: a wide tile marches further east-west than a narrow one
+    assert not np.array_equal(plan_for_shape(base, 17, 90, 1.0).sun_tab, plan_for_shape(base, 300, 12, 1.0).sun_tab)
