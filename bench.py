@@ -39,9 +39,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="side of the CPU-oracle tile (0: the benchmark tile itself)")
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "edits"), default="c2",
                     help="c2: one 1000x1000 tile (default); c3: batched tiles 300/700/1200/2000 squared plus "
-                         "64 edit scenarios of a 500x500 base")
+                         "64 edit scenarios of a 500x500 base; edits: single-tree edits of a 2000x2000 site "
+                         "(p50/p99 latency)")
+    ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run k_pack_heights over 16 Mi pixels (12 B read + 16 B written per pixel) "
                          "as the known-bytes reference for rocprof FETCH_SIZE / WRITE_SIZE")
@@ -218,6 +220,87 @@ def run_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def run_edits(args):
+    """Single-tree edits of a device-resident 2000 x 2000 site: latency from the edit request (new
+    window values on the host) to the edited outputs in HBM, beside a met-only day re-run and a full
+    re-solve of the site."""
+    import numpy as np
+    import torch
+    from solweig_project_amd import incremental as I
+    from solweig_project_amd import synth
+    from solweig_project_amd import walls as WG
+    from solweig_project_amd.solver import HipSolver, TileInputs
+
+    dev = torch.device("cuda:0")
+    n = 2000
+    seed = args.seed + 40
+    log(f"generating the {n}x{n} site (walls/aspect on the device)")
+    base = synth.make_site(n, n, seed=seed, with_aspect=False)
+    w, a = WG.walls_and_aspect(base.dsm, base.scale)
+    tile = TileInputs.from_site(synth.site_with(base, base.dsm, base.cdsm, base.landcover, walls=w, aspect=a))
+    sun = sun_for_site(base.latitude, base.longitude, base.utc_offset, 2021, 180)
+    met = synth.make_met(sun.altitude, seed=seed + 1)
+    solver = HipSolver(dev)
+    st = I.solve_site(solver, tile, met, sun, base.latitude)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = I.solve_site(solver, tile, met, sun, base.latitude)
+    full_ms = (time.perf_counter() - t0) * 1e3
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    lat, modes, fracs = [], [], []
+    ground = np.flatnonzero((st.tile.dsm == st.tile.dem).ravel())
+    for k in range(args.edits + 5):
+        ci, cj = divmod(int(ground[rng.integers(0, ground.size)]), n)
+        r = int(rng.integers(2, 7))
+        r0, r1, c0, c1 = max(0, ci - r), min(n, ci + r + 1), max(0, cj - r), min(n, cj + r + 1)
+        cd = st.tile.cdsm[r0:r1, c0:c1].copy()
+        if rng.random() < 0.5 or not (cd > 0).any():
+            ii, jj = np.mgrid[r0:r1, c0:c1]
+            d2 = (ii - ci) ** 2 + (jj - cj) ** 2
+            crown = (float(rng.uniform(3.0, 15.0)) * np.exp(-d2 / (2.0 * (r / 2.0) ** 2))).astype(np.float32)
+            crown[(d2 > r * r) | (st.tile.dsm[r0:r1, c0:c1] != st.tile.dem[r0:r1, c0:c1])] = 0
+            cd = np.maximum(cd, crown)
+        else:
+            cd[:] = 0
+        info = I.apply_edit_window(solver, st, (r0, r1, c0, c1), cdsm=cd)
+        if k >= 5 and info["mode"] != "none":          # the first edits warm caches
+            lat.append(info["ms"])
+            modes.append(info["mode"])
+            if info["mode"] == "tiles":
+                fracs.append(info["flux_tiles"] / info["grid_tiles"])
+    # met-only request: re-plan the day and re-run it over the whole site
+    met2 = synth.make_met(sun.altitude, seed=seed + 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    from solweig_project_amd.solver import plan_day, _owned_plan
+    plan2 = plan_day(st.tile, met2, sun, base.latitude)
+    dplan2 = _owned_plan(plan2, dev)
+    solver.tile_static(st.scene, dplan2)
+    solver.run_day(st.scene, plan2, dplan2, st.outputs, out=st.out)
+    torch.cuda.synchronize()
+    met_ms = (time.perf_counter() - t0) * 1e3
+    lat = np.array(lat)
+    line = {
+        "metric": "p99 single-edit latency (ms, 2000² site x 24 h, edited outputs in HBM)",
+        "value": round(float(np.percentile(lat, 99)), 2),
+        "unit": "ms",
+        "n_gpus": 1, "steps": len(lat), "warmup": 5, "ms_per_step": round(float(lat.mean()), 2),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 2000² city; random single-tree additions/removals of radius 2-6 px)",
+        "config": {"workload": "single-tree edits of a device-resident 2000x2000x24h site: walls/aspect "
+                               "window, SVF and day recomputed on the tiles the edit can reach "
+                               "(sw_svf_sweep_tiles, sw_solve_day_tiles)", "rows": n, "cols": n},
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p90_ms": round(float(np.percentile(lat, 90)), 2),
+        "max_ms": round(float(lat.max()), 2),
+        "modes": {m: modes.count(m) for m in sorted(set(modes))},
+        "mean_flux_tile_fraction": round(float(np.mean(fracs)), 4) if fracs else None,
+        "met_only_day_rerun_ms": round(met_ms, 2),
+        "full_resolve_ms": round(full_ms, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -226,6 +309,9 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "edits":
+        run_edits(args)
         return
     from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
     pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used

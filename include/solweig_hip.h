@@ -288,6 +288,25 @@ int sw_solve_day_timed(const sw_tile_t *tile, const sw_step_t *steps, const sw_m
                        const int32_t *sun_tab, const int32_t *is_day, void **events, int t0, int t1, float *out,
                        int out_mask, int walk_halo, int reset_state, void *stream);
 
+/* Sparse recomputation over listed 16 x 16 tiles (tile index = tile_row *
+ * ceil(cols / 16) + tile_col; device arrays). Replaces the reference's
+ * sparse recompute of edited regions (solweig_gpu/incremental/sparse_work.py,
+ * solver.py:595-957). sw_svf_sweep_tiles: step counts and padded heights of
+ * the whole tile, then the march and the fold only on the listed tiles (the
+ * others keep their masks and fields). sw_solve_day_tiles: sw_solve_day with
+ * the sun march on sun_tiles and the flux step on flux_tiles only; the caller
+ * lists every tile whose outputs can change in flux_tiles and every tile
+ * within the walk reach of those in sun_tiles. Outputs of the other tiles
+ * are left untouched. */
+int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                       const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                       const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                       float *fields, const int32_t *tiles, int ntiles, int rows, int cols, void *stream);
+int sw_solve_day_tiles(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                       const int32_t *sun_tab, const int32_t *is_day, int t0, int t1, float *out, int out_mask,
+                       int walk_halo, int reset_state, const int32_t *sun_tiles, int nsun,
+                       const int32_t *flux_tiles, int nflux, void *stream);
+
 /* Batched day loop (the reference solves tiles one after another,
  * utci_process.py:1199-1663; edit scenarios one request at a time,
  * solweig_gpu/incremental/worker.py:1588-1595). sw_batch_layout assigns each

@@ -164,6 +164,8 @@ SIGNATURES = {
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
+    "sw_svf_sweep_tiles": (_I, [_VP] * 15 + [_I, _I, _I, _VP]),
+    "sw_solve_day_tiles": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP]),
     "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),
     "sw_solve_batch": (_I, [_VP, _VP, _I, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),

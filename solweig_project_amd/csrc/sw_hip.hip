@@ -389,12 +389,18 @@ __device__ __forceinline__ uint32_t march_svf_checked(__amdgpu_buffer_rsrc_t rs,
 __global__ void __launch_bounds__(256)
 k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
             const int32_t *__restrict__ counts, uint32_t *__restrict__ masks,
-            const uint32_t *__restrict__ stats, int rows, int cols) {
+            const uint32_t *__restrict__ stats, int rows, int cols, const int32_t *__restrict__ tiles,
+            int ntiles) {
     const long n = (long)rows * cols;
     const float hmax = funkey(stats[1]);
     const bool checked = stats[3] != 0u;
     const int tiles_x = (cols + SVF_TILE - 1) / SVF_TILE;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tiles) {                               // listed tiles only (uniform per block)
+        if (tile >= ntiles) return;
+        tile = tiles[tile];
+        if (tile < 0 || tile >= tiles_x * ((rows + SVF_TILE - 1) / SVF_TILE)) return;
+    }
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const int r = (tile / tiles_x) * SVF_TILE + (wv >> 1) * 8 + (ln >> 3);
     const int c = (tile % tiles_x) * SVF_TILE + (wv & 1) * 8 + (ln & 7);
@@ -477,13 +483,11 @@ __device__ __forceinline__ void fold_patch(float sh, float vs, float vb, const f
 // ---------------------------------------------------------------------------
 // SVF fold, literal per-annulus order (bit-exact with the reference's sums)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_svf_fold(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
-           const float *__restrict__ w_iso, const float *__restrict__ w_aniso,
-           const int32_t *__restrict__ ring, const int32_t *__restrict__ nann,
-           const int32_t *__restrict__ dirm, float *__restrict__ out, long n) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+__device__ __forceinline__ void fold_pixel(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
+                                           const float *__restrict__ w_iso, const float *__restrict__ w_aniso,
+                                           const int32_t *__restrict__ ring, const int32_t *__restrict__ nann,
+                                           const int32_t *__restrict__ dirm, float *__restrict__ out, long n,
+                                           long t) {
     uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
     load_masks(masks, n, t, msh, mvs, mv1, mv2);
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
@@ -528,6 +532,34 @@ k_svf_fold(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
     out[5 * n + t] = v0; out[6 * n + t] = v1; out[7 * n + t] = v2; out[8 * n + t] = v3; out[9 * n + t] = v4;
     out[10 * n + t] = b0; out[11 * n + t] = b1; out[12 * n + t] = b2; out[13 * n + t] = b3; out[14 * n + t] = b4;
     out[15 * n + t] = s0 - (1.f - v0) * (1.f - 0.03f);
+}
+
+__global__ void __launch_bounds__(256)
+k_svf_fold(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
+           const float *__restrict__ w_iso, const float *__restrict__ w_aniso,
+           const int32_t *__restrict__ ring, const int32_t *__restrict__ nann,
+           const int32_t *__restrict__ dirm, float *__restrict__ out, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    fold_pixel(masks, veg2, w_iso, w_aniso, ring, nann, dirm, out, n, t);
+}
+
+// the fold over listed 16 x 16 tiles
+__global__ void __launch_bounds__(256)
+k_svf_fold_tiles(const uint32_t *__restrict__ masks, const float *__restrict__ veg2,
+                 const float *__restrict__ w_iso, const float *__restrict__ w_aniso,
+                 const int32_t *__restrict__ ring, const int32_t *__restrict__ nann,
+                 const int32_t *__restrict__ dirm, float *__restrict__ out, int rows, int cols,
+                 const int32_t *__restrict__ tiles, int ntiles) {
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= ntiles) return;
+    const int tiles_x = (cols + SVF_TILE - 1) / SVF_TILE;
+    const int tile = tiles[g];
+    if (tile < 0 || tile >= tiles_x * ((rows + SVF_TILE - 1) / SVF_TILE)) return;
+    const int r = (tile / tiles_x) * SVF_TILE + (threadIdx.x >> 4);
+    const int c = (tile % tiles_x) * SVF_TILE + (threadIdx.x & 15);
+    if (r >= rows || c >= cols) return;
+    fold_pixel(masks, veg2, w_iso, w_aniso, ring, nann, dirm, out, (long)rows * cols, (long)r * cols + c);
 }
 
 __global__ void k_unpack(const uint32_t *__restrict__ masks, float *__restrict__ shm,
@@ -733,6 +765,7 @@ __device__ __forceinline__ float pow4(float x) {
 #define FLUX_TY 16
 
 // one 256-pixel block of the sun step; `tile` is the block's index within its tile
+template <bool TILED>
 __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, const float *__restrict__ walls,
                                               const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
                                               const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
@@ -742,11 +775,26 @@ __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, con
     const long n = (long)rows * cols;
     const float amax = funkey(stats[0]);
     const float hmax = funkey(stats[1]);
-    const long t = (long)tile * 256 + threadIdx.x;
-    const bool valid = (t < n);
+    long t;
+    bool valid;
+    int r, c;
+    if (TILED) {
+        // a 16 x 16 tile of the flux grid, 8 x 8 pixels per wave
+        const int tiles_x = (cols + FLUX_TX - 1) / FLUX_TX;
+        const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+        const int rr = (tile / tiles_x) * FLUX_TY + (wv >> 1) * 8 + (ln >> 3);
+        const int cc = (tile % tiles_x) * FLUX_TX + (wv & 1) * 8 + (ln & 7);
+        valid = rr < rows && cc < cols;
+        r = valid ? rr : 0;
+        c = valid ? cc : 0;
+        t = (long)r * cols + c;
+    } else {
+        t = (long)tile * 256 + threadIdx.x;
+        valid = (t < n);
+        r = valid ? (int)(t / cols) : 0;
+        c = valid ? (int)(t - (long)r * cols) : 0;
+    }
     const sw_step_t *sp = steps + step;
-    const int r = valid ? (int)(t / cols) : 0;
-    const int c = valid ? (int)(t - (long)r * cols) : 0;
     float at = 0.f, vegt = 0.f, w = 0.f;
     if (valid) {
         const float4 h = hp[t];
@@ -798,8 +846,23 @@ k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
            const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
            const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
            const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols) {
-    sun_step_body(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
-                  xcd_tile(blockIdx.x, gridDim.x));
+    sun_step_body<false>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
+                         xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// the sun step over listed 16 x 16 tiles
+__global__ void __launch_bounds__(256)
+k_sun_tiles(const float4 *__restrict__ hp, const float *__restrict__ walls,
+            const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
+            const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
+            const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
+            const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols,
+            const int32_t *__restrict__ tiles, int ntiles) {
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= ntiles) return;
+    const int tile = tiles[g];
+    if (tile < 0 || tile >= ((cols + FLUX_TX - 1) / FLUX_TX) * ((rows + FLUX_TY - 1) / FLUX_TY)) return;
+    sun_step_body<true>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols, tile);
 }
 
 // The batch kernels read the item table through this mirror of
@@ -852,7 +915,7 @@ k_sun_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const
     if (g >= total) return;
     const BatchItemDev *it = items + batch_owner<0>(items, nitems, g);
     const BatchTileDev &T = it->tile;
-    sun_step_body((const float4 *)T.hpack, (const float *)T.walls, (const float *)T.aspect_rad,
+    sun_step_body<false>((const float4 *)T.hpack, (const float *)T.walls, (const float *)T.aspect_rad,
                   (const uint8_t *)T.cls, (const float *)T.buildings, (const sw_step_t *)it->steps, model,
                   (const int4 *)it->sun_tab, (const uint32_t *)T.stats, (uint2 *)T.src, step, T.rows, T.cols,
                   g - it->sun_block0);
@@ -1306,6 +1369,23 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
 
 template <bool WALK>
 __global__ void __launch_bounds__(256)
+k_flux_tiles(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
+             const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
+             const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
+             const float *__restrict__ bu, const uint8_t *__restrict__ cls,
+             const sw_step_t *__restrict__ steps, const sw_model_t *__restrict__ model,
+             float *__restrict__ state, float *__restrict__ out, int out_mask, int step, int halo,
+             int rows, int cols, const int32_t *__restrict__ tiles, int ntiles) {
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= ntiles) return;
+    const int tile = tiles[g];
+    if (tile < 0) return;                      // flux_body skips tiles past the grid
+    flux_body<WALK>(masks, src, svfbuveg, tan_asvf, tan_sa, walkface, bu, cls, steps, model, state, out, out_mask,
+                    step, halo, rows, cols, tile);
+}
+
+template <bool WALK>
+__global__ void __launch_bounds__(256)
 k_flux_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const sw_model_t *__restrict__ model,
              int out_mask, int out_step, int step, int halo) {
     const int g = xcd_tile(blockIdx.x, gridDim.x);
@@ -1460,8 +1540,94 @@ int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int
     const long tiles = (long)((rows + SVF_TILE - 1) / SVF_TILE) * ((cols + SVF_TILE - 1) / SVF_TILE);
     dim3 grid((unsigned)(((tiles + 7) / 8) * 8), SW_MASK_WORDS);
     hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float *)hpad, offs, (const int4 *)tab,
-                       counts, masks, stats, rows, cols);
+                       counts, masks, stats, rows, cols, (const int32_t *)nullptr, 0);
     return hip_check(hipGetLastError(), "k_svf_march");
+}
+
+int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                       const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                       const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                       float *fields, const int32_t *tiles, int ntiles, int rows, int cols, void *stream) {
+    int rc = check_shape(rows, cols, "sw_svf_sweep_tiles");
+    if (rc) return rc;
+    if (!hpack || !stats || !offs || !tab || !vegdsm2 || !w_iso || !w_aniso || !ring || !nann || !dirm || !counts ||
+        !hpad || !masks || !fields || (ntiles > 0 && !tiles))
+        return set_err(SW_E_ARG, "sw_svf_sweep_tiles: NULL pointer");
+    if (ntiles < 0) return set_err(SW_E_ARG, "sw_svf_sweep_tiles: negative tile count");
+    if (ntiles == 0) return SW_OK;
+    const long padded = (long)(rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD);
+    if (padded * 12 >= (1L << 31)) return set_err(SW_E_ARG, "sw_svf_sweep_tiles: tile too large");
+    hipLaunchKernelGGL(k_svf_counts, dim3(SW_NPATCH), dim3(64), 0, S(stream), (const int4 *)tab, offs, stats,
+                       counts);
+    rc = hip_check(hipGetLastError(), "k_svf_counts");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pad_heights, dim3(grid_for(padded, 256)), dim3(256), 0, S(stream), (const float4 *)hpack,
+                       (float *)hpad, stats, rows, cols);
+    rc = hip_check(hipGetLastError(), "k_pad_heights");
+    if (rc) return rc;
+    const unsigned g = (unsigned)(((ntiles + 7) / 8) * 8);
+    hipLaunchKernelGGL(k_svf_march, dim3(g, SW_MASK_WORDS), dim3(256), 0, S(stream), (const float *)hpad, offs,
+                       (const int4 *)tab, counts, masks, stats, rows, cols, tiles, ntiles);
+    rc = hip_check(hipGetLastError(), "k_svf_march");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_svf_fold_tiles, dim3(g), dim3(256), 0, S(stream), masks, vegdsm2, w_iso, w_aniso, ring,
+                       nann, dirm, fields, rows, cols, tiles, ntiles);
+    return hip_check(hipGetLastError(), "k_svf_fold_tiles");
+}
+
+int sw_solve_day_tiles(const sw_tile_t *tile, const sw_step_t *steps, const sw_model_t *model,
+                       const int32_t *sun_tab, const int32_t *is_day, int t0, int t1, float *out, int out_mask,
+                       int walk_halo, int reset_state, const int32_t *sun_tiles, int nsun,
+                       const int32_t *flux_tiles, int nflux, void *stream) {
+    if (!tile || !steps || !model || !sun_tab || !is_day || !out || (nsun > 0 && !sun_tiles) ||
+        (nflux > 0 && !flux_tiles))
+        return set_err(SW_E_ARG, "sw_solve_day_tiles: NULL pointer");
+    int rc = check_shape(tile->rows, tile->cols, "sw_solve_day_tiles");
+    if (rc) return rc;
+    if (t0 < 0 || t1 < t0) return set_err(SW_E_ARG, "sw_solve_day_tiles: invalid step range");
+    if (out_mask <= 0 || out_mask > 127) return set_err(SW_E_ARG, "sw_solve_day_tiles: out_mask selects nothing");
+    if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_solve_day_tiles: walk halo out of range");
+    if (nsun < 0 || nflux < 0) return set_err(SW_E_ARG, "sw_solve_day_tiles: negative tile count");
+    const sw_tile_t &T = *tile;
+    const int rows = T.rows, cols = T.cols;
+    const long n = (long)rows * cols;
+    const long stride = (long)__builtin_popcount((unsigned)out_mask) * n;
+    if (reset_state) {
+        rc = sw_fill_f32(T.state, 0.f, 6 * (int64_t)n, stream);
+        if (rc) return rc;
+    }
+    if (nflux == 0) return SW_OK;
+    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    if (lds > 65536) {
+        rc = hip_check(hipFuncSetAttribute((const void *)k_flux_tiles<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                       "hipFuncSetAttribute");
+        if (rc) return rc;
+    }
+    const unsigned gs = (unsigned)(((nsun + 7) / 8) * 8), gf = (unsigned)(((nflux + 7) / 8) * 8);
+    for (int i = t0; i < t1; ++i) {
+        float *o = out + (long)(i - t0) * stride;
+        if (is_day[i - t0]) {
+            if (nsun > 0) {
+                hipLaunchKernelGGL(k_sun_tiles, dim3(gs), dim3(256), 0, S(stream), (const float4 *)T.hpack, T.walls,
+                                   T.aspect_rad, T.cls, T.buildings, steps, model, (const int4 *)sun_tab, T.stats,
+                                   (uint2 *)T.src, i, rows, cols, sun_tiles, nsun);
+                rc = hip_check(hipGetLastError(), "k_sun_tiles");
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_flux_tiles<true>, dim3(gf), dim3(256), lds, S(stream), T.masks,
+                               (const uint2 *)T.src, T.svfbuveg, T.tan_asvf, T.tan_svfalfa, T.walkface, T.buildings,
+                               T.cls, steps, model, T.state, o, out_mask, i, walk_halo, rows, cols, flux_tiles, nflux);
+            rc = hip_check(hipGetLastError(), "k_flux_tiles");
+        } else {
+            hipLaunchKernelGGL(k_flux_tiles<false>, dim3(gf), dim3(256), 0, S(stream), T.masks, (const uint2 *)T.src,
+                               T.svfbuveg, T.tan_asvf, T.tan_svfalfa, T.walkface, T.buildings, T.cls, steps, model,
+                               T.state, o, out_mask, i, -1, rows, cols, flux_tiles, nflux);
+            rc = hip_check(hipGetLastError(), "k_flux_tiles<night>");
+        }
+        if (rc) return rc;
+    }
+    return SW_OK;
 }
 
 int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso, const float *w_aniso,

@@ -533,10 +533,12 @@ class HipSolver:
             L.call("sw_compose", P(i["dsm"]), P(i["dem"]), P(i["cdsm"]), P(i["aspect_deg"]), lc, P(ds.hpack),
                    P(ds.vegdsm2), P(ds.buildings), P(ds.cls), P(ds.aspect), P(ds.stats), ds.rows, ds.cols, sp)
 
-    def svf(self, ds: DeviceScene, stream=None, tables=None):
+    def svf(self, ds: DeviceScene, stream=None, tables=None, compose=True):
+        """Compose (unless ``compose`` is False) and the SVF sweep; returns the field planes."""
         sp = _stream_ptr(stream)
         P = L.ptr
-        self.compose(ds, stream)
+        if compose:
+            self.compose(ds, stream)
         offs, tab = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
         with self.timer.span("svf_march", stream):
             L.call("sw_svf_march", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.hpad), P(ds.masks),
