@@ -362,6 +362,55 @@ int sw_wall_filter_size(double scale);
 int sw_wall_aspect(const float *walls, const float *dsm, const sw_wall_filters_t *filters, float *aspect,
                    int rows, int cols, double scale, void *stream);
 
+/* ---- SVF surrogate network (no counterpart in the reference: a learned
+ * replacement of the SVF sweep, gated by and falling back to the exact
+ * kernel) ---------------------------------------------------------------------
+ * Activations are NHWC bf16 (uint16_t storage) with channel counts that are
+ * multiples of 16; weights are bf16 [cout][ks * ks][cin]; biases, gradients
+ * and optimiser state are f32. A network pass is a list of ops run in order
+ * on one stream (capturable); sw_cnn_fwd and sw_cnn_bwd run forward and
+ * backward lists. */
+#define SW_CNN_CONV 1             /* out = act(conv(in0 [up] ++ in1) + bias), split at o1 */
+#define SW_CNN_CONV_BWD_DATA 2    /* d(input) = conv(dz = in0, flipped weight); ws holds the flip */
+#define SW_CNN_CONV_BWD_WEIGHT 3  /* gw, gb from dz and the forward inputs; ws: chunk partials */
+#define SW_CNN_POOL2 4            /* 2x2 average pool, in0 (n, h, w, c1) -> out0 (n, h/2, w/2, c1) */
+#define SW_CNN_POOL2_BWD 5        /* out0 = in1 (optional addend) + 0.25 * in0 broadcast */
+#define SW_CNN_UP2_BWD 6          /* out0 (half) = 2x2 sums of in0 (n, h, w, c1) */
+#define SW_CNN_RELU_BWD 7         /* out0 = in0 where in1 > 0 (c1 channels) */
+#define SW_CNN_HEAD_LOSS 8        /* sigmoid-MSE over the core window; ws: per-block losses */
+typedef struct sw_cnn_op {
+    int32_t kind;
+    int32_t n, h, w;              /* batch and full-resolution size of the op */
+    int32_t c1, c2, up, o1;       /* input channels (in0 half resolution when up), output split */
+    int32_t cout, ks, act, nchunk;/* act: 0 none, 1 ReLU, 2 sigmoid; nchunk: weight-gradient split */
+    int32_t core, c0, nl, pad;    /* head loss: core window [c0, c0 + core), nl labelled channels */
+    const void *in0, *in1;
+    void *out0, *out1;
+    float *outf;
+    const void *weight;
+    const float *bias;
+    const void *dz;
+    float *gw, *gb;
+    void *ws;
+} sw_cnn_op_t;
+
+int sw_cnn_fwd(const sw_cnn_op_t *ops, int nops, void *stream);
+int sw_cnn_bwd(const sw_cnn_op_t *ops, int nops, void *stream);
+/* workspace bytes an op needs (data gradient: the flipped kernel; weight gradient: chunk partials) */
+int sw_cnn_bwd_scratch_bytes(const sw_cnn_op_t *op, int64_t *bytes);
+/* Adam step on f32 master weights (bias-corrected with the step count); p_bf16 receives the bf16 copy */
+int sw_adam(float *p, float *m, float *v, const float *g, uint16_t *p_bf16, int64_t n, float lr, float b1, float b2,
+            float eps, int step, void *stream);
+/* network input from composed heights (hpack of sw_compose) at window origins (int32 pairs row, col) */
+int sw_cnn_pack_input(const void *hpack, const int32_t *origins, uint16_t *x, int n, int hw, int rows, int cols,
+                      void *stream);
+/* labels [n][core][core][nl] from SVF field planes at the window cores (origin + c0) */
+int sw_cnn_gather_labels(const float *fields, const int32_t *origins, const int32_t *field_map, float *labels, int n,
+                         int core, int c0, int nl, int rows, int cols, void *stream);
+/* surrogate outputs (f32 [n][hw][hw][cs]) of the window cores into SVF field planes */
+int sw_cnn_scatter_cores(const float *out, const int32_t *origins, const int32_t *field_map, float *fields, int n,
+                         int hw, int cs, int core, int c0, int nl, int rows, int cols, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

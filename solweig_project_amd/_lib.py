@@ -99,6 +99,18 @@ class BatchItemStruct(ctypes.Structure):
     ]
 
 
+class CnnOp(ctypes.Structure):
+    """Mirror of ``sw_cnn_op_t`` (one op of a surrogate network pass)."""
+
+    _fields_ = [(k, _c_i32) for k in ("kind", "n", "h", "w", "c1", "c2", "up", "o1", "cout", "ks", "act", "nchunk",
+                                      "core", "c0", "nl", "pad")] + \
+        [(k, ctypes.c_void_p) for k in ("in0", "in1", "out0", "out1", "outf", "weight", "bias", "dz", "gw", "gb",
+                                        "ws")]
+
+
+CNN_CONV, CNN_CONV_BWD_DATA, CNN_CONV_BWD_WEIGHT, CNN_POOL2, CNN_POOL2_BWD, CNN_UP2_BWD, CNN_RELU_BWD, \
+    CNN_HEAD_LOSS = range(1, 9)
+
 WALL_ROTATIONS = 180
 WALL_LIST_CAP = 128
 
@@ -168,6 +180,13 @@ SIGNATURES = {
     "sw_solve_day_tiles": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP]),
     "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),
     "sw_solve_batch": (_I, [_VP, _VP, _I, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "sw_cnn_fwd": (_I, [_VP, _I, _VP]),
+    "sw_cnn_bwd": (_I, [_VP, _I, _VP]),
+    "sw_cnn_bwd_scratch_bytes": (_I, [_VP, _VP]),
+    "sw_adam": (_I, [_VP] * 5 + [_I64, _F, _F, _F, _F, _I, _VP]),
+    "sw_cnn_pack_input": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "sw_cnn_gather_labels": (_I, [_VP] * 4 + [_I] * 6 + [_VP]),
+    "sw_cnn_scatter_cores": (_I, [_VP] * 4 + [_I] * 8 + [_VP]),
     "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),
     "sw_wall_filter_size": (_I, [ctypes.c_double]),
     "sw_wall_aspect": (_I, [_VP] * 4 + [_I, _I, ctypes.c_double, _VP]),

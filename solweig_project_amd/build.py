@@ -15,7 +15,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libsolweig_hip.so"
-SOURCES = [CSRC / "sw_hip.hip", CSRC / "sw_runtime.hip", CSRC / "sw_walls.hip"]
+SOURCES = [CSRC / "sw_hip.hip", CSRC / "sw_runtime.hip", CSRC / "sw_walls.hip", CSRC / "sw_cnn.hip"]
 ARCH = os.environ.get("SW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,313 @@
+"""Learned SVF surrogate: a 4-level convolutional encoder-decoder run and
+trained on the GPU by the library's own BF16 MFMA kernels (``sw_cnn_fwd``,
+``sw_cnn_bwd``, ``sw_adam``), gated by and falling back to the exact SVF
+kernel.
+
+The reference has no surrogate; this is a new path beside the exact sweep.
+Input: 384 x 384 windows (256 x 256 core + 64-pixel halo) of the composed
+march heights {DSM, vegetation DSM, trunk DSM} / 50 m. Output: the 10 SVF
+fields total, E, S, W, N and their vegetation variants (the first 10
+field planes of the exact sweep) over the core. Training data: windows of
+seeded synthetic cities labelled by the exact SVF sweep on the whole city
+(so labels see occluders beyond the halo). At inference the gate compares
+the surrogate with the exact kernel on sampled 16 x 16 tiles of every
+window core and re-solves the windows above the tolerance exactly.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+LAYERS = ("e1a", "e1", "e2a", "e2", "e3a", "e3", "e4a", "e4", "d3a", "d3", "d2a", "d2", "d1a", "d1", "head")
+CIN0 = 16           # input channels (3 heights, zero padded to one MFMA K chunk)
+COUT_HEAD = 16      # head channels (10 fields, zero padded)
+N_FIELDS = 10
+FIELD_MAP = list(range(N_FIELDS))   # svf, svfE, svfS, svfW, svfN, svfveg, svfEveg, svfSveg, svfWveg, svfNveg
+HALO = 64
+CORE = 256
+HW = CORE + 2 * HALO
+
+
+def layer_shapes(ch):
+    c1, c2, c3, c4 = ch
+    return {"e1a": (c1, 3, CIN0), "e1": (c1, 3, c1), "e2a": (c2, 3, c1), "e2": (c2, 3, c2),
+            "e3a": (c3, 3, c2), "e3": (c3, 3, c3), "e4a": (c4, 3, c3), "e4": (c4, 3, c4),
+            "d3a": (c3, 3, c4 + c3), "d3": (c3, 3, c3), "d2a": (c2, 3, c3 + c2), "d2": (c2, 3, c2),
+            "d1a": (c1, 3, c2 + c1), "d1": (c1, 3, c1), "head": (COUT_HEAD, 1, c1)}
+
+
+class SurrogateNet:
+    """Parameters (f32 master + bf16 copy), gradients and Adam state in flat device buffers."""
+
+    def __init__(self, channels=(32, 64, 128, 256), device="cuda:0", seed: int = 0):
+        if any(c % 16 for c in channels) or len(channels) != 4:
+            raise ValueError("four level channel counts, each a multiple of 16")
+        self.channels = tuple(int(c) for c in channels)
+        self.device = torch.device(device)
+        self.shapes = layer_shapes(self.channels)
+        self.offsets = {}
+        off = 0
+        for name in LAYERS:
+            co, ks, ci = self.shapes[name]
+            nw = co * ks * ks * ci
+            self.offsets[name] = (off, nw, off + nw, co)
+            off += nw + co
+        self.n_params = off
+        rng = np.random.Generator(np.random.PCG64(seed))
+        host = np.zeros(off, np.float32)
+        for name in LAYERS:
+            co, ks, ci = self.shapes[name]
+            w0, nw, b0, _ = self.offsets[name]
+            fan = ks * ks * (3 if name == "e1a" else ci)
+            w = rng.standard_normal((co, ks * ks, ci)).astype(np.float32) * np.float32(np.sqrt(2.0 / fan))
+            if name == "e1a":
+                w[:, :, 3:] = 0
+            if name == "head":
+                w *= np.float32(0.1)
+                w[N_FIELDS:] = 0
+                host[b0:b0 + N_FIELDS] = 0.4          # sigmoid(0.4) ~ a typical SVF
+            host[w0:w0 + nw] = w.ravel()
+        dev = self.device
+        self.p = torch.from_numpy(host).to(dev)
+        self.pb = self.p.to(torch.bfloat16)
+        self.g = torch.zeros_like(self.p)
+        self.m = torch.zeros_like(self.p)
+        self.v = torch.zeros_like(self.p)
+        self.step = 0
+
+    def w(self, name):
+        w0, nw, _, _ = self.offsets[name]
+        return self.pb[w0:w0 + nw]
+
+    def b(self, name):
+        _, _, b0, co = self.offsets[name]
+        return self.p[b0:b0 + co]
+
+    def gw(self, name):
+        w0, nw, _, _ = self.offsets[name]
+        return self.g[w0:w0 + nw]
+
+    def gb(self, name):
+        _, _, b0, co = self.offsets[name]
+        return self.g[b0:b0 + co]
+
+    def numpy_params(self):
+        """{name: (w, b)} as float32 numpy arrays (weights as the kernels read them: bf16-rounded)."""
+        pb = self.pb.float().cpu().numpy()
+        p = self.p.cpu().numpy()
+        out = {}
+        for name in LAYERS:
+            co, ks, ci = self.shapes[name]
+            w0, nw, b0, _ = self.offsets[name]
+            out[name] = (pb[w0:w0 + nw].reshape(co, ks * ks, ci).copy(), p[b0:b0 + co].copy())
+        return out
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, stream=None):
+        self.step += 1
+        L.call("sw_adam", L.ptr(self.p), L.ptr(self.m), L.ptr(self.v), L.ptr(self.g), L.ptr(self.pb),
+               self.n_params, lr, b1, b2, eps, self.step, _sp(stream))
+
+    def state_dict(self):
+        return {"channels": np.array(self.channels, np.int32), "p": self.p.cpu().numpy()}
+
+    def load_state_dict(self, d):
+        if tuple(int(c) for c in d["channels"]) != self.channels:
+            raise ValueError("channel counts differ")
+        p = torch.from_numpy(np.asarray(d["p"], np.float32)).to(self.device)
+        if p.numel() != self.n_params:
+            raise ValueError("parameter count differs")
+        self.p.copy_(p)
+        self.pb.copy_(p.to(torch.bfloat16))
+
+
+def _sp(stream):
+    return stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+
+
+class Passes:
+    """Activation/gradient buffers and the forward and backward op lists for one batch shape."""
+
+    def __init__(self, net: SurrogateNet, batch: int, hw: int = HW, core: int = CORE, c0: int = HALO,
+                 nchunk: int = 64):
+        if hw % 8:
+            raise ValueError("window size must be a multiple of 8")
+        self.net, self.n, self.hw, self.core, self.c0 = net, batch, hw, core, c0
+        c1, c2, c3, c4 = net.channels
+        dev = net.device
+        bf = torch.bfloat16
+        dims = {1: hw, 2: hw // 2, 3: hw // 4, 4: hw // 8}
+        lvl = {"e1a": 1, "e1": 1, "e2a": 2, "e2": 2, "e3a": 3, "e3": 3, "e4a": 4, "e4": 4,
+               "d3a": 3, "d3": 3, "d2a": 2, "d2": 2, "d1a": 1, "d1": 1}
+        self.a, self.d = {}, {}
+
+        def t(level, c):
+            s = dims[level]
+            return torch.zeros((batch, s, s, c), dtype=bf, device=dev)
+        self.x = t(1, CIN0)
+        for name, lv in lvl.items():
+            co = net.shapes[name][0]
+            self.a[name] = t(lv, co)
+            self.d[name] = t(lv, co)
+        for k, lv, c in (("p1", 2, c1), ("p2", 3, c2), ("p3", 4, c3)):
+            self.a[k] = t(lv, c)
+            self.d[k] = t(lv, c)
+        # data gradients of the concat layers: upsampled part (full res) and skip part
+        self.d["up4"], self.d["sk3"] = t(3, c4), t(3, c3)
+        self.d["up3"], self.d["sk2"] = t(2, c3), t(2, c2)
+        self.d["up2"], self.d["sk1"] = t(1, c2), t(1, c1)
+        self.gz = {}                 # ReLU-gradient scratch per (level, channels), reused layer to layer
+        self.out = torch.zeros((batch, hw, hw, COUT_HEAD), dtype=torch.float32, device=dev)
+        self.dhead = t(1, COUT_HEAD)
+        self.labels = torch.zeros((batch, core, core, N_FIELDS), dtype=torch.float32, device=dev)
+        nparts = -(-(batch * hw * hw * COUT_HEAD) // 256)
+        self.parts = torch.zeros(nparts, dtype=torch.float32, device=dev)
+        self.dims, self.lvl, self.nchunk = dims, lvl, nchunk
+        self.fwd_ops = self._forward_ops()
+        self.bwd_ops, ws = self._backward_ops()
+        self.ws = torch.zeros(max(ws // 4, 1), dtype=torch.float32, device=dev)
+        for op in self.bwd_ops:
+            if op.kind in (L.CNN_CONV_BWD_DATA, L.CNN_CONV_BWD_WEIGHT):
+                op.ws = L.ptr(self.ws)
+        self.fwd_arr = (L.CnnOp * len(self.fwd_ops))(*self.fwd_ops)
+        self.bwd_arr = (L.CnnOp * len(self.bwd_ops))(*self.bwd_ops)
+
+    # -- op builders -----------------------------------------------------------
+    def _conv(self, name, in0, in1=None, up=False, out=None, act=1, outf=None):
+        net = self.net
+        co, ks, ci = net.shapes[name]
+        c1 = in0.shape[3]
+        c2 = 0 if in1 is None else in1.shape[3]
+        assert c1 + c2 == ci, (name, c1, c2, ci)
+        s = self.dims[self.lvl.get(name, 1)]
+        o = L.CnnOp(kind=L.CNN_CONV, n=self.n, h=s, w=s, c1=c1, c2=c2, up=1 if up else 0, o1=co, cout=co, ks=ks,
+                    act=act)
+        o.in0, o.in1 = L.ptr(in0), (L.ptr(in1) if in1 is not None else None)
+        o.out0 = L.ptr(out) if out is not None else None
+        o.outf = L.ptr(outf) if outf is not None else None
+        o.weight, o.bias = L.ptr(net.w(name)), L.ptr(net.b(name))
+        return o
+
+    def _forward_ops(self):
+        a, ops = self.a, []
+        ops.append(self._conv("e1a", self.x, out=a["e1a"]))
+        ops.append(self._conv("e1", a["e1a"], out=a["e1"]))
+        for lv, (e, p) in enumerate((("e1", "p1"), ("e2", "p2"), ("e3", "p3")), start=1):
+            s = self.dims[lv]
+            o = L.CnnOp(kind=L.CNN_POOL2, n=self.n, h=s, w=s, c1=a[e].shape[3])
+            o.in0, o.out0 = L.ptr(a[e]), L.ptr(a[p])
+            ops.append(o)
+            nxt = ("e2a", "e2") if lv == 1 else (("e3a", "e3") if lv == 2 else ("e4a", "e4"))
+            ops.append(self._conv(nxt[0], a[p], out=a[nxt[0]]))
+            ops.append(self._conv(nxt[1], a[nxt[0]], out=a[nxt[1]]))
+        ops.append(self._conv("d3a", a["e4"], a["e3"], up=True, out=a["d3a"]))
+        ops.append(self._conv("d3", a["d3a"], out=a["d3"]))
+        ops.append(self._conv("d2a", a["d3"], a["e2"], up=True, out=a["d2a"]))
+        ops.append(self._conv("d2", a["d2a"], out=a["d2"]))
+        ops.append(self._conv("d1a", a["d2"], a["e1"], up=True, out=a["d1a"]))
+        ops.append(self._conv("d1", a["d1a"], out=a["d1"]))
+        ops.append(self._conv("head", a["d1"], act=2, outf=self.out))
+        return ops
+
+    def _backward_ops(self):
+        net, a, d = self.net, self.a, self.d
+        ops = []
+        ws_need = 0
+        s1 = self.dims[1]
+        o = L.CnnOp(kind=L.CNN_HEAD_LOSS, n=self.n, h=s1, w=s1, c1=COUT_HEAD, core=self.core, c0=self.c0,
+                    nl=N_FIELDS)
+        o.outf, o.in0, o.out0, o.ws = L.ptr(self.out), L.ptr(self.labels), L.ptr(self.dhead), L.ptr(self.parts)
+        ops.append(o)
+
+        def scratch(op):
+            b = ctypes.c_int64()
+            L.call("sw_cnn_bwd_scratch_bytes", ctypes.byref(op), ctypes.byref(b))
+            return b.value
+
+        def back(name, dz, in0, in1=None, up=False, dx0=None, dx1=None):
+            nonlocal ws_need
+            co, ks, ci = net.shapes[name]
+            c1 = in0.shape[3]
+            c2 = 0 if in1 is None else in1.shape[3]
+            s = self.dims[self.lvl.get(name, 1)]
+            w = L.CnnOp(kind=L.CNN_CONV_BWD_WEIGHT, n=self.n, h=s, w=s, c1=c1, c2=c2, up=1 if up else 0, o1=co,
+                        cout=co, ks=ks, nchunk=self.nchunk)
+            w.in0, w.in1 = L.ptr(in0), (L.ptr(in1) if in1 is not None else None)
+            w.dz, w.gw, w.gb, w.weight = L.ptr(dz), L.ptr(net.gw(name)), L.ptr(net.gb(name)), L.ptr(net.w(name))
+            ws_need = max(ws_need, scratch(w))
+            ops.append(w)
+            if dx0 is None:
+                return
+            dd = L.CnnOp(kind=L.CNN_CONV_BWD_DATA, n=self.n, h=s, w=s, c1=co, c2=0, up=0, o1=c1, cout=ci, ks=ks,
+                         act=0)
+            dd.in0, dd.weight = L.ptr(dz), L.ptr(net.w(name))
+            dd.out0, dd.out1 = L.ptr(dx0), (L.ptr(dx1) if dx1 is not None else None)
+            ws_need = max(ws_need, scratch(dd))
+            ops.append(dd)
+
+        def relu_b(dy, y, level):
+            s = self.dims[level]
+            c = y.shape[3]
+            z = self.gz.get((level, c))
+            if z is None:
+                z = self.gz[(level, c)] = torch.zeros_like(y)
+            o = L.CnnOp(kind=L.CNN_RELU_BWD, n=self.n, h=s, w=s, c1=c)
+            o.in0, o.in1, o.out0 = L.ptr(dy), L.ptr(y), L.ptr(z)
+            ops.append(o)
+            return z
+
+        def pool_b(dout, add, out, level):
+            s = self.dims[level]
+            o = L.CnnOp(kind=L.CNN_POOL2_BWD, n=self.n, h=s, w=s, c1=out.shape[3])
+            o.in0, o.in1, o.out0 = L.ptr(dout), L.ptr(add), L.ptr(out)
+            ops.append(o)
+
+        def up_b(dfull, dhalf, level):
+            s = self.dims[level]
+            o = L.CnnOp(kind=L.CNN_UP2_BWD, n=self.n, h=s, w=s, c1=dfull.shape[3])
+            o.in0, o.out0 = L.ptr(dfull), L.ptr(dhalf)
+            ops.append(o)
+
+        back("head", self.dhead, a["d1"], dx0=d["d1"])
+        back("d1", relu_b(d["d1"], a["d1"], 1), a["d1a"], dx0=d["d1a"])
+        back("d1a", relu_b(d["d1a"], a["d1a"], 1), a["d2"], a["e1"], up=True, dx0=d["up2"], dx1=d["sk1"])
+        up_b(d["up2"], d["d2"], 1)
+        back("d2", relu_b(d["d2"], a["d2"], 2), a["d2a"], dx0=d["d2a"])
+        back("d2a", relu_b(d["d2a"], a["d2a"], 2), a["d3"], a["e2"], up=True, dx0=d["up3"], dx1=d["sk2"])
+        up_b(d["up3"], d["d3"], 2)
+        back("d3", relu_b(d["d3"], a["d3"], 3), a["d3a"], dx0=d["d3a"])
+        back("d3a", relu_b(d["d3a"], a["d3a"], 3), a["e4"], a["e3"], up=True, dx0=d["up4"], dx1=d["sk3"])
+        up_b(d["up4"], d["e4"], 3)
+        back("e4", relu_b(d["e4"], a["e4"], 4), a["e4a"], dx0=d["e4a"])
+        back("e4a", relu_b(d["e4a"], a["e4a"], 4), a["p3"], dx0=d["p3"])
+        pool_b(d["p3"], d["sk3"], d["e3"], 3)
+        back("e3", relu_b(d["e3"], a["e3"], 3), a["e3a"], dx0=d["e3a"])
+        back("e3a", relu_b(d["e3a"], a["e3a"], 3), a["p2"], dx0=d["p2"])
+        pool_b(d["p2"], d["sk2"], d["e2"], 2)
+        back("e2", relu_b(d["e2"], a["e2"], 2), a["e2a"], dx0=d["e2a"])
+        back("e2a", relu_b(d["e2a"], a["e2a"], 2), a["p1"], dx0=d["p1"])
+        pool_b(d["p1"], d["sk1"], d["e1"], 1)
+        back("e1", relu_b(d["e1"], a["e1"], 1), a["e1a"], dx0=d["e1a"])
+        back("e1a", relu_b(d["e1a"], a["e1a"], 1), self.x)
+        return ops, ws_need
+
+    # -- passes ----------------------------------------------------------------
+    def forward(self, stream=None):
+        L.call("sw_cnn_fwd", self.fwd_arr, len(self.fwd_ops), _sp(stream))
+        return self.out
+
+    def backward(self, stream=None):
+        L.call("sw_cnn_bwd", self.bwd_arr, len(self.bwd_ops), _sp(stream))
+
+    def loss(self) -> float:
+        return float(self.parts.sum().item())
+
+    def train_step(self, lr, stream=None):
+        self.forward(stream)
+        self.backward(stream)
+        self.net.adam(lr, stream=stream)

@@ -1,0 +1,146 @@
+"""Surrogate network kernels (sw_cnn_fwd / sw_cnn_bwd / sw_adam) against the
+CPU oracle (oracle/cnn.py, pinned by finite differences), with the oracle
+rounding to bf16 wherever the device stores bf16.
+
+Bars: forward outputs within 1e-2 (max) and 1e-3 (mean) absolute; every
+layer's weight and bias gradient within 3 % relative L2 of the oracle's;
+the loss within 1e-3 relative. Plus: Adam against its formula, 60 steps of
+training on one batch cut the loss by a fifth or more, and the input/label/core copies
+against numpy indexing.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def S():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    from solweig_project_amd import surrogate
+    return surrogate
+
+
+def _small(S, seed=1):
+    net = S.SurrogateNet(channels=(16, 32, 32, 48), seed=seed)
+    ps = S.Passes(net, batch=2, hw=64, core=32, c0=16, nchunk=4)
+    rng = np.random.default_rng(seed)
+    x = np.zeros((2, 64, 64, 16), np.float32)
+    x[..., :3] = rng.uniform(0, 1, (2, 64, 64, 3))
+    ps.x.copy_(torch.from_numpy(x).to(torch.bfloat16))
+    lab = rng.uniform(0, 1, (2, 32, 32, 10)).astype(np.float32)
+    ps.labels.copy_(torch.from_numpy(lab))
+    return net, ps, lab
+
+
+def test_forward_and_gradients_match_oracle(S):
+    from oracle import cnn as C
+    _say("surrogate fwd/bwd vs oracle")
+    net, ps, lab = _small(S)
+    ps.forward()
+    ps.backward()
+    torch.cuda.synchronize()
+    params = net.numpy_params()
+    x = ps.x.float().cpu().numpy()
+    y_ref, cache = C.forward(params, x, store=C.bf16_round)
+    got = ps.out.cpu().numpy()
+    err = np.abs(got - y_ref)
+    assert err.max() <= 1e-2 and err.mean() <= 1e-3, (err.max(), err.mean())
+    ref_loss = C.loss(y_ref, lab, 16, 32, 10)
+    assert abs(ps.loss() - ref_loss) <= 1e-3 * ref_loss
+    g_ref = C.backward(params, cache, lab, 16, 32, 10, store=C.bf16_round)
+    for name in S.LAYERS:
+        for got_g, ref_g in ((net.gw(name), g_ref[name][0]), (net.gb(name), g_ref[name][1])):
+            gg = got_g.cpu().numpy().ravel()
+            rr = ref_g.astype(np.float32).ravel()
+            if name == "head":          # padded head channels carry no gradient
+                rr = rr.reshape(16, -1)[:10].ravel() if rr.size > 16 else rr[:10]
+                gg = gg.reshape(16, -1)[:10].ravel() if gg.size > 16 else gg[:10]
+            rel = np.linalg.norm(gg - rr) / max(np.linalg.norm(rr), 1e-30)
+            assert rel <= 3e-2, (name, rel)
+
+
+def test_adam_step_matches_formula(S):
+    net, ps, _ = _small(S, seed=2)
+    ps.forward()
+    ps.backward()
+    torch.cuda.synchronize()
+    p0 = net.p.cpu().numpy().copy()
+    g = net.g.cpu().numpy().copy()
+    net.adam(lr=1e-3)
+    torch.cuda.synchronize()
+    m = 0.1 * g
+    v = 0.001 * g * g
+    exp = p0 - 1e-3 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    got = net.p.cpu().numpy()
+    assert np.allclose(got, exp, rtol=1e-5, atol=1e-7)
+    assert torch.equal(net.pb, net.p.to(torch.bfloat16))
+
+
+def test_training_steps_reduce_the_loss(S):
+    _say("surrogate: 60 Adam steps on one batch")
+    net, ps, _ = _small(S, seed=3)
+    # a learnable target: the input heights of the core, repeated over the 10 channels
+    x = ps.x.float().cpu().numpy()
+    lab = np.concatenate([x[:, 16:48, 16:48, :3]] * 4, axis=3)[..., :10]
+    ps.labels.copy_(torch.from_numpy(np.ascontiguousarray(lab)))
+    ps.forward()
+    ps.backward()
+    torch.cuda.synchronize()
+    l0 = ps.loss()
+    for _ in range(60):
+        ps.train_step(3e-3)
+    ps.forward()
+    ps.backward()
+    torch.cuda.synchronize()
+    assert ps.loss() < 0.8 * l0, (l0, ps.loss())
+
+
+def test_window_copies(S):
+    from solweig_project_amd import _lib as L
+    rows, cols = 50, 70
+    rng = np.random.default_rng(4)
+    hp = rng.uniform(0, 40, (rows * cols, 4)).astype(np.float32)
+    fields = rng.uniform(0, 1, (16, rows, cols)).astype(np.float32)
+    org = np.array([[-8, -8], [10, 30], [40, 60]], np.int32)
+    d_hp = torch.from_numpy(hp).cuda()
+    d_org = torch.from_numpy(org).cuda()
+    x = torch.zeros((3, 24, 24, 16), dtype=torch.bfloat16, device="cuda")
+    L.call("sw_cnn_pack_input", L.ptr(d_hp), L.ptr(d_org), L.ptr(x), 3, 24, rows, cols, None)
+    fm = torch.arange(10, dtype=torch.int32, device="cuda")
+    lab = torch.zeros((3, 8, 8, 10), dtype=torch.float32, device="cuda")
+    d_f = torch.from_numpy(fields).cuda()
+    L.call("sw_cnn_gather_labels", L.ptr(d_f), L.ptr(d_org), L.ptr(fm), L.ptr(lab), 3, 8, 8, 10, rows, cols, None)
+    out = torch.from_numpy(rng.uniform(0, 1, (3, 24, 24, 16)).astype(np.float32)).cuda()
+    dst = torch.full((16, rows, cols), -1.0, device="cuda")
+    L.call("sw_cnn_scatter_cores", L.ptr(out), L.ptr(d_org), L.ptr(fm), L.ptr(dst), 3, 24, 16, 8, 8, 10, rows, cols,
+           None)
+    torch.cuda.synchronize()
+    H = hp.reshape(rows, cols, 4)
+    xe = np.zeros((3, 24, 24, 16), np.float32)
+    le = np.zeros((3, 8, 8, 10), np.float32)
+    de = np.full((16, rows, cols), -1.0, np.float32)
+    o = out.cpu().numpy()
+    for k, (r0, c0) in enumerate(org):
+        for i in range(24):
+            for j in range(24):
+                r, c = r0 + i, c0 + j
+                if 0 <= r < rows and 0 <= c < cols:
+                    xe[k, i, j, :3] = H[r, c, :3] * np.float32(0.02)
+        for i in range(8):
+            for j in range(8):
+                r, c = r0 + 8 + i, c0 + 8 + j
+                if 0 <= r < rows and 0 <= c < cols:
+                    le[k, i, j] = fields[:10, r, c]
+                    de[:10, r, c] = o[k, 8 + i, 8 + j, :10]
+    assert np.array_equal(x.float().cpu().numpy(), xe.astype(np.float32).astype(np.float32)) or \
+        np.allclose(x.float().cpu().numpy(), xe, rtol=8e-3, atol=0)
+    assert np.array_equal(lab.cpu().numpy(), le)
+    assert np.array_equal(dst.cpu().numpy(), de)
