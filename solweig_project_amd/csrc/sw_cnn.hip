@@ -3,7 +3,7 @@
 // and the optimiser. Activations are NHWC bf16 with channel counts that are
 // multiples of 16 (16 channels = one K chunk of a 32x32x16 MFMA).
 //
-//   k_conv          implicit-GEMM 3x3 / 1x1 convolution, M = pixels, N = output
+//   k_conv_t        implicit-GEMM 3x3 / 1x1 convolution, M = pixels, N = output
 //                   channels, K = taps x input channels. Inputs: an optional
 //                   half-resolution tensor read through nearest upsampling and
 //                   an optional full-resolution skip tensor (channel concat);
@@ -47,68 +47,229 @@ struct ConvArgs {
     int c1, c2, up1, o1, cout, n, h, w, act;   // act: 0 none, 1 ReLU, 2 sigmoid
 };
 
-template <int KS, int NT>
-__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
-    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Implicit-GEMM convolution, LDS tiled: a block computes BM = 32 * WAVES_M * WM pixels x
+// BN = 32 * WAVES_N * WN output channels. K runs over (tap, 16-channel chunk) pairs; a stage of KST
+// chunks is gathered from global memory into registers while the previous stage's MFMAs run, then
+// written to the other half of a double-buffered LDS tile (A: [chunk][pixel][k half], B:
+// [chunk][co][k half], 16 bytes per entry). Each wave issues KST x WM x WN MFMAs per barrier.
+template <int KS, int WAVES_M, int WM, int WN, int KST>
+__global__ void __launch_bounds__(256) k_conv_t(ConvArgs a) {
+    constexpr int WAVES_N = 4 / WAVES_M;
+    constexpr int BM = 32 * WAVES_M * WM, BN = 32 * WAVES_N * WN;
+    constexpr int AV = (2 * BM + 255) / 256;     // A entries per thread per chunk
+    static_assert(2 * BM % 256 == 0 && 2 * BN <= 256, "tile shape");
+    __shared__ bf16x8 sA[2][KST][2 * BM];
+    __shared__ bf16x8 sB[2][KST][2 * BN];
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
     const long M = (long)a.n * a.h * a.w;
-    const long m0 = ((long)blockIdx.x * 4 + wave) * 32;
-    if (m0 >= M) return;                                    // whole wave idle
-    const int n0 = blockIdx.y * 32 * NT;
+    const long m0 = (long)blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
     const int cin = a.c1 + a.c2;
-    const int half = l >> 5;
-    // this lane's A row (pixel)
-    const long m = m0 + (l & 31);
-    const bool mv = m < M;
-    const long mm = mv ? m : 0;
-    const int x = (int)(mm % a.w);
-    const int y = (int)((mm / a.w) % a.h);
-    const int img = (int)(mm / ((long)a.w * a.h));
+    const int cpt = cin / 16;                    // chunks per tap
+    const int nchunk = KS * KS * cpt;
     const int hh = a.h >> 1, wh = a.w >> 1;
-    f32x16 acc[NT];
+    // A loader: entry e = tid + 256 v -> pixel e / 2, k half e % 2
+    int ax[AV], ay[AV], aimg[AV];
+    bool amv[AV];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (f32x16){};
-    for (int tap = 0; tap < KS * KS; ++tap) {
-        const int yy = y + tap / KS - KS / 2, xx = x + tap % KS - KS / 2;
-        const bool inb = mv && (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
-        for (int cb = 0; cb < cin; cb += 16) {
-            const int ch = cb + 8 * half;
-            bf16x8 av = (bf16x8){};
-            if (inb) {
-                if (ch < a.c1) {
-                    const long p = a.up1 ? (((long)img * hh + (yy >> 1)) * wh + (xx >> 1))
-                                         : (((long)img * a.h + yy) * a.w + xx);
-                    av = *(const bf16x8 *)(a.in1 + p * a.c1 + ch);
-                } else {
-                    const long p = ((long)img * a.h + yy) * a.w + xx;
-                    av = *(const bf16x8 *)(a.in2 + p * a.c2 + (ch - a.c1));
+    for (int v = 0; v < AV; ++v) {
+        const int e = tid + 256 * v;
+        const long am = m0 + (e >> 1);
+        amv[v] = am < M;
+        const long amm = amv[v] ? am : 0;
+        ax[v] = (int)(amm % a.w);
+        ay[v] = (int)((amm / a.w) % a.h);
+        aimg[v] = (int)(amm / ((long)a.w * a.h));
+    }
+    const int ah = tid & 1;
+    const bool ldB = tid < 2 * BN;
+    const int bco = n0 + (tid >> 1);
+    const bool bcv = ldB && bco < a.cout;
+    bf16x8 ga[KST][AV], gb[KST];
+    auto gload = [&](int st) {
+#pragma unroll
+        for (int c = 0; c < KST; ++c) {
+            const int kc = st * KST + c;
+            const bool kv = kc < nchunk;
+            const int tap = kv ? kc / cpt : 0;
+            const int cb = kv ? (kc % cpt) * 16 : 0;
+            const int ch = cb + 8 * ah;
+#pragma unroll
+            for (int v = 0; v < AV; ++v) {
+                bf16x8 av = (bf16x8){};
+                const int yy = ay[v] + tap / KS - KS / 2, xx = ax[v] + tap % KS - KS / 2;
+                if (kv && amv[v] && (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w) {
+                    if (ch < a.c1) {
+                        const long p = a.up1 ? (((long)aimg[v] * hh + (yy >> 1)) * wh + (xx >> 1))
+                                             : (((long)aimg[v] * a.h + yy) * a.w + xx);
+                        av = *(const bf16x8 *)(a.in1 + p * a.c1 + ch);
+                    } else {
+                        const long p = ((long)aimg[v] * a.h + yy) * a.w + xx;
+                        av = *(const bf16x8 *)(a.in2 + p * a.c2 + (ch - a.c1));
+                    }
                 }
+                ga[c][v] = av;
             }
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int co = n0 + 32 * t + (l & 31);
-                bf16x8 bv = (bf16x8){};
-                if (co < a.cout) bv = *(const bf16x8 *)(a.wt + ((long)co * KS * KS + tap) * cin + ch);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
-            }
+            bf16x8 bv = (bf16x8){};
+            if (kv && bcv) bv = *(const bf16x8 *)(a.wt + ((long)bco * KS * KS + tap) * cin + cb + 8 * (tid & 1));
+            gb[c] = bv;
         }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < KST; ++c) {
+#pragma unroll
+            for (int v = 0; v < AV; ++v) sA[buf][c][tid + 256 * v] = ga[c][v];
+            if (ldB) sB[buf][c][tid] = gb[c];
+        }
+    };
+    f32x16 acc[WM][WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = (f32x16){};
+    const int nst = (nchunk + KST - 1) / KST;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) gload(st + 1);             // in flight during this stage's MFMAs
+#pragma unroll
+        for (int c = 0; c < KST; ++c) {
+            if (st * KST + c >= nchunk) break;        // uniform
+            bf16x8 av[WM], bv[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i) av[i] = sA[buf][c][((wm * WM + i) * 32 + (l & 31)) * 2 + (l >> 5)];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) bv[j] = sB[buf][c][((wn * WN + j) * 32 + (l & 31)) * 2 + (l >> 5)];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (st + 1 < nst) lstore(buf ^ 1);
+        __syncthreads();
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = n0 + 32 * t + (l & 31);
+    for (int j = 0; j < WN; ++j) {
+        const int co = n0 + (wn * WN + j) * 32 + (l & 31);
         if (co >= a.cout) continue;
         const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const long p = m0 + 8 * (i >> 2) + 4 * half + (i & 3);
-            if (p >= M) continue;
-            float v = acc[t][i] + b;
-            if (a.act == 1) v = v > 0.f ? v : 0.f;
-            else if (a.act == 2) v = 1.f / (1.f + __expf(-v));
-            if (a.outf) a.outf[p * a.cout + co] = v;
-            if (co < a.o1) {
-                if (a.out1) a.out1[p * a.o1 + co] = f2bf(v);
-            } else if (a.out2) {
-                a.out2[p * (a.cout - a.o1) + (co - a.o1)] = f2bf(v);
+        for (int i = 0; i < WM; ++i) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const long p = m0 + (wm * WM + i) * 32 + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
+                if (p >= M) continue;
+                float v = acc[i][j][q] + b;
+                if (a.act == 1) v = v > 0.f ? v : 0.f;
+                else if (a.act == 2) v = 1.f / (1.f + __expf(-v));
+                if (a.outf) a.outf[p * a.cout + co] = v;
+                if (co < a.o1) {
+                    if (a.out1) a.out1[p * a.o1 + co] = f2bf(v);
+                } else if (a.out2) {
+                    a.out2[p * (a.cout - a.o1) + (co - a.o1)] = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
+// Direct 3x3 convolution, input stationary: a block owns an 8 x 32 pixel tile of one image and
+// BN = 32 * WN output channels. Per 16-channel input chunk it stages the 10 x 34 input halo tile
+// and the chunk's weights of all 9 taps in LDS (each input cell is fetched once instead of once
+// per tap), then every wave (two image rows = 64 pixels) issues 9 x 2 x WN MFMAs from LDS.
+template <int WN>
+__global__ void __launch_bounds__(256) k_conv_d(ConvArgs a) {
+    constexpr int TH = 8, TW = 32, HH = TH + 2, HW2 = TW + 2, BN = 32 * WN;
+    __shared__ bf16x8 sX[HH * HW2 * 2];         // [halo cell][k half]
+    __shared__ bf16x8 sW[9 * BN * 2];           // [tap][co][k half]
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+    const int t = blockIdx.x;
+    const int img = t / (tiles_x * tiles_y);
+    const int y0 = ((t / tiles_x) % tiles_y) * TH, x0 = (t % tiles_x) * TW;
+    const int n0 = blockIdx.y * BN;
+    const int cin = a.c1 + a.c2;
+    const int hh = a.h >> 1, wh = a.w >> 1;
+    f32x16 acc[2][WN];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = (f32x16){};
+    for (int cb = 0; cb < cin; cb += 16) {
+        // halo tile: HH * HW2 cells x 2 halves = 680 16-byte entries
+        for (int e = tid; e < HH * HW2 * 2; e += 256) {
+            const int cell = e >> 1, half = e & 1;
+            const int yy = y0 - 1 + cell / HW2, xx = x0 - 1 + cell % HW2;
+            bf16x8 v = (bf16x8){};
+            if ((unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w) {
+                const int ch = cb + 8 * half;
+                if (ch < a.c1) {
+                    const long p = a.up1 ? (((long)img * hh + (yy >> 1)) * wh + (xx >> 1))
+                                         : (((long)img * a.h + yy) * a.w + xx);
+                    v = *(const bf16x8 *)(a.in1 + p * a.c1 + ch);
+                } else {
+                    const long p = ((long)img * a.h + yy) * a.w + xx;
+                    v = *(const bf16x8 *)(a.in2 + p * a.c2 + (ch - a.c1));
+                }
+            }
+            sX[e] = v;
+        }
+        // weights: 9 taps x BN co x 2 halves
+        for (int e = tid; e < 9 * BN * 2; e += 256) {
+            const int half = e & 1, co_l = (e >> 1) % BN, tap = (e >> 1) / BN;
+            const int co = n0 + co_l;
+            bf16x8 v = (bf16x8){};
+            if (co < a.cout) v = *(const bf16x8 *)(a.wt + ((long)co * 9 + tap) * cin + cb + 8 * half);
+            sW[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            bf16x8 av[2], bv[WN];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int ry = wave * 2 + i;          // tile row of MFMA row tile i
+                av[i] = sX[((ry + dy) * HW2 + (l & 31) + dx) * 2 + (l >> 5)];
+            }
+#pragma unroll
+            for (int j = 0; j < WN; ++j) bv[j] = sW[(tap * BN + j * 32 + (l & 31)) * 2 + (l >> 5)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+        const int co = n0 + j * 32 + (l & 31);
+        if (co >= a.cout) continue;
+        const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int yy = y0 + wave * 2 + i;
+            if (yy >= a.h) continue;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int xx = x0 + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
+                if (xx >= a.w) continue;
+                const long p = ((long)img * a.h + yy) * a.w + xx;
+                float v = acc[i][j][q] + b;
+                if (a.act == 1) v = v > 0.f ? v : 0.f;
+                else if (a.act == 2) v = 1.f / (1.f + __expf(-v));
+                if (a.outf) a.outf[p * a.cout + co] = v;
+                if (co < a.o1) {
+                    if (a.out1) a.out1[p * a.o1 + co] = f2bf(v);
+                } else if (a.out2) {
+                    a.out2[p * (a.cout - a.o1) + (co - a.o1)] = f2bf(v);
+                }
             }
         }
     }
@@ -144,17 +305,18 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WgradArgs a) {
     const int dy = tap / KS - KS / 2, dx = tap % KS - KS / 2;
     const int hh = a.h >> 1, wh = a.w >> 1;
     f32x16 acc = (f32x16){};
+    // pixel coordinates of this lane's first pixel, advanced by 16 per step without divisions
+    long pq = p0 + 8 * half;
+    int px = (int)(pq % a.w), py = (int)((pq / a.w) % a.h), pimg = (int)(pq / ((long)a.w * a.h));
     for (long pb = p0; pb < p1; pb += 16) {
         bf16x8 av, bv;
+        int x = px, y = py, img = pimg;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const long p = pb + 8 * half + j;
             float dzv = 0.f, xv = 0.f;
             if (p < p1) {
                 if (co < a.cout) dzv = bf2f(a.dz[p * a.cout + co]);
-                const int x = (int)(p % a.w);
-                const int y = (int)((p / a.w) % a.h);
-                const int img = (int)(p / ((long)a.w * a.h));
                 const int yy = y + dy, xx = x + dx;
                 if (ci < cin && (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w) {
                     if (ci < a.c1) {
@@ -169,8 +331,12 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WgradArgs a) {
             }
             av[j] = f2bf(dzv);
             bv[j] = f2bf(xv);
+            if (++x == a.w) { x = 0; if (++y == a.h) { y = 0; ++img; } }
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        // advance the lane's first pixel by 16
+        for (int k = 0; k < 16; ++k)
+            if (++px == a.w) { px = 0; if (++py == a.h) { py = 0; ++pimg; } }
     }
     if (ci >= cin) return;
     float *o = a.ws + (long)chunk * a.cout * KS * KS * cin;
@@ -181,16 +347,25 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WgradArgs a) {
     }
 }
 
-// bias gradient partials: ws[chunk][co] = sum over the chunk's pixels of dz[p][co]
+// bias gradient partials: ws[chunk][co] = sum over the chunk's pixels of dz[p][co]; the block's
+// threads split the pixels (lane group g sums pixels p0 + g, p0 + g + groups, ...) and combine in LDS
 __global__ void __launch_bounds__(256) k_bias_partial(const __bf16 *__restrict__ dz, float *__restrict__ ws, long M,
                                                        int cout, int chunk) {
-    const int co = blockIdx.y * 256 + threadIdx.x;
-    if (co >= cout) return;
+    __shared__ float red[256];
+    const int co = blockIdx.y * 32 + (threadIdx.x & 31);
+    const int g = threadIdx.x >> 5;              // 8 pixel groups
     const long p0 = (long)blockIdx.x * chunk;
     const long p1 = p0 + chunk < M ? p0 + chunk : M;
     float s = 0.f;
-    for (long p = p0; p < p1; ++p) s += bf2f(dz[p * cout + co]);
-    ws[(long)blockIdx.x * cout + co] = s;
+    if (co < cout)
+        for (long p = p0 + g; p < p1; p += 8) s += bf2f(dz[p * cout + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (g == 0 && co < cout) {
+        float t = 0.f;
+        for (int k = 0; k < 8; ++k) t += red[k * 32 + (threadIdx.x & 31)];
+        ws[(long)blockIdx.x * cout + co] = t;
+    }
 }
 
 // out[i] = sum over chunks of ws[chunk][i], in chunk order
@@ -398,18 +573,29 @@ static int launch_conv(const sw_cnn_op_t &o, const __bf16 *w, void *stream) {
     a.c1 = o.c1; a.c2 = o.c2; a.up1 = o.up; a.o1 = o.o1; a.cout = o.cout;
     a.n = o.n; a.h = o.h; a.w = o.w; a.act = o.act;
     const long M = (long)o.n * o.h * o.w;
-    const int nt = o.cout >= 96 ? 4 : (o.cout >= 64 ? 2 : 1);
-    const dim3 grid((unsigned)blocks(M, 128), (unsigned)blocks(o.cout, 32 * nt));
     if (o.ks == 3) {
-        if (nt == 4) hipLaunchKernelGGL((k_conv<3, 4>), grid, dim3(256), 0, S(stream), a);
-        else if (nt == 2) hipLaunchKernelGGL((k_conv<3, 2>), grid, dim3(256), 0, S(stream), a);
-        else hipLaunchKernelGGL((k_conv<3, 1>), grid, dim3(256), 0, S(stream), a);
-    } else {
-        if (nt == 4) hipLaunchKernelGGL((k_conv<1, 4>), grid, dim3(256), 0, S(stream), a);
-        else if (nt == 2) hipLaunchKernelGGL((k_conv<1, 2>), grid, dim3(256), 0, S(stream), a);
-        else hipLaunchKernelGGL((k_conv<1, 1>), grid, dim3(256), 0, S(stream), a);
+        // direct convolution: 8 x 32 pixel tiles x 32 * WN channels
+        const int wn = o.cout > 32 ? 2 : 1;
+        const long tiles = (long)o.n * ((o.h + 7) / 8) * ((o.w + 31) / 32);
+        const dim3 g((unsigned)tiles, (unsigned)blocks(o.cout, 32 * wn));
+        if (wn == 2) hipLaunchKernelGGL((k_conv_d<2>), g, dim3(256), 0, S(stream), a);
+        else hipLaunchKernelGGL((k_conv_d<1>), g, dim3(256), 0, S(stream), a);
+        return hip_check(hipGetLastError(), "k_conv_d");
     }
-    return hip_check(hipGetLastError(), "k_conv");
+    // block tiles: 128 pixels x 128 channels, 256 x 64 or 256 x 32 (narrow layers), 4 K chunks per stage
+    const int bn = o.cout > 64 ? 128 : (o.cout > 32 ? 64 : 32);
+    const int bm = bn == 128 ? 128 : 256;
+    const dim3 grid((unsigned)blocks(M, bm), (unsigned)blocks(o.cout, bn));
+    if (o.ks == 3) {
+        if (bn == 128) hipLaunchKernelGGL((k_conv_t<3, 2, 2, 2, 4>), grid, dim3(256), 0, S(stream), a);
+        else if (bn == 64) hipLaunchKernelGGL((k_conv_t<3, 4, 2, 2, 4>), grid, dim3(256), 0, S(stream), a);
+        else hipLaunchKernelGGL((k_conv_t<3, 4, 2, 1, 4>), grid, dim3(256), 0, S(stream), a);
+    } else {
+        if (bn == 128) hipLaunchKernelGGL((k_conv_t<1, 2, 2, 2, 4>), grid, dim3(256), 0, S(stream), a);
+        else if (bn == 64) hipLaunchKernelGGL((k_conv_t<1, 4, 2, 2, 4>), grid, dim3(256), 0, S(stream), a);
+        else hipLaunchKernelGGL((k_conv_t<1, 4, 2, 1, 4>), grid, dim3(256), 0, S(stream), a);
+    }
+    return hip_check(hipGetLastError(), "k_conv_t");
 }
 
 static int check_conv(const sw_cnn_op_t &o, const char *who) {
@@ -466,7 +652,7 @@ static int cnn_run(const sw_cnn_op_t *ops, int nops, void *stream) {
             if ((rc = hip_check(hipGetLastError(), "k_reduce_chunks"))) return rc;
             if (o.gb) {
                 float *wsb = (float *)o.ws + (long)nchunk * per;
-                hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)nchunk, (unsigned)blocks(o.cout, 256)), dim3(256),
+                hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)nchunk, (unsigned)blocks(o.cout, 32)), dim3(256),
                                    0, S(stream), (const __bf16 *)o.dz, wsb, pix, o.cout, chunk);
                 if ((rc = hip_check(hipGetLastError(), "k_bias_partial"))) return rc;
                 hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(o.cout, 256)), dim3(256), 0, S(stream), wsb,

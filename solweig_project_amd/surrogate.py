@@ -311,3 +311,208 @@ class Passes:
         self.forward(stream)
         self.backward(stream)
         self.net.adam(lr, stream=stream)
+
+
+# ---------------------------------------------------------------------------
+# training data, training loop, gated inference
+# ---------------------------------------------------------------------------
+class SiteField:
+    """A device-resident city: composed heights (hpack) and exact SVF field planes."""
+
+    def __init__(self, hpack, fields, rows, cols):
+        self.hpack, self.fields, self.rows, self.cols = hpack, fields, rows, cols
+
+
+def exact_sites(solver, n_sites: int, size: int, seed: int):
+    """``n_sites`` seeded synthetic cities of size x size, labelled by the exact SVF sweep."""
+    from . import synth
+    from .solver import TileInputs
+    out = []
+    for k in range(n_sites):
+        s = synth.make_site(size, size, seed=seed + k, with_aspect=False)
+        ds = solver.upload(TileInputs.from_site(s))
+        solver.svf(ds)
+        out.append(SiteField(ds.hpack, ds.fields, size, size))
+        del ds
+    torch.cuda.synchronize()
+    return out
+
+
+def window_origins(n: int, rows: int, cols: int, seed: int, hw: int = HW, halo: int = HALO) -> np.ndarray:
+    """``n`` window origins whose cores lie inside a rows x cols site (the halo may leave it)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    core = hw - 2 * halo
+    r = rng.integers(-halo, rows - core - halo + 1, n)
+    c = rng.integers(-halo, cols - core - halo + 1, n)
+    return np.stack([r, c], axis=1).astype(np.int32)
+
+
+def load_batch(ps: Passes, site: SiteField, origins: np.ndarray, with_labels: bool = True, stream=None):
+    """Input windows (and labels) of ``origins`` into the pass buffers."""
+    dev = ps.net.device
+    org = torch.from_numpy(np.ascontiguousarray(origins, np.int32)).to(dev)
+    n = len(origins)
+    sp = _sp(stream)
+    L.call("sw_cnn_pack_input", L.ptr(site.hpack), L.ptr(org), L.ptr(ps.x), n, ps.hw, site.rows, site.cols, sp)
+    if with_labels:
+        fm = _field_map(dev)
+        L.call("sw_cnn_gather_labels", L.ptr(site.fields), L.ptr(org), L.ptr(fm), L.ptr(ps.labels), n, ps.core,
+               ps.c0, N_FIELDS, site.rows, site.cols, sp)
+    return org
+
+
+_FM = {}
+
+
+def _field_map(dev):
+    t = _FM.get(str(dev))
+    if t is None:
+        t = _FM[str(dev)] = torch.tensor(FIELD_MAP, dtype=torch.int32, device=dev)
+    return t
+
+
+def train(net: SurrogateNet, sites, *, steps: int, batch: int = 8, lr: float = 2e-3, windows: int = 20000,
+          seed: int = 0, log=None):
+    """Adam with cosine decay over ``steps``; every batch is drawn from one site's fixed window list."""
+    ps = Passes(net, batch)
+    per = max(1, windows // len(sites))
+    lists = [window_origins(per, s.rows, s.cols, seed + 101 * k) for k, s in enumerate(sites)]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    losses = []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        k = int(rng.integers(0, len(sites)))
+        pick = rng.choice(len(lists[k]), batch, replace=False)
+        load_batch(ps, sites[k], lists[k][pick])
+        cur = lr * 0.5 * (1.0 + np.cos(np.pi * it / max(steps, 1)))
+        ps.train_step(float(cur))
+        if it % 50 == 0 or it == steps - 1:
+            losses.append((it, ps.loss()))
+            if log:
+                log(f"step {it}: loss {losses[-1][1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    torch.cuda.synchronize()
+    return {"losses": losses, "seconds": time.perf_counter() - t0, "steps": steps, "batch": batch,
+            "windows": per * len(sites)}
+
+
+def core_origins(rows: int, cols: int, core: int = CORE, halo: int = HALO) -> np.ndarray:
+    """Window origins whose cores tile a rows x cols site."""
+    rr = np.arange(0, rows, core)
+    cc = np.arange(0, cols, core)
+    g = np.stack(np.meshgrid(rr, cc, indexing="ij"), axis=-1).reshape(-1, 2) - halo
+    return g.astype(np.int32)
+
+
+@dataclass
+class GateReport:
+    windows: int
+    accepted: int
+    sampled_tiles: int
+    exact_tiles: int
+    ms_total: float
+    ms_network: float
+
+
+class SvfSurrogate:
+    """Gated surrogate SVF of a device-resident scene (fields 0..9; the other six field planes keep
+    the exact sweep's values only where a window falls back)."""
+
+    def __init__(self, net: SurrogateNet, batch: int = 16, tol: float = 0.02, samples: int = 4, seed: int = 0):
+        self.net, self.batch, self.tol, self.samples, self.seed = net, batch, tol, samples, seed
+        self.ps = Passes(net, batch)
+
+    def predict(self, ds, fields, stream=None):
+        """Network outputs of every core of scene ``ds`` into ``fields`` (16, rows * cols)."""
+        org = core_origins(ds.rows, ds.cols)
+        site = SiteField(ds.hpack, fields, ds.rows, ds.cols)
+        sp = _sp(stream)
+        fm = _field_map(self.net.device)
+        for k in range(0, len(org), self.batch):
+            part = org[k:k + self.batch]
+            n = len(part)
+            if n < self.batch:
+                part = np.concatenate([part, np.repeat(part[-1:], self.batch - n, axis=0)])
+            d_org = load_batch(self.ps, site, part, with_labels=False, stream=stream)
+            self.ps.forward(stream)
+            L.call("sw_cnn_scatter_cores", L.ptr(self.ps.out), L.ptr(d_org), L.ptr(fm), L.ptr(fields), n, self.ps.hw,
+                   COUT_HEAD, self.ps.core, self.ps.c0, N_FIELDS, ds.rows, ds.cols, sp)
+        return org
+
+    def svf(self, solver, ds, stream=None) -> GateReport:
+        """Gated surrogate SVF into ``ds.fields``: network on every core, the exact kernel on
+        ``samples`` sampled 16 x 16 tiles per core, and an exact re-solve of every core whose sampled
+        error exceeds ``tol``. ``ds`` must be composed (sw_compose) beforehand."""
+        from .incremental import TileGrid
+        t0 = time.perf_counter()
+        rows, cols = ds.rows, ds.cols
+        surr = torch.empty_like(ds.fields)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        org = self.predict(ds, surr, stream)
+        ev[1].record()
+        g = TileGrid(rows, cols)
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        per_core = CORE // 16
+        samp = []
+        owner = []
+        for w, (r0, c0) in enumerate(org + HALO):
+            tr0, tc0 = r0 // 16, c0 // 16
+            ty = min(per_core, g.ty - tr0)
+            tx = min(per_core, g.tx - tc0)
+            for _ in range(self.samples):
+                samp.append((tr0 + int(rng.integers(0, ty))) * g.tx + tc0 + int(rng.integers(0, tx)))
+                owner.append(w)
+        samp = np.array(samp, np.int32)
+        uniq, inv = np.unique(samp, return_inverse=True)
+        self._exact_tiles(solver, ds, uniq, stream)
+        # compare on the sampled tiles (fields 0..9): per-tile max abs difference, per window max
+        diff = (surr[:N_FIELDS].view(N_FIELDS, rows, cols) - ds.fields[:N_FIELDS].view(N_FIELDS, rows, cols)).abs()
+        dmax = diff.amax(dim=0)
+        tiles_err = _tile_max(dmax, uniq, g)
+        werr = np.zeros(len(org), np.float32)
+        np.maximum.at(werr, np.array(owner), tiles_err[inv])
+        bad = np.nonzero(werr > self.tol)[0]
+        # accepted windows take the surrogate outside the sampled (exact) tiles
+        keep = torch.ones((rows, cols), dtype=torch.bool, device=ds.fields.device)
+        for w in bad:
+            r0, c0 = (org[w] + HALO).tolist()
+            keep[r0:r0 + CORE, c0:c0 + CORE] = False
+        kt = torch.zeros(g.ty * g.tx, dtype=torch.bool)
+        kt[torch.from_numpy(uniq.astype(np.int64))] = True
+        kt = kt.view(g.ty, g.tx).repeat_interleave(16, 0).repeat_interleave(16, 1)[:rows, :cols].to(keep.device)
+        keep &= ~kt
+        f = ds.fields[:N_FIELDS].view(N_FIELDS, rows, cols)
+        f.copy_(torch.where(keep, surr[:N_FIELDS].view(N_FIELDS, rows, cols), f))
+        ex = 0
+        if len(bad):
+            m = g.empty()
+            for w in bad:
+                r0, c0 = (org[w] + HALO).tolist()
+                m[r0 // 16:min(g.ty, (r0 + CORE) // 16), c0 // 16:min(g.tx, (c0 + CORE) // 16)] = True
+            ids = TileGrid.ids(m)
+            ex = len(ids)
+            self._exact_tiles(solver, ds, ids, stream)
+        torch.cuda.synchronize()
+        return GateReport(windows=len(org), accepted=len(org) - len(bad), sampled_tiles=len(uniq), exact_tiles=ex,
+                          ms_total=(time.perf_counter() - t0) * 1e3, ms_network=ev[0].elapsed_time(ev[1]))
+
+    @staticmethod
+    def _exact_tiles(solver, ds, ids, stream=None):
+        if len(ids) == 0:
+            return
+        P = L.ptr
+        d = torch.from_numpy(np.ascontiguousarray(ids, np.int32)).to(ds.fields.device)
+        offs, tab = solver.svf_tables(ds.rows, ds.cols, ds.tile.scale)
+        c = solver._fold_consts
+        L.call("sw_svf_sweep_tiles", P(ds.hpack), P(ds.stats), P(offs), P(tab), P(ds.vegdsm2), P(c["w_iso"]),
+               P(c["w_aniso"]), P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(ds.counts), P(ds.hpad), P(ds.masks),
+               P(ds.fields), P(d), int(len(ids)), ds.rows, ds.cols, _sp(stream))
+
+
+def _tile_max(dmax: torch.Tensor, ids: np.ndarray, g) -> np.ndarray:
+    """Max of a (rows, cols) plane over each listed 16 x 16 tile."""
+    rows, cols = dmax.shape
+    pad = torch.zeros((g.ty * 16, g.tx * 16), dtype=dmax.dtype, device=dmax.device)
+    pad[:rows, :cols] = dmax
+    t = pad.view(g.ty, 16, g.tx, 16).amax(dim=(1, 3)).reshape(-1)
+    return t[torch.from_numpy(ids.astype(np.int64)).to(t.device)].cpu().numpy()
