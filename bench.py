@@ -354,7 +354,7 @@ def run_surrogate(args):
     exact = ds.fields[:SG.N_FIELDS].clone()
 
     def gated(model):
-        gate = SG.SvfSurrogate(model, batch=16, tol=0.02, samples=4, seed=args.seed)
+        gate = SG.SvfSurrogate(model, tol=0.02, samples=4, seed=args.seed)
         gate.svf(solver, ds)                     # warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -366,7 +366,7 @@ def run_surrogate(args):
     rep, wall, err = gated(net)
     e = err.flatten()
     q = torch.quantile(e[torch.randperm(e.numel(), device=e.device)[:4_000_000]], 0.99).item()
-    raw = SG.SvfSurrogate(net, batch=16)
+    raw = SG.SvfSurrogate(net)
     surr = torch.empty_like(ds.fields)
     raw.predict(ds, surr)
     raw_err = (surr[:SG.N_FIELDS] - exact).abs()
@@ -381,8 +381,8 @@ def run_surrogate(args):
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "bf16 (MFMA, f32 accumulate)",
         "data": "synthetic (seeded cities; labels from the exact SVF sweep of whole 1024² cities)",
         "config": {"workload": "4-level conv encoder-decoder on 384² windows (256² core + 64 px halo), gate: "
-                               "exact kernel on 4 sampled 16x16 tiles per core, tolerance 0.02, exact re-solve of "
-                               "rejected cores", "channels": list(ch), "params": net.n_params, "rows": n,
+                               "exact kernel on 4 sampled 16x16 tiles per core, accepted when their mean absolute "
+                               "error over the 10 fields is <= 0.02, exact re-solve of rejected cores", "channels": list(ch), "params": net.n_params, "rows": n,
                    "cols": n},
         "exact_svf_ms": round(exact_ms, 3),
         "gated_ms": round(wall, 3), "network_ms": round(rep.ms_network, 3),

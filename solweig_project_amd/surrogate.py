@@ -411,108 +411,139 @@ class GateReport:
     exact_tiles: int
     ms_total: float
     ms_network: float
+    window_errors: np.ndarray = None
 
 
 class SvfSurrogate:
-    """Gated surrogate SVF of a device-resident scene (fields 0..9; the other six field planes keep
-    the exact sweep's values only where a window falls back)."""
+    """Gated surrogate SVF of a device-resident scene.
 
-    def __init__(self, net: SurrogateNet, batch: int = 16, tol: float = 0.02, samples: int = 4, seed: int = 0):
-        self.net, self.batch, self.tol, self.samples, self.seed = net, batch, tol, samples, seed
-        self.ps = Passes(net, batch)
+    The network predicts fields 0..9 of every 256 x 256 core; SVFtotal
+    (field 15) follows from them as in the fold (svf - (1 - svfveg) * 0.97).
+    The gate runs the exact kernel on ``samples`` random 16 x 16 tiles of
+    every core, accepts a core when the mean absolute difference over its
+    sampled tiles and the 10 fields is at most ``tol``, and re-solves the
+    rejected cores exactly. Sampled tiles and rejected cores carry exact
+    values; the aveg fields (10..14) are exact only there.
+    """
+
+    def __init__(self, net: SurrogateNet, tol: float = 0.02, samples: int = 4, seed: int = 0):
+        self.net, self.tol, self.samples, self.seed = net, tol, samples, seed
+        self._geo = {}
+        self.ps = None
+
+    def _geometry(self, rows, cols):
+        key = (rows, cols)
+        g = self._geo.get(key)
+        if g is None:
+            from .incremental import TileGrid
+            dev = self.net.device
+            org = core_origins(rows, cols)
+            grid = TileGrid(rows, cols)
+            rng = np.random.Generator(np.random.PCG64(self.seed))
+            per = CORE // 16
+            tiles, owner = [], []
+            for w, (r0, c0) in enumerate(org + HALO):
+                tr0, tc0 = r0 // 16, c0 // 16
+                ty, tx = min(per, grid.ty - tr0), min(per, grid.tx - tc0)
+                pick = rng.choice(ty * tx, min(self.samples, ty * tx), replace=False)
+                for k in pick:
+                    tiles.append((tr0 + int(k) // tx) * grid.tx + tc0 + int(k) % tx)
+                    owner.append(w)
+            tiles = np.array(tiles, np.int32)
+            # pixel indices of the sampled tiles (inside the site) and their window
+            pix, pw = [], []
+            for t, w in zip(tiles, owner):
+                ty, tx = divmod(int(t), grid.tx)
+                rr = np.arange(ty * 16, min(rows, ty * 16 + 16))
+                cc = np.arange(tx * 16, min(cols, tx * 16 + 16))
+                idx = (rr[:, None] * cols + cc[None, :]).ravel()
+                pix.append(idx)
+                pw.append(np.full(idx.size, w, np.int64))
+            pix = np.concatenate(pix)
+            pw = np.concatenate(pw)
+            win = np.zeros((rows, cols), np.int64)
+            for w, (r0, c0) in enumerate(org + HALO):
+                win[r0:r0 + CORE, c0:c0 + CORE] = w
+            sampled = np.zeros(rows * cols, bool)
+            sampled[pix] = True
+            g = dict(org=org, grid=grid, tiles=tiles, d_tiles=torch.from_numpy(tiles).to(dev),
+                     pix=torch.from_numpy(pix).to(dev), pw=torch.from_numpy(pw).to(dev),
+                     npix=torch.bincount(torch.from_numpy(pw), minlength=len(org)).to(dev).float(),
+                     win=torch.from_numpy(win.ravel()).to(dev), sampled=torch.from_numpy(sampled).to(dev))
+            self._geo[key] = g
+        return g
 
     def predict(self, ds, fields, stream=None):
-        """Network outputs of every core of scene ``ds`` into ``fields`` (16, rows * cols)."""
-        org = core_origins(ds.rows, ds.cols)
+        """Network outputs of every core of scene ``ds`` into fields 0..9 of ``fields`` (one batch)."""
+        g = self._geometry(ds.rows, ds.cols)
+        org = g["org"]
+        if self.ps is None or self.ps.n != len(org):
+            self.ps = Passes(self.net, len(org))
         site = SiteField(ds.hpack, fields, ds.rows, ds.cols)
-        sp = _sp(stream)
-        fm = _field_map(self.net.device)
-        for k in range(0, len(org), self.batch):
-            part = org[k:k + self.batch]
-            n = len(part)
-            if n < self.batch:
-                part = np.concatenate([part, np.repeat(part[-1:], self.batch - n, axis=0)])
-            d_org = load_batch(self.ps, site, part, with_labels=False, stream=stream)
-            self.ps.forward(stream)
-            L.call("sw_cnn_scatter_cores", L.ptr(self.ps.out), L.ptr(d_org), L.ptr(fm), L.ptr(fields), n, self.ps.hw,
-                   COUT_HEAD, self.ps.core, self.ps.c0, N_FIELDS, ds.rows, ds.cols, sp)
+        d_org = load_batch(self.ps, site, org, with_labels=False, stream=stream)
+        self.ps.forward(stream)
+        L.call("sw_cnn_scatter_cores", L.ptr(self.ps.out), L.ptr(d_org), L.ptr(_field_map(self.net.device)),
+               L.ptr(fields), len(org), self.ps.hw, COUT_HEAD, self.ps.core, self.ps.c0, N_FIELDS, ds.rows, ds.cols,
+               _sp(stream))
         return org
 
     def svf(self, solver, ds, stream=None) -> GateReport:
-        """Gated surrogate SVF into ``ds.fields``: network on every core, the exact kernel on
-        ``samples`` sampled 16 x 16 tiles per core, and an exact re-solve of every core whose sampled
-        error exceeds ``tol``. ``ds`` must be composed (sw_compose) beforehand."""
+        """Gated surrogate SVF into ``ds.fields``; ``ds`` must be composed (sw_compose) beforehand."""
         from .incremental import TileGrid
         t0 = time.perf_counter()
         rows, cols = ds.rows, ds.cols
+        g = self._geometry(rows, cols)
+        n = rows * cols
         surr = torch.empty_like(ds.fields)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()
         org = self.predict(ds, surr, stream)
         ev[1].record()
-        g = TileGrid(rows, cols)
-        rng = np.random.Generator(np.random.PCG64(self.seed))
-        per_core = CORE // 16
-        samp = []
-        owner = []
-        for w, (r0, c0) in enumerate(org + HALO):
-            tr0, tc0 = r0 // 16, c0 // 16
-            ty = min(per_core, g.ty - tr0)
-            tx = min(per_core, g.tx - tc0)
-            for _ in range(self.samples):
-                samp.append((tr0 + int(rng.integers(0, ty))) * g.tx + tc0 + int(rng.integers(0, tx)))
-                owner.append(w)
-        samp = np.array(samp, np.int32)
-        uniq, inv = np.unique(samp, return_inverse=True)
-        self._exact_tiles(solver, ds, uniq, stream)
-        # compare on the sampled tiles (fields 0..9): per-tile max abs difference, per window max
-        diff = (surr[:N_FIELDS].view(N_FIELDS, rows, cols) - ds.fields[:N_FIELDS].view(N_FIELDS, rows, cols)).abs()
-        dmax = diff.amax(dim=0)
-        tiles_err = _tile_max(dmax, uniq, g)
-        werr = np.zeros(len(org), np.float32)
-        np.maximum.at(werr, np.array(owner), tiles_err[inv])
-        bad = np.nonzero(werr > self.tol)[0]
-        # accepted windows take the surrogate outside the sampled (exact) tiles
-        keep = torch.ones((rows, cols), dtype=torch.bool, device=ds.fields.device)
-        for w in bad:
-            r0, c0 = (org[w] + HALO).tolist()
-            keep[r0:r0 + CORE, c0:c0 + CORE] = False
-        kt = torch.zeros(g.ty * g.tx, dtype=torch.bool)
-        kt[torch.from_numpy(uniq.astype(np.int64))] = True
-        kt = kt.view(g.ty, g.tx).repeat_interleave(16, 0).repeat_interleave(16, 1)[:rows, :cols].to(keep.device)
-        keep &= ~kt
-        f = ds.fields[:N_FIELDS].view(N_FIELDS, rows, cols)
-        f.copy_(torch.where(keep, surr[:N_FIELDS].view(N_FIELDS, rows, cols), f))
+        self._exact_tiles(solver, ds, g["d_tiles"], len(g["tiles"]), stream)
+        f = ds.fields.view(L.NFIELDS, n)
+        sv = surr.view(L.NFIELDS, n)
+        e = (sv[:N_FIELDS, g["pix"]] - f[:N_FIELDS, g["pix"]]).abs().mean(dim=0)
+        werr = torch.zeros(len(org), device=f.device).index_add_(0, g["pw"], e) / g["npix"]
+        acc = werr <= self.tol
+        keep = acc[g["win"]] & ~g["sampled"]
+        sv[15] = sv[0] - (1.0 - sv[5]) * (1.0 - 0.03)
+        for k in list(range(N_FIELDS)) + [15]:
+            f[k] = torch.where(keep, sv[k], f[k])
+        werr_h = werr.cpu().numpy()
+        bad = np.nonzero(werr_h > self.tol)[0]
         ex = 0
         if len(bad):
-            m = g.empty()
+            m = g["grid"].empty()
             for w in bad:
                 r0, c0 = (org[w] + HALO).tolist()
-                m[r0 // 16:min(g.ty, (r0 + CORE) // 16), c0 // 16:min(g.tx, (c0 + CORE) // 16)] = True
+                m[r0 // 16:min(g["grid"].ty, (r0 + CORE) // 16), c0 // 16:min(g["grid"].tx, (c0 + CORE) // 16)] = True
             ids = TileGrid.ids(m)
             ex = len(ids)
-            self._exact_tiles(solver, ds, ids, stream)
+            self._exact_tiles(solver, ds, torch.from_numpy(ids).to(f.device), ex, stream)
         torch.cuda.synchronize()
-        return GateReport(windows=len(org), accepted=len(org) - len(bad), sampled_tiles=len(uniq), exact_tiles=ex,
-                          ms_total=(time.perf_counter() - t0) * 1e3, ms_network=ev[0].elapsed_time(ev[1]))
+        return GateReport(windows=len(org), accepted=len(org) - len(bad), sampled_tiles=len(g["tiles"]),
+                          exact_tiles=ex, ms_total=(time.perf_counter() - t0) * 1e3,
+                          ms_network=ev[0].elapsed_time(ev[1]), window_errors=werr_h)
 
     @staticmethod
-    def _exact_tiles(solver, ds, ids, stream=None):
-        if len(ids) == 0:
+    def _exact_tiles(solver, ds, d_ids, count, stream=None):
+        if count == 0:
             return
         P = L.ptr
-        d = torch.from_numpy(np.ascontiguousarray(ids, np.int32)).to(ds.fields.device)
         offs, tab = solver.svf_tables(ds.rows, ds.cols, ds.tile.scale)
         c = solver._fold_consts
         L.call("sw_svf_sweep_tiles", P(ds.hpack), P(ds.stats), P(offs), P(tab), P(ds.vegdsm2), P(c["w_iso"]),
                P(c["w_aniso"]), P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(ds.counts), P(ds.hpad), P(ds.masks),
-               P(ds.fields), P(d), int(len(ids)), ds.rows, ds.cols, _sp(stream))
+               P(ds.fields), P(d_ids), int(count), ds.rows, ds.cols, _sp(stream))
 
 
-def _tile_max(dmax: torch.Tensor, ids: np.ndarray, g) -> np.ndarray:
-    """Max of a (rows, cols) plane over each listed 16 x 16 tile."""
-    rows, cols = dmax.shape
-    pad = torch.zeros((g.ty * 16, g.tx * 16), dtype=dmax.dtype, device=dmax.device)
-    pad[:rows, :cols] = dmax
-    t = pad.view(g.ty, 16, g.tx, 16).amax(dim=(1, 3)).reshape(-1)
-    return t[torch.from_numpy(ids.astype(np.int64)).to(t.device)].cpu().numpy()
+DEFAULT_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "svf_surrogate_16_32_64_128.npz"
+
+
+def load_default(device="cuda:0") -> SurrogateNet:
+    """The trained network shipped with the package (channels 16, 32, 64, 128)."""
+    with np.load(DEFAULT_WEIGHTS, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    net = SurrogateNet(tuple(int(c) for c in d["channels"]), device)
+    net.load_state_dict(d)
+    return net

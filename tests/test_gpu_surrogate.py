@@ -144,3 +144,37 @@ def test_window_copies(S):
         np.allclose(x.float().cpu().numpy(), xe, rtol=8e-3, atol=0)
     assert np.array_equal(lab.cpu().numpy(), le)
     assert np.array_equal(dst.cpu().numpy(), de)
+
+
+def test_gated_surrogate_on_a_city(S):
+    """The shipped network on an unseen 768 x 640 city: every rejected core is exact, accepted cores
+    hold the gate's mean error, and an untrained network is rejected everywhere (the gate falls back)."""
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import HipSolver, TileInputs
+    _say("gated surrogate on a 768 x 640 city")
+    solver = HipSolver("cuda:0")
+    site = synth.make_site(768, 640, seed=777, with_aspect=False)
+    ds = solver.upload(TileInputs.from_site(site))
+    solver.svf(ds)
+    torch.cuda.synchronize()
+    exact = ds.fields.clone()
+    net = S.load_default()
+    gate = S.SvfSurrogate(net, tol=0.02, samples=4)
+    rep = gate.svf(solver, ds)
+    f = ds.fields.view(16, 768, 640)
+    ex = exact.view(16, 768, 640)
+    org = S.core_origins(768, 640) + S.HALO
+    err = (f[:10] - ex[:10]).abs()
+    assert rep.accepted >= rep.windows // 2, rep
+    for w, (r0, c0) in enumerate(org):
+        win = err[:, r0:r0 + 256, c0:c0 + 256]
+        if rep.window_errors[w] > 0.02:
+            assert float(win.max()) == 0.0, w           # rejected core: exact re-solve
+        else:
+            assert float(win.mean()) <= 0.03, (w, float(win.mean()))
+    assert float(err.mean()) <= 0.02
+    untrained = S.SurrogateNet(net.channels, seed=5)
+    solver.svf(ds)
+    rep_u = S.SvfSurrogate(untrained, tol=0.02, samples=4).svf(solver, ds)
+    assert rep_u.accepted == 0
+    assert torch.equal(ds.fields[:10], exact[:10])
