@@ -39,11 +39,15 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="side of the CPU-oracle tile (0: the benchmark tile itself)")
-    ap.add_argument("--workload", choices=("c2", "c3", "edits", "surrogate"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "edits", "surrogate", "serve"), default="c2",
                     help="c2: one 1000x1000 tile (default); c3: batched tiles 300/700/1200/2000 squared plus "
                          "64 edit scenarios of a 500x500 base; edits: single-tree edits of a 2000x2000 site "
                          "(p50/p99 latency); surrogate: train the SVF surrogate, then gated surrogate vs "
-                         "exact SVF on a 2000x2000 city")
+                         "exact SVF on a 2000x2000 city; serve: Poisson mix of edits, met-only and full "
+                         "re-solves of a 2000x2000 site at --rate requests/s")
+    ap.add_argument("--requests", type=int, default=1000, help="requests served by --workload serve")
+    ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
+    ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
     ap.add_argument("--train-seconds", type=float, default=240.0, help="surrogate training budget")
     ap.add_argument("--train-sites", type=int, default=40, help="1024x1024 training cities for the surrogate")
@@ -400,6 +404,67 @@ def run_surrogate(args):
     print(json.dumps(line), flush=True)
 
 
+def run_serve(args, pipeline):
+    """C5: Poisson arrivals (60 % single-tree edits, 30 % met-only, 10 % full re-solves) against a
+    device-resident 2000 x 2000 x 24 h site; latency = completion - arrival, queueing included."""
+    import numpy as np
+    import torch
+    from solweig_project_amd import incremental as I
+    from solweig_project_amd import serving as SV
+    from solweig_project_amd import synth
+    from solweig_project_amd import walls as WG
+    from solweig_project_amd.solver import HipSolver, TileInputs
+
+    dev = torch.device("cuda:0")
+    n = 2000
+    seed = args.seed + 40
+    base = synth.make_site(n, n, seed=seed, with_aspect=False)
+    w, a = WG.walls_and_aspect(base.dsm, base.scale)
+    tile = TileInputs.from_site(synth.site_with(base, base.dsm, base.cdsm, base.landcover, walls=w, aspect=a))
+    sun = sun_for_site(base.latitude, base.longitude, base.utc_offset, 2021, 180)
+    mets = [synth.make_met(sun.altitude, seed=seed + 1 + k) for k in range(4)]
+    solver = HipSolver(dev)
+    st = I.solve_site(solver, tile, mets[0], sun, base.latitude)
+    from solweig_project_amd.pipeline import Job
+    pipeline.warm(st.tile, Job(mets[0], sun, base.latitude))
+    server = SV.Server(solver, st, mets, sun, base.latitude, pipeline, slots=args.slots)
+    # warm every path once
+    warm = SV.poisson_workload(6, rate=100.0, mix={"edit": 0.34, "met": 0.33, "full": 0.33}, seed=seed + 7,
+                               site_tile=st.tile, met_pool=mets)
+    server.run(warm, checksums=False)
+    reqs = SV.poisson_workload(args.requests, rate=args.rate, mix={"edit": 0.6, "met": 0.3, "full": 0.1},
+                               seed=seed + 8, site_tile=st.tile, met_pool=mets)
+    log(f"serving {len(reqs)} requests at {args.rate}/s over {reqs[-1].arrival:.1f} s with "
+        f"{len(server.slots)} slots")
+    t0 = time.perf_counter()
+    done = server.run(reqs, checksums=False)
+    wall = time.perf_counter() - t0
+    allp = SV.percentiles(done)
+    T = st.plan.n_steps
+    px_steps = sum(n * n * T for r in done if r.kind in ("met", "full"))
+    line = {
+        "metric": "p99 request latency (ms), C5 serving mix on a 2000² x 24 h site",
+        "value": allp["p99_ms"], "unit": "ms",
+        "n_gpus": 1, "steps": len(done), "warmup": len(warm), "ms_per_step": allp["mean_ms"],
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 2000² city; Poisson arrivals, seeded request mix)",
+        "config": {"workload": f"Poisson arrivals at {args.rate}/s: 60% single-tree edits, 30% met-only day "
+                               f"re-runs, 10% full re-solves; continuous batching over {len(server.slots)} slots "
+                               "(own stream and scene copy each), edits in order on the site",
+                   "rows": n, "cols": n, "requests": len(done)},
+        "all": allp,
+        "edit": SV.percentiles(done, ("edit",)),
+        "met": SV.percentiles(done, ("met",)),
+        "full": SV.percentiles(done, ("full",)),
+        "wall_s": round(wall, 2),
+        "completed_per_s": round(len(done) / wall, 2),
+        "met_full_mpixel_steps_per_s": round(px_steps / wall / 1e6, 1),
+        "edit_modes": {m: sum(1 for r in done if r.kind == "edit" and r.payload.get("mode") == m)
+                       for m in ("tiles", "full", "none")},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -414,6 +479,14 @@ def main():
         return
     if args.workload == "surrogate":
         run_surrogate(args)
+        return
+    if args.workload == "serve":
+        from solweig_project_amd.pipeline import PlanPipeline
+        pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used
+        try:
+            run_serve(args, pipeline)
+        finally:
+            pipeline.close()
         return
     from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
     pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used

@@ -48,6 +48,11 @@ F32 = np.float32
 TILE = 16
 
 
+def _sync(stream):
+    """Wait for the edit's own stream only (other streams keep running)."""
+    (stream if stream is not None else torch.cuda.current_stream()).synchronize()
+
+
 def tile_amax(dsm: np.ndarray, dem: np.ndarray, cdsm: np.ndarray) -> np.float32:
     """max(a, trees + DEM) as the compose step forms it (float32, trees clamped at 0)."""
     t1 = np.maximum(np.asarray(cdsm, F32), F32(0))
@@ -335,7 +340,7 @@ def apply_edit_window(solver: HipSolver, st: SiteState, box, dsm=None, cdsm=None
         solver.svf(ds, stream)
         solver.tile_static(ds, st.dplan, stream)
         solver.run_day(ds, st.plan, st.dplan, st.outputs, out=st.out, stream=stream)
-        torch.cuda.synchronize()
+        _sync(stream)
         st.amax = amax
         st.offsets = march_offsets(solver, tile, st.plan, amax)
         info.update(mode="full", ms=(time.perf_counter() - t0) * 1e3)
@@ -359,7 +364,7 @@ def apply_edit_window(solver: HipSolver, st: SiteState, box, dsm=None, cdsm=None
     L.call("sw_solve_day_tiles", ctypes.byref(ds.tile_struct()), P(st.dplan["steps"]), P(st.dplan["model"]),
            P(st.dplan["sun_tab"]), is_day, 0, T, P(st.out), st.mask, st.plan.walk_halo, 1, P(dev_ids[1]),
            int(ids[1].size), P(dev_ids[2]), int(ids[2].size), sp)
-    torch.cuda.synchronize()
+    _sync(stream)
     g = TileGrid(rows, cols)
     info.update(mode="tiles", svf_tiles=int(ids[0].size), sun_tiles=int(ids[1].size), flux_tiles=int(ids[2].size),
                 grid_tiles=g.ty * g.tx, ms=(time.perf_counter() - t0) * 1e3)
