@@ -85,7 +85,9 @@ typedef struct sw_step {
     float sunlit_k, shaded_k, sunlit_l, shaded_l, veg_l;
     float tgwall, lwall;
     float tg_class[SW_NCLASS];
-    int32_t pad0[3];
+    float wbgt_w;             /* f32(0.7 * isobaric wet-bulb temperature, degC) */
+    float wbgt_a;             /* black globe: h_cg / (emissivity * sigma), h_cg = 13.455 va^0.6 */
+    int32_t pad0[1];
     sw_patch_k_t pk[SW_NPATCH];
     sw_patch_l_t pl[SW_NPATCH];
 } sw_step_t;
@@ -255,7 +257,9 @@ int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
  * solweig.py:2101-2311; utci_calculator, calculate_utci.py:259-330).
  * state: 6 planes (Tgmap1, E, S, W, N, TgOut1), read and updated.
  * out: planes selected by out_mask bits: 1 tmrt, 2 utci, 4 shadow,
- * 8 kdown, 16 kup, 32 ldown, 64 lup, written at out + k * n for the k-th
+ * 8 kdown, 16 kup, 32 ldown, 64 lup, 128 wbgt (utci_process.py:1021-1037:
+ * black globe from Tmrt, the step's air temperature and wbgt_a, blended
+ * with wbgt_w by shadow < 0.1), written at out + k * n for the k-th
  * selected plane in that bit order. walk_halo sizes the shared-memory tile
  * of walk sources and must be at least model->walk_halo for a day step;
  * -1 launches without the tile (night steps: less shared memory, more

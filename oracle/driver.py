@@ -21,6 +21,7 @@ from pathlib import Path
 
 import numpy as np
 
+from oracle import wbgt as WB
 from solweig_project_amd import met as M
 from solweig_project_amd import skyvault as SV
 
@@ -165,6 +166,7 @@ def _step_struct(sp: M.StepParams):
 
 
 OUT_NAMES = ("tmrt", "utci", "kdown", "kup", "ldown", "lup", "shadow")
+ALL_OUT = OUT_NAMES + ("wbgt",)
 
 
 def day(sc: Scene, fields, shm, vm, bm, walls, aspect_deg, landcover, met, sun: M.SunTable,
@@ -190,7 +192,7 @@ def day(sc: Scene, fields, shm, vm, bm, walls, aspect_deg, landcover, met, sun: 
     loop = M.DayLoop(met, sun, latitude=latitude, landcover=lcc, scale=sc.scale, amax=sc.amax,
                      rows=rows, cols=cols)
     T = met.shape[0] if steps is None else steps
-    outs = {k: np.empty((T, rows, cols), F32) for k in OUT_NAMES}
+    outs = {k: np.empty((T, rows, cols), F32) for k in ALL_OUT}
     tgmap = np.zeros(6 * n, F32)
     sunout = np.empty(8 * n, F32); work = np.empty(25 * n, F32)
     gvf = np.empty(15 * n, F32)
@@ -226,6 +228,7 @@ def day(sc: Scene, fields, shm, vm, bm, walls, aspect_deg, landcover, met, sun: 
                   _p(o["ldown"]), _p(o["lup"]), _p(o["shadow"]))
         for k in OUT_NAMES:
             outs[k][i] = o[k].reshape(rows, cols)
+        outs["wbgt"][i] = WB.wbgt(outs["tmrt"][i], outs["shadow"][i], F32(sp.ta_mat), sp.wbgt_w, F32(sp.wbgt_a))
     return outs
 
 

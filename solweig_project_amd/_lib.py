@@ -54,7 +54,8 @@ class StepStruct(ctypes.Structure):
         ("sunlit_k", _c_f), ("shaded_k", _c_f), ("sunlit_l", _c_f), ("shaded_l", _c_f), ("veg_l", _c_f),
         ("tgwall", _c_f), ("lwall", _c_f),
         ("tg_class", _c_f * NCLASS),
-        ("pad0", _c_i32 * 3),
+        ("wbgt_w", _c_f), ("wbgt_a", _c_f),
+        ("pad0", _c_i32 * 1),
         ("pk", PatchK * NPATCH), ("pl", PatchL * NPATCH),
     ]
 

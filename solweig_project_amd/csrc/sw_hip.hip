@@ -950,6 +950,34 @@ __device__ __forceinline__ float utci_point(float ta, float rh, float tmrt, floa
     return u;
 }
 
+// ---------------------------------------------------------------------------
+// WBGT (utci_process.py:1021-1033, black globe calculate_wbgt.py:717-761)
+// ---------------------------------------------------------------------------
+// Black-globe temperature from the closed-form root of the globe heat balance
+// Tg^4 + a Tg = (Tmrt + 273.15)^4 + a (Ta + 273.15), a = h_cg / (eps sigma),
+// in float32 with one rounding per operation; powers are correctly rounded
+// (through double). Large a overflows n to inf and Tg to -inf, as float32
+// arithmetic does. The clamps let NaN through. WBGT blends the step's
+// f32(0.7 Tw) with the globe (and air) temperature by the sun/shade test
+// shadow < 0.1.
+__device__ __forceinline__ float d_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+
+__device__ __forceinline__ float wbgt_point(float tmrt, float shadow, float ta, float w07, float a) {
+    const float b = d_pow(tmrt + 273.15f, 4.f) + a * (ta + 273.15f);
+    const float m = 9.f * (a * a);
+    const float n = 27.f * d_pow(a, 4.f);
+    const float p = 256.f * ((b * b) * b);
+    const float E = d_pow(m + 1.73205f * sqrtf(n + p), 0.333333343f);
+    const float Q = 3.4943f * b;
+    const float k = 0.381571f * E - Q / E;
+    const float ks = (k < 1.17549435e-38f) ? 1.17549435e-38f : k;
+    float inner = 2.f * a / sqrtf(ks) - k;
+    inner = (inner < 0.f) ? 0.f : inner;
+    const float kz = (k < 0.f) ? 0.f : k;
+    const float tg = 0.5f * sqrtf(inner) - 0.5f * sqrtf(kz) - 273.15f;
+    return (shadow < 0.1f) ? w07 + 0.3f * tg : (w07 + 0.2f * tg) + 0.1f * ta;
+}
+
 __global__ void k_utci(const float *__restrict__ ta, const float *__restrict__ rh,
                        const float *__restrict__ tmrt, const float *__restrict__ va,
                        float *__restrict__ out, long n) {
@@ -1163,7 +1191,7 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     if ((!WALK && sp->is_day) || (day && !staged)) {
         // the shared tile is smaller than the walk reach: refuse loudly
         int k = 0;
-        for (int b = 1; b <= 64; b <<= 1)
+        for (int b = 1; b <= 128; b <<= 1)
             if (out_mask & b) out[(long)(k++) * n + t] = __builtin_nanf("");
         return;
     }
@@ -1352,6 +1380,7 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     if (out_mask & 16) out[(long)(k++) * n + t] = Kup;
     if (out_mask & 32) out[(long)(k++) * n + t] = Ldown;
     if (out_mask & 64) out[(long)(k++) * n + t] = Lup;
+    if (out_mask & 128) out[(long)(k++) * n + t] = wbgt_point(tmrt, shadow, sp->ta_mat, sp->wbgt_w, sp->wbgt_a);
 }
 
 template <bool WALK>
@@ -1585,7 +1614,7 @@ int sw_solve_day_tiles(const sw_tile_t *tile, const sw_step_t *steps, const sw_m
     int rc = check_shape(tile->rows, tile->cols, "sw_solve_day_tiles");
     if (rc) return rc;
     if (t0 < 0 || t1 < t0) return set_err(SW_E_ARG, "sw_solve_day_tiles: invalid step range");
-    if (out_mask <= 0 || out_mask > 127) return set_err(SW_E_ARG, "sw_solve_day_tiles: out_mask selects nothing");
+    if (out_mask <= 0 || out_mask > 255) return set_err(SW_E_ARG, "sw_solve_day_tiles: out_mask selects nothing");
     if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_solve_day_tiles: walk halo out of range");
     if (nsun < 0 || nflux < 0) return set_err(SW_E_ARG, "sw_solve_day_tiles: negative tile count");
     const sw_tile_t &T = *tile;
@@ -1824,7 +1853,7 @@ int sw_solve_batch(const sw_batch_item_t *items_dev, const sw_batch_item_t *item
     if (!items_dev || !items_host || nitems <= 0 || !model || !is_day)
         return set_err(SW_E_ARG, "sw_solve_batch: NULL pointer or empty batch");
     if (t0 < 0 || t1 < t0) return set_err(SW_E_ARG, "sw_solve_batch: invalid step range");
-    if (out_mask <= 0 || out_mask > 127) return set_err(SW_E_ARG, "sw_solve_batch: out_mask selects nothing");
+    if (out_mask <= 0 || out_mask > 255) return set_err(SW_E_ARG, "sw_solve_batch: out_mask selects nothing");
     if (walk_halo < 0 || walk_halo > 48) return set_err(SW_E_ARG, "sw_solve_batch: walk halo out of range");
     long sun_total = 0, flux_total = 0;
     int rc = check_batch(items_host, nitems, &sun_total, &flux_total);

@@ -28,12 +28,6 @@ __global__ void k_fill_f32(float *__restrict__ p, float v, long n) {
     if (i < n) p[i] = v;
 }
 
-int popcount7(int m) {
-    int k = 0;
-    for (int b = 0; b < 7; ++b) k += (m >> b) & 1;
-    return k;
-}
-
 }  // namespace
 
 extern "C" {
@@ -233,10 +227,10 @@ static int solve_day(const sw_tile_t *tile, const sw_step_t *steps, const sw_mod
     int rc = swi_check_shape(tile->rows, tile->cols, "sw_solve_day");
     if (rc) return rc;
     if (t0 < 0 || t1 < t0) return swi_set_err(SW_E_ARG, "sw_solve_day: invalid step range");
-    if (out_mask <= 0 || out_mask > 127) return swi_set_err(SW_E_ARG, "sw_solve_day: out_mask selects nothing");
+    if (out_mask <= 0 || out_mask > 255) return swi_set_err(SW_E_ARG, "sw_solve_day: out_mask selects nothing");
     const int rows = tile->rows, cols = tile->cols;
     const long n = (long)rows * cols;
-    const long stride = (long)popcount7(out_mask) * n;
+    const long stride = (long)__builtin_popcount((unsigned)out_mask) * n;
     if (reset_state) {
         rc = sw_fill_f32(tile->state, 0.f, 6 * (int64_t)n, stream);
         if (rc) return rc;

@@ -191,6 +191,9 @@ class StepParams:
     tw_update: bool = False    # timeadd >= 59/1440 -> state blended
     tw_w: float = 0.0          # f32(weight1)
     tw_omw: float = 0.0        # f32(1 - weight1)
+    # WBGT
+    wbgt_w: float = 0.0        # f32(0.7 * wet-bulb temperature in degC, float64)
+    wbgt_a: float = 0.0        # black-globe h_cg / (emissivity * sigma), float32
     # sky patches (153)
     lv: np.ndarray = field(default_factory=lambda: np.zeros(153, F32))
     lumchi: np.ndarray = field(default_factory=lambda: np.zeros(153, F32))
@@ -512,6 +515,143 @@ def _tgwave(amp, tmaxlst, frac_minus_snup, snup24):
     return amp * torch.sin((frac_minus_snup / den) * np.pi / 2)
 
 
+# ---------------------------------------------------------------------------
+# WBGT forcing scalars: isobaric wet-bulb temperature (liquid phase, Romps
+# Newton iteration; calculate_wbgt.py:17-63, 156-295, 412-511, 512-715) and
+# the black-globe convection coefficient (utci_process.py:1021-1030,
+# calculate_wbgt.py:717-745).
+# ---------------------------------------------------------------------------
+WB_RD, WB_RV = 287.0, 461.5          # gas constants of dry air / water vapour (J/kg/K)
+WB_EPS = WB_RD / WB_RV
+WB_CPD, WB_CPV, WB_CPL = 1005.0, 2040.0, 4220.0   # isobaric specific heats (J/kg/K)
+WB_T0, WB_ES0, WB_LV0 = 273.16, 611.657, 2.501e6  # triple point (K, Pa) and latent heat there (J/kg)
+WB_PRECISION, WB_MAX_ITER = 0.001, 20
+GLOBE_EMISSIVITY, GLOBE_SIGMA = 0.95, 5.670374419e-8
+GLOBE_HCG_COEF = 6.3 / 0.46821     # h_cg = coef * va^0.6
+
+
+def latent_heat_liquid(T):
+    """Lv(T) = Lv0 + (cpv - cpl) (T - T0) (calculate_wbgt.py:93-107)."""
+    return WB_LV0 + (WB_CPV - WB_CPL) * (T - WB_T0)
+
+
+def svp_liquid(T):
+    """Saturation vapour pressure over liquid water in Pa (Ambaum 2020 eq. 13,
+    calculate_wbgt.py:196-205): es0 (T0/T)^((cpl-cpv)/Rv) exp(Lv0/(Rv T0) - Lv(T)/(Rv T))."""
+    lv = latent_heat_liquid(T)
+    return WB_ES0 * np.power(WB_T0 / T, (WB_CPL - WB_CPV) / WB_RV) * np.exp(
+        WB_LV0 / (WB_RV * WB_T0) - lv / (WB_RV * T))
+
+
+def q_from_vapour_pressure(p, e):
+    """Specific humidity from vapour pressure (inverse of calculate_wbgt.py:171-174)."""
+    return WB_EPS * e / (p - (1 - WB_EPS) * e)
+
+
+# PSEM rational approximations of the lower Lambert-W branch (Vazquez-Leal et
+# al. 2019), on [-1/e, -0.34), [-0.34, -0.1) and [-0.1, 0): numerator and
+# denominator coefficients a1.., b1.. (calculate_wbgt.py:314-349).
+_W1_A = (-7.874564067684664, -63.11879948166995, -168.6110850408981, -150.1089086912451)
+_W1_B = (15.97679839497612, 98.26612857148953, 293.9558944644677, 430.4471947824411, 247.8576700279611)
+_W2_A = (-1362.78381643109, -1386.04132570149, 11892.1649836015, 16904.0507511421)
+_W2_B = (251.440197724561, -1264.99554712435, -5687.63429510978, -2639.24130979048)
+_W3_A = (1.01999365162218, -12.6917365519443, -45.1506015092455)
+_W3_B = (-22.9809693297808, -104.692066099727, -95.2085341727207)
+
+
+def _horner(x, coefs):
+    """x * (c1 + x * (c2 + ... x * cn))."""
+    acc = coefs[-1]
+    for c in coefs[-2::-1]:
+        acc = c + x * acc
+    return x * acc
+
+
+def lambertw_lower(x):
+    """W_{-1}(x) for x in [-1/e, 0), NaN elsewhere: the PSEM approximation of
+    the interval containing x plus one Fritsch-Shafer-Crowley step
+    (calculate_wbgt.py:297-361). Evaluated in float64."""
+    x = np.asarray(x, np.float64)
+    y = np.full(x.shape, np.nan)
+    with np.errstate(all="ignore"):
+        s1 = (x >= -0.3678794411714423) & (x < -0.34)
+        y1 = (_horner(x, _W1_A) / (1.0 + _horner(x, _W1_B))) * (x + math.exp(-1.0)) - 1.0
+        s2 = (x >= -0.34) & (x < -0.1)
+        y2 = _horner(x, _W2_A) / (1.0 + _horner(x, _W2_B))
+        s3 = (x >= -0.1) & (x < 0.0)
+        l1 = np.log(-x)
+        y3 = _horner(x, _W3_A) / (1.0 + _horner(x, _W3_B)) + (l1 - np.log(-l1) + np.log(-l1) / l1)
+        y = np.where(s1, y1, np.where(s2, y2, np.where(s3, y3, y)))
+        z = np.log(x / y) - y
+        t = 2.0 * (1.0 + y) * (1.0 + y + (2.0 / 3.0) * z)
+        return y * (1.0 + (z / (1.0 + y)) * (t - z) / (t - 2.0 * z))
+
+
+def dewpoint_liquid(p, T, q):
+    """Dewpoint over liquid water (Romps 2021 closed form through W_{-1},
+    calculate_wbgt.py:419-428), limited to T (:469-470)."""
+    e = p * q / (WB_EPS * (1 - q) + q)
+    rh = e / svp_liquid(T)
+    beta = (WB_CPL - WB_CPV) / WB_RV
+    alpha = -(1 / beta) * (WB_LV0 + (WB_CPL - WB_CPV) * WB_T0) / WB_RV
+    with np.errstate(all="ignore"):
+        arg = np.power(rh, 1 / beta) * (alpha / T) * np.exp(alpha / T)
+    return np.minimum(alpha / lambertw_lower(arg).astype(np.result_type(arg)), T)
+
+
+@dataclass
+class WetBulb:
+    """Isobaric wet-bulb temperature of every forcing row (degC) and the
+    Newton iterations the whole table took."""
+    tw_c: np.ndarray
+    iterations: int
+    converged: bool
+
+
+def wet_bulb_from_rh(p_pa, t_k, rh_pct) -> WetBulb:
+    """Liquid-phase isobaric wet-bulb temperature by Romps' Newton iteration
+    (calculate_wbgt.py:476-511, 596-679, 691-715).
+
+    The iteration runs over the whole array and stops when the largest
+    update (NaN ignored) is below 0.001 K, or after 20 updates; the result is
+    limited to T. Arithmetic is in the inputs' dtype (float64 for forcing
+    tables)."""
+    p = np.asarray(p_pa)
+    T = np.asarray(t_k)
+    es = svp_liquid(T)
+    q = q_from_vapour_pressure(p, (np.asarray(rh_pct) / 100.0) * es)
+    tw = T - (1 / 3) * (T - dewpoint_liquid(p, T, q))
+    cpm = (1 - q) * WB_CPD + q * WB_CPV
+    dlv = WB_CPV - WB_CPL
+    count, converged = 0, False
+    with np.errstate(all="ignore"):
+        while True:
+            prev = tw
+            esw = svp_liquid(tw)
+            qs = WB_EPS * esw / (p - (1 - WB_EPS) * esw)
+            lv = latent_heat_liquid(tw)
+            dqs = qs * (1 - qs + qs / WB_EPS) * lv / (WB_RV * tw ** 2)
+            f = cpm * (T - tw) * (1 - qs) - (qs - q) * lv
+            fp = -(cpm * (T - tw) + lv) * dqs - cpm * (1 - qs) - (qs - q) * dlv
+            tw = tw - f / fp
+            d = np.abs(tw - prev)
+            if d.size and not np.all(np.isnan(d)) and np.nanmax(d) < WB_PRECISION:
+                converged = True
+                break
+            count += 1
+            if count == WB_MAX_ITER:
+                break
+    return WetBulb(np.minimum(T, tw) - 273.15, count + 1, converged)
+
+
+def globe_coefficient(va: float) -> float:
+    """a = h_cg / (emissivity * sigma) with h_cg = coef * va^0.6, in float32
+    like the reference's tensors (utci_process.py:1022-1025,
+    calculate_wbgt.py:733-740)."""
+    hcg = F32(GLOBE_HCG_COEF) * np.power(F32(va), F32(0.6))
+    return float(hcg / (F32(GLOBE_EMISSIVITY) * F32(GLOBE_SIGMA)))
+
+
 class DayLoop:
     """Iterates the host chain over the forcing rows of one tile/day."""
 
@@ -545,6 +685,9 @@ class DayLoop:
         self.timestepdec = 0 if n == 1 else (sun.dectime[1] - sun.dectime[0])
         self.dectime = np.asarray(sun.dectime, dtype=np.float64)
         self.jday_row = np.asarray(sun.jday, dtype=np.float64)
+        # the wet-bulb iteration spans every forcing row at once (utci_process.py:817-823)
+        self.wet_bulb = wet_bulb_from_rh(self.P.numpy() * 1000.0, (self.Ta + self.uhii).numpy() + 273.15,
+                                         self.RH.numpy())
 
     # -- midnight bookkeeping (utci_process.py:917-934) --------------------
     def _midnight(self, i):
@@ -588,6 +731,8 @@ class DayLoop:
         sp.rh = float((torch.zeros(1) + RH)[0])
         va = torch.clamp(torch.ones(1) * self.Ws[i], min=0.15)
         sp.va = float(va[0])
+        sp.wbgt_w = float(F32(0.7 * float(self.wet_bulb.tw_c[i])))
+        sp.wbgt_a = globe_coefficient(sp.va)
         tak4_64 = (Ta + 273.15) ** 4
         sp.tak4 = float(np.float32(tak4_64.item()))
         deg2rad_t = torch.tensor(np.pi / 180)

@@ -31,8 +31,8 @@ from . import skyvault as SV
 F32 = np.float32
 
 #: output plane bits of sw_flux_step, in packing order
-OUT_BITS = {"tmrt": 1, "utci": 2, "shadow": 4, "kdown": 8, "kup": 16, "ldown": 32, "lup": 64}
-OUT_ORDER = ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup")
+OUT_BITS = {"tmrt": 1, "utci": 2, "shadow": 4, "kdown": 8, "kup": 16, "ldown": 32, "lup": 64, "wbgt": 128}
+OUT_ORDER = ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup", "wbgt")
 
 
 class UnsupportedInput(ValueError):
@@ -173,7 +173,7 @@ def _fill(arr, vals):
 STEP_DTYPE = np.dtype(L.StepStruct)
 _SCALARS = ("ta", "ta_mat", "rh", "va", "tak4", "radI", "radD", "radG", "sinalt", "cosalt_k",
             "tan_zen", "one_m_psi", "tw_w", "tw_omw", "sunlit_k", "shaded_k", "sunlit_l",
-            "shaded_l", "veg_l", "tgwall", "lwall")
+            "shaded_l", "veg_l", "tgwall", "lwall", "wbgt_w", "wbgt_a")
 
 
 def step_records(params, sun_offs) -> np.ndarray:
@@ -270,6 +270,7 @@ class DayPlan:
     walk_halo: int
     is_day: list
     params: list = field(default_factory=list)
+    wet_bulb: object = None    # met.WetBulb of the whole forcing table (WBGT)
 
 
 def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float,
@@ -301,7 +302,7 @@ def plan_day(tile: TileInputs, met: np.ndarray, sun: M.SunTable, latitude: float
     mstruct = _model_struct_cached(tile.scale, lcc)
     return DayPlan(steps=steps, sun_tab=np.ascontiguousarray(sun_tab), model=L.struct_bytes(mstruct),
                    n_steps=len(params), walk_halo=int(mstruct.walk_halo), is_day=[p.is_day for p in params],
-                   params=params)
+                   params=params, wet_bulb=loop.wet_bulb)
 
 
 def plan_for_shape(plan: DayPlan, rows: int, cols: int, scale: float) -> DayPlan:

@@ -43,9 +43,7 @@ def test_model_parameters():
         api.run_utci_window(**_kw(model_parameters={"absK": 0.6, "absL": 0.95}))
 
 
-def test_wbgt_and_directional_wind_are_refused():
-    with pytest.raises(UnsupportedInput, match="WBGT"):
-        api.run_utci_window(**_kw(save_wbgt=True))
+def test_directional_wind_is_refused():
     with pytest.raises(UnsupportedInput, match="windcoeff_by_dir"):
         api.run_utci_window(**_kw(windcoeff_by_dir={0: np.ones((8, 8))}))
 

@@ -1,7 +1,7 @@
 """HIP path (through the C ABI) against the reference's golden vectors and the CPU oracle.
 
 Bars: shadow / vegetation / wall masks bit-exact; SVF <= 1e-5; Tmrt <= 0.01 K;
-UTCI <= 0.01 C. The oracle is only the checker here.
+UTCI and WBGT <= 0.01 C. The oracle is only the checker here.
 """
 import numpy as np
 import pytest
@@ -141,7 +141,7 @@ def test_c1_matches_oracle(solver, c1_case):
     s = c1_case["site"]
     scene = compose_scene(s.dsm, s.dem, s.cdsm, s.walls, s.aspect, s.landcover, scale=s.scale)
     fields, out, ds = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude,
-                                   outputs=("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup"))
+                                   outputs=("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup", "wbgt"))
     cubes = [c.cpu().numpy() for c in solver.unpack_masks(ds)]
     rows, cols = scene.shape
     for got, ref in zip(cubes, c1_case["cubes"]):
@@ -153,6 +153,8 @@ def test_c1_matches_oracle(solver, c1_case):
     m = ~np.isnan(o["utci"])
     assert np.array_equal(np.isnan(out["utci"]), ~m)
     assert np.abs(out["utci"][m] - o["utci"][m]).max() <= 0.01
+    assert np.all(np.isfinite(out["wbgt"]))
+    assert np.abs(out["wbgt"] - o["wbgt"]).max() <= 0.01
 
 
 def test_repeat_runs_bitwise_identical(solver, c1_case):
