@@ -26,6 +26,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA
 
 
 def parse():
@@ -51,9 +52,12 @@ def parse():
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
     ap.add_argument("--train-seconds", type=float, default=240.0, help="surrogate training budget")
     ap.add_argument("--train-sites", type=int, default=40, help="1024x1024 training cities for the surrogate")
-    ap.add_argument("--surrogate-weights", default="", help="trained surrogate weights (.npz); trains when absent")
+    ap.add_argument("--surrogate-weights",
+                    default=str(Path(__file__).resolve().parent / "solweig_project_amd" / "data"
+                                / "svf_surrogate_16_32_64_128.npz"),
+                    help="trained surrogate weights (.npz; default: the shipped network); '' trains a new one")
     ap.add_argument("--save-weights", default="", help="write the trained surrogate weights here")
-    ap.add_argument("--channels", default="32,64,128,256", help="surrogate level channels")
+    ap.add_argument("--channels", default="16,32,64,128", help="surrogate level channels")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run k_pack_heights over 16 Mi pixels (12 B read + 16 B written per pixel) "
                          "as the known-bytes reference for rocprof FETCH_SIZE / WRITE_SIZE")
@@ -377,6 +381,14 @@ def run_surrogate(args):
     untrained = SG.SurrogateNet(ch, dev, seed=args.seed + 7)
     rep_u, wall_u, err_u = gated(untrained)
     exact_ms = float(np.median(ex_ms))
+    # MFMA roofline of the network: 2 x MACs per 384^2 window (model channels) x windows / network time
+    flops = 2.0 * SG.macs_per_window(ch) * rep.windows
+    ach = flops / (rep.ms_network * 1e-3) / 1e12
+    roof = {"kernel": "sw_cnn_fwd (window pack + 15 conv layers + scatter)", "bound": "mfma",
+            "achieved": round(ach, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ach / BF16_PEAK_TFLOPS, 4), "traffic": None,
+            "flop_per_window": 2 * SG.macs_per_window(ch), "windows": rep.windows,
+            "network_ms": round(rep.ms_network, 3)}
     line = {
         "metric": "surrogate vs exact SVF (speedup of the gated surrogate path, 2000² city, 10 SVF fields)",
         "value": round(exact_ms / wall, 3),
@@ -388,6 +400,7 @@ def run_surrogate(args):
                                "exact kernel on 4 sampled 16x16 tiles per core, accepted when their mean absolute "
                                "error over the 10 fields is <= 0.02, exact re-solve of rejected cores", "channels": list(ch), "params": net.n_params, "rows": n,
                    "cols": n},
+        "roofline": roof,
         "exact_svf_ms": round(exact_ms, 3),
         "gated_ms": round(wall, 3), "network_ms": round(rep.ms_network, 3),
         "windows": rep.windows, "accepted": rep.accepted, "sampled_tiles": rep.sampled_tiles,

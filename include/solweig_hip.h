@@ -396,6 +396,12 @@ typedef struct sw_cnn_op {
     const void *dz;
     float *gw, *gb;
     void *ws;
+    /* forward conv epilogues (3x3, 16 output channels only): pool_out receives the 2x2 average
+     * pool of the stored output (bit-identical to an SW_CNN_POOL2 op after it); head_w (bf16
+     * [16][16]) and head_b (f32 [16]) run a 1x1 sigmoid head on the stored output into outf */
+    void *pool_out;
+    const void *head_w;
+    const float *head_b;
 } sw_cnn_op_t;
 
 int sw_cnn_fwd(const sw_cnn_op_t *ops, int nops, void *stream);
@@ -414,6 +420,16 @@ int sw_cnn_gather_labels(const float *fields, const int32_t *origins, const int3
 /* surrogate outputs (f32 [n][hw][hw][cs]) of the window cores into SVF field planes */
 int sw_cnn_scatter_cores(const float *out, const int32_t *origins, const int32_t *field_map, float *fields, int n,
                          int hw, int cs, int core, int c0, int nl, int rows, int cols, void *stream);
+/* Surrogate gate (no counterpart in the reference). sw_gate_error: per window w, the mean over its
+ * sampled pixels pix[seg[w] .. seg[w + 1]) (flat indices inside window w's core) of the mean
+ * absolute difference between the network output out [w][hw][hw][cs] and the exact field planes
+ * fields[fm[k]] over k < nl; fixed-order reduction (repeatable). sw_gate_write: the output cores
+ * [c0, c0 + core) of every window into fields[fm[k]] and SVFtotal (plane 15 = svf - (1 - svfveg) *
+ * 0.97, from output channels 0 and 5) except at pixels with sampled[p] != 0. */
+int sw_gate_error(const float *out, const float *fields, const int32_t *org, const int32_t *pix, const int32_t *seg,
+                  const int32_t *fm, float *werr, int nwin, int hw, int cs, int nl, int rows, int cols, void *stream);
+int sw_gate_write(const float *out, const int32_t *org, const int32_t *fm, const uint8_t *sampled, float *fields,
+                  int nwin, int hw, int cs, int core, int c0, int nl, int rows, int cols, void *stream);
 
 #ifdef __cplusplus
 }

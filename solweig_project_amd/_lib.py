@@ -106,7 +106,7 @@ class CnnOp(ctypes.Structure):
     _fields_ = [(k, _c_i32) for k in ("kind", "n", "h", "w", "c1", "c2", "up", "o1", "cout", "ks", "act", "nchunk",
                                       "core", "c0", "nl", "pad")] + \
         [(k, ctypes.c_void_p) for k in ("in0", "in1", "out0", "out1", "outf", "weight", "bias", "dz", "gw", "gb",
-                                        "ws")]
+                                        "ws", "pool_out", "head_w", "head_b")]
 
 
 CNN_CONV, CNN_CONV_BWD_DATA, CNN_CONV_BWD_WEIGHT, CNN_POOL2, CNN_POOL2_BWD, CNN_UP2_BWD, CNN_RELU_BWD, \
@@ -188,6 +188,8 @@ SIGNATURES = {
     "sw_cnn_pack_input": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _VP]),
     "sw_cnn_gather_labels": (_I, [_VP] * 4 + [_I] * 6 + [_VP]),
     "sw_cnn_scatter_cores": (_I, [_VP] * 4 + [_I] * 8 + [_VP]),
+    "sw_gate_error": (_I, [_VP] * 7 + [_I] * 6 + [_VP]),
+    "sw_gate_write": (_I, [_VP] * 5 + [_I] * 8 + [_VP]),
     "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),
     "sw_wall_filter_size": (_I, [ctypes.c_double]),
     "sw_wall_aspect": (_I, [_VP] * 4 + [_I, _I, ctypes.c_double, _VP]),

@@ -275,6 +275,168 @@ __global__ void __launch_bounds__(256) k_conv_d(ConvArgs a) {
     }
 }
 
+// Direct 3x3 convolution with 16 output channels (the full-resolution layers and the data
+// gradients into 16-channel tensors), output stationary on v_mfma_f32_16x16x32_bf16 with the
+// weights as the row operand: C[co][pixel], so a lane ends with 4 consecutive channels of one
+// pixel (8-byte stores, 512 contiguous bytes per wave store). K pairs two taps of one 16-channel
+// chunk (lane groups 0-1: tap t, groups 2-3: tap t + 1; the ninth tap pairs with zeros). A block
+// owns an 8 x 32 pixel tile; each wave two rows as four 16-pixel column tiles.
+//   pool (POOL): also writes the 2x2 average of the stored bf16 outputs to pool_out, in the
+//   summation order of k_pool2 (bit-identical to a separate pool op); needs even h, w.
+//   head (HEAD): runs the 1x1 sigmoid head (16 -> 16 channels) on the bf16-rounded outputs with
+//   v_mfma_f32_16x16x16bf16_1k, whose B operand layout (k = 4 * (lane / 16) + j) is this
+//   kernel's C layout, and writes the f32 sigmoid to outf; out1 may then be null.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct Conv16Args {
+    ConvArgs c;
+    __bf16 *pool_out;        // (n, h/2, w/2, 16) or null
+    const __bf16 *head_w;    // [16][16] (co2, ci) or null
+    const float *head_b;     // [16] or null
+};
+
+template <int NT, bool POOL, bool HEAD>
+__global__ void __launch_bounds__(256) k_conv_n16(Conv16Args ca) {
+    constexpr int TH = 8, TW = 32, HH = TH + 2, HW2 = TW + 2;
+    static_assert(!HEAD || NT == 1, "the head follows a 16-channel layer");
+    const ConvArgs &a = ca.c;
+    __shared__ bf16x8 sX[HH * HW2 * 2];         // [halo cell][k half]
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6, g = l >> 4, r16 = l & 15;
+    const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+    const int t = blockIdx.x;
+    const int img = t / (tiles_x * tiles_y);
+    const int y0 = ((t / tiles_x) % tiles_y) * TH, x0 = (t % tiles_x) * TW;
+    const int cin = a.c1 + a.c2;
+    const int hh = a.h >> 1, wh = a.w >> 1;
+    const int co0 = blockIdx.y * 16 * NT;         // first output channel of the block
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = (f32x4){};
+    for (int cb = 0; cb < cin; cb += 16) {
+        for (int e = tid; e < HH * HW2 * 2; e += 256) {
+            const int cell = e >> 1, half = e & 1;
+            const int yy = y0 - 1 + cell / HW2, xx = x0 - 1 + cell % HW2;
+            bf16x8 v = (bf16x8){};
+            if ((unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w) {
+                const int ch = cb + 8 * half;
+                if (ch < a.c1) {
+                    const long p = a.up1 ? (((long)img * hh + (yy >> 1)) * wh + (xx >> 1))
+                                         : (((long)img * a.h + yy) * a.w + xx);
+                    v = *(const bf16x8 *)(a.in1 + p * a.c1 + ch);
+                } else {
+                    const long p = ((long)img * a.h + yy) * a.w + xx;
+                    v = *(const bf16x8 *)(a.in2 + p * a.c2 + (ch - a.c1));
+                }
+            }
+            sX[e] = v;
+        }
+        // weight fragments of the 5 tap pairs: lane (co = co0 + 16 j + r16, group g) holds taps 2p + (g >> 1)
+        bf16x8 wf[5][NT];
+#pragma unroll
+        for (int pr = 0; pr < 5; ++pr) {
+            const int tap = 2 * pr + (g >> 1);
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                wf[pr][j] = (tap < 9)
+                    ? *(const bf16x8 *)(a.wt + ((long)(co0 + 16 * j + r16) * 9 + tap) * cin + cb + 8 * (g & 1))
+                    : (bf16x8){};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pr = 0; pr < 5; ++pr) {
+            const int tap = 2 * pr + (g >> 1);
+            const int dy = tap < 9 ? tap / 3 : 0, dx = tap < 9 ? tap % 3 : 0;
+#pragma unroll
+            for (int pt = 0; pt < 4; ++pt) {
+                const int ry = wave * 2 + (pt >> 1), cx = 16 * (pt & 1) + r16;
+                const bf16x8 xv = sX[((ry + dy) * HW2 + cx + dx) * 2 + (g & 1)];
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[pt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[pr][j], xv, acc[pt][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds channels co0 + 16 j + 4 g + (0..3) of pixel (y0 + ry, x0 + cx)
+    float bias[NT][4];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bias[j][i] = a.bias ? a.bias[co0 + 16 * j + 4 * g + i] : 0.f;
+    float hb[4];
+    s16x4 hw[1];
+    if (HEAD) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hb[i] = ca.head_b ? ca.head_b[4 * g + i] : 0.f;
+        // head A operand: row co2 = r16, k = ci = 4 g + j
+        hw[0] = *(const s16x4 *)(ca.head_w + r16 * 16 + 4 * g);
+    }
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    float st[4][NT][4];     // stored (bf16-rounded) values, for the pool
+#pragma unroll
+    for (int pt = 0; pt < 4; ++pt) {
+        const int ry = wave * 2 + (pt >> 1), cx = 16 * (pt & 1) + r16;
+        const int yy = y0 + ry, xx = x0 + cx;
+        const bool ok = yy < a.h && xx < a.w;
+        const long p = ((long)img * a.h + yy) * a.w + xx;
+        __bf16 ob[NT][4];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[pt][j][i] + bias[j][i];
+                if (a.act == 1) v = v > 0.f ? v : 0.f;
+                else if (a.act == 2) v = 1.f / (1.f + __expf(-v));
+                ob[j][i] = f2bf(v);
+                st[pt][j][i] = bf2f(ob[j][i]);
+            }
+            if (ok && a.out1)
+                *(bf16x4 *)(a.out1 + p * a.cout + co0 + 16 * j + 4 * g) =
+                    (bf16x4){ob[j][0], ob[j][1], ob[j][2], ob[j][3]};
+        }
+        if (HEAD) {
+            s16x4 xb;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xb[i] = (short)__builtin_bit_cast(unsigned short, ob[0][i]);
+            f32x4 h = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hw[0], xb, (f32x4){}, 0, 0, 0);
+            float4 o;
+            o.x = 1.f / (1.f + __expf(-(h[0] + hb[0])));
+            o.y = 1.f / (1.f + __expf(-(h[1] + hb[1])));
+            o.z = 1.f / (1.f + __expf(-(h[2] + hb[2])));
+            o.w = 1.f / (1.f + __expf(-(h[3] + hb[3])));
+            if (ok) *(float4 *)(a.outf + p * 16 + 4 * g) = o;
+        }
+    }
+    if (POOL) {
+        // 2x2 windows: rows ry (pt 0/1) and ry + 1 (pt 2/3) of this lane, columns x and x + 1 (lane ^ 1)
+#pragma unroll
+        for (int pc = 0; pc < 2; ++pc) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                float nb_t[4], nb_b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    nb_t[i] = __shfl_xor(st[pc][j][i], 1);
+                    nb_b[i] = __shfl_xor(st[pc + 2][j][i], 1);
+                }
+                const int cx = 16 * pc + r16;
+                const int yo = (y0 + wave * 2) >> 1, xo = (x0 + cx) >> 1;
+                if ((r16 & 1) == 0 && yo < (a.h >> 1) && xo < (a.w >> 1)) {
+                    __bf16 pb[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        pb[i] = f2bf(0.25f * ((st[pc][j][i] + nb_t[i]) + (st[pc + 2][j][i] + nb_b[i])));
+                    const long q = ((long)img * (a.h >> 1) + yo) * (a.w >> 1) + xo;
+                    *(bf16x4 *)(ca.pool_out + q * a.cout + co0 + 16 * j + 4 * g) = (bf16x4){pb[0], pb[1], pb[2], pb[3]};
+                }
+            }
+        }
+    }
+}
+
 // weight gradient partials: ws[chunk][co][tap][ci] = sum over the chunk's pixels of
 // dz[p][co] * x[p + tap offset][ci]; one wave per (co tile, ci tile, tap)
 struct WgradArgs {
@@ -554,6 +716,55 @@ __global__ void __launch_bounds__(256) k_scatter_cores(const float *__restrict__
         out[(((long)img * hw + c0 + yy) * hw + c0 + xx) * cs + ch];
 }
 
+// gate: mean absolute difference between the network output and the exact fields over the sampled
+// pixels of each window (pixels pix[seg[w] .. seg[w + 1]) of window w, per pixel the mean over the nl
+// fields), one block per window with a fixed-order reduction (deterministic)
+__global__ void __launch_bounds__(256) k_gate_error(const float *__restrict__ out, const float *__restrict__ fields,
+                                                     const int2 *__restrict__ org, const int32_t *__restrict__ pix,
+                                                     const int32_t *__restrict__ seg, const int32_t *__restrict__ fm,
+                                                     float *__restrict__ werr, int hw, int cs, int nl, int rows,
+                                                     int cols) {
+    __shared__ float red[256];
+    const int w = blockIdx.x;
+    const int s0 = seg[w], s1 = seg[w + 1];
+    const long n = (long)rows * cols;
+    float acc = 0.f;
+    for (int i = s0 + (int)threadIdx.x; i < s1; i += 256) {
+        const int p = pix[i];
+        const int r = p / cols, c = p - r * cols;
+        const long o = (((long)w * hw + (r - org[w].x)) * hw + (c - org[w].y)) * cs;
+        float e = 0.f;
+        for (int k = 0; k < nl; ++k) e += fabsf(out[o + k] - fields[(long)fm[k] * n + p]);
+        acc += e / (float)nl;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) werr[w] = s1 > s0 ? red[0] / (float)(s1 - s0) : 0.f;
+}
+
+// gate: network output cores into the field planes except at sampled pixels (which keep the exact
+// values), plus SVFtotal = svf - (1 - svfveg) * 0.97 from the written fields 0 and 5
+__global__ void __launch_bounds__(256) k_gate_write(const float *__restrict__ out, const int2 *__restrict__ org,
+                                                     const int32_t *__restrict__ fm, const uint8_t *__restrict__ sampled,
+                                                     float *__restrict__ fields, int nwin, int hw, int cs, int core,
+                                                     int c0, int nl, int rows, int cols) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)nwin * core * core;
+    if (i >= total) return;
+    const int xx = (int)(i % core), yy = (int)((i / core) % core), img = (int)(i / ((long)core * core));
+    const int r = org[img].x + c0 + yy, c = org[img].y + c0 + xx;
+    if ((unsigned)r >= (unsigned)rows || (unsigned)c >= (unsigned)cols) return;
+    const long n = (long)rows * cols, p = (long)r * cols + c;
+    if (sampled[p]) return;
+    const float *o = out + (((long)img * hw + c0 + yy) * hw + c0 + xx) * cs;
+    for (int k = 0; k < nl; ++k) fields[(long)fm[k] * n + p] = o[k];
+    fields[15 * n + p] = o[0] - (1.f - o[5]) * 0.97f;
+}
+
 inline int blocks(long n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -573,6 +784,38 @@ static int launch_conv(const sw_cnn_op_t &o, const __bf16 *w, void *stream) {
     a.c1 = o.c1; a.c2 = o.c2; a.up1 = o.up; a.o1 = o.o1; a.cout = o.cout;
     a.n = o.n; a.h = o.h; a.w = o.w; a.act = o.act;
     const long M = (long)o.n * o.h * o.w;
+    const bool fused = o.pool_out || o.head_w;
+    const long tiles8x32 = (long)o.n * ((o.h + 7) / 8) * ((o.w + 31) / 32);
+    // the channel-row kernel stages its input once per 32 output channels: it wins for 16 output
+    // channels, for shallow inputs and for grids the 64-channel tiles of k_conv_d leave underfilled
+    // (under 512 blocks); deep inputs with many output channels keep k_conv_d (measured per layer)
+    const bool rows16 = o.cout == 16 || fused || o.c1 + o.c2 <= 32 || tiles8x32 * ((o.cout + 63) / 64) < 512;
+    if (o.ks == 3 && rows16 && o.cout % 16 == 0 && o.o1 >= o.cout && !o.outf == !o.head_w &&
+        (!o.head_w || o.cout == 16)) {
+        // channels as MFMA rows: 8 x 32 pixel tiles x 16 (cout 16) or 32 output channels per block on
+        // 16x16x32 MFMAs, optional pool / head epilogues
+        Conv16Args ca;
+        ca.c = a;
+        ca.pool_out = (__bf16 *)o.pool_out;
+        ca.head_w = (const __bf16 *)o.head_w;
+        ca.head_b = o.head_b;
+        const int nt = o.cout % 32 == 0 ? 2 : 1;
+        const dim3 g((unsigned)tiles8x32, (unsigned)(o.cout / (16 * nt)));
+        if (nt == 2) {
+            if (o.pool_out) hipLaunchKernelGGL((k_conv_n16<2, true, false>), g, dim3(256), 0, S(stream), ca);
+            else hipLaunchKernelGGL((k_conv_n16<2, false, false>), g, dim3(256), 0, S(stream), ca);
+        } else if (o.pool_out && o.head_w) {
+            hipLaunchKernelGGL((k_conv_n16<1, true, true>), g, dim3(256), 0, S(stream), ca);
+        } else if (o.pool_out) {
+            hipLaunchKernelGGL((k_conv_n16<1, true, false>), g, dim3(256), 0, S(stream), ca);
+        } else if (o.head_w) {
+            hipLaunchKernelGGL((k_conv_n16<1, false, true>), g, dim3(256), 0, S(stream), ca);
+        } else {
+            hipLaunchKernelGGL((k_conv_n16<1, false, false>), g, dim3(256), 0, S(stream), ca);
+        }
+        return hip_check(hipGetLastError(), "k_conv_n16");
+    }
+    if (fused) return set_err(SW_E_ARG, "sw_cnn: pool / head epilogues need a 3x3 conv without a split output");
     if (o.ks == 3) {
         // direct convolution: 8 x 32 pixel tiles x 32 * WN channels
         const int wn = o.cout > 32 ? 2 : 1;
@@ -604,6 +847,8 @@ static int check_conv(const sw_cnn_op_t &o, const char *who) {
         return set_err(SW_E_ARG, who);
     if (o.up && (o.h % 2 || o.w % 2)) return set_err(SW_E_ARG, who);
     if (!o.in0 || (o.c2 && !o.in1) || !o.weight) return set_err(SW_E_ARG, who);
+    if (o.pool_out && (o.h % 2 || o.w % 2)) return set_err(SW_E_ARG, who);
+    if (o.head_w && !o.outf) return set_err(SW_E_ARG, who);
     return SW_OK;
 }
 
@@ -764,4 +1009,27 @@ extern "C" int sw_cnn_scatter_cores(const float *out, const int32_t *origins, co
     hipLaunchKernelGGL(k_scatter_cores, dim3(blocks(total, 256)), dim3(256), 0, S(stream), out,
                        (const int2 *)origins, field_map, fields, n, hw, cs, core, c0, nl, rows, cols);
     return hip_check(hipGetLastError(), "k_scatter_cores");
+}
+
+extern "C" int sw_gate_error(const float *out, const float *fields, const int32_t *org, const int32_t *pix, const int32_t *seg,
+                  const int32_t *fm, float *werr, int nwin, int hw, int cs, int nl, int rows, int cols, void *stream) {
+    if (!out || !fields || !org || !pix || !seg || !fm || !werr) return set_err(SW_E_ARG, "sw_gate_error: NULL pointer");
+    if (nwin < 0 || hw <= 0 || cs <= 0 || nl <= 0 || nl > cs || rows <= 0 || cols <= 0)
+        return set_err(SW_E_ARG, "sw_gate_error: bad sizes");
+    if (nwin == 0) return SW_OK;
+    hipLaunchKernelGGL(k_gate_error, dim3(nwin), dim3(256), 0, S(stream), out, fields, (const int2 *)org, pix, seg, fm,
+                       werr, hw, cs, nl, rows, cols);
+    return hip_check(hipGetLastError(), "k_gate_error");
+}
+
+extern "C" int sw_gate_write(const float *out, const int32_t *org, const int32_t *fm, const uint8_t *sampled, float *fields,
+                  int nwin, int hw, int cs, int core, int c0, int nl, int rows, int cols, void *stream) {
+    if (!out || !org || !fm || !sampled || !fields) return set_err(SW_E_ARG, "sw_gate_write: NULL pointer");
+    if (nwin < 0 || core <= 0 || c0 < 0 || c0 + core > hw || nl < 6 || nl > cs || rows <= 0 || cols <= 0)
+        return set_err(SW_E_ARG, "sw_gate_write: bad sizes");
+    if (nwin == 0) return SW_OK;
+    const long total = (long)nwin * core * core;
+    hipLaunchKernelGGL(k_gate_write, dim3(blocks(total, 256)), dim3(256), 0, S(stream), out, (const int2 *)org, fm,
+                       sampled, fields, nwin, hw, cs, core, c0, nl, rows, cols);
+    return hip_check(hipGetLastError(), "k_gate_write");
 }

@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256)
 k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
             const int32_t *__restrict__ counts, uint32_t *__restrict__ masks,
             const uint32_t *__restrict__ stats, int rows, int cols, const int32_t *__restrict__ tiles,
-            int ntiles) {
+            int ntiles, int split) {
     const long n = (long)rows * cols;
     const float hmax = funkey(stats[1]);
     const bool checked = stats[3] != 0u;
@@ -417,11 +417,14 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         at = h.x;
         veg2t = h.z;
     }
-    const int w = blockIdx.y;
+    // split 1: one block per mask word (32 patches); split 4: one block per byte of a word (8 patches,
+    // short tile lists: four times the blocks, a quarter of the latency)
+    const int w = blockIdx.y / split, sub = blockIdx.y - w * split;
     const int p0 = w * 32;
-    const int p1 = min(p0 + 32, SW_NPATCH);
+    const int pb = p0 + sub * (32 / split);
+    const int p1 = min(pb + 32 / split, SW_NPATCH);
     uint32_t bsh = 0, bvs = 0, bv1 = 0, bv2 = 0;
-    for (int p = p0; p < p1; ++p) {
+    for (int p = pb; p < p1; ++p) {
         // the zero-height pad stands for the reference's zero source only when
         // dz >= 0; float32 tan(90 deg) makes the zenith patch's dz negative
         const bool chk = checked || (counts[p] > 0 && __int_as_float(tab[offs[p]].z) < 0.f);
@@ -435,11 +438,18 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         if (vbv == 1u) bv1 |= bit;
         if (vbv == 2u) bv2 |= bit;
     }
-    if (valid) {
+    if (valid && split == 1) {
         masks[(long)(0 * SW_MASK_WORDS + w) * n + t] = bsh;
         masks[(long)(1 * SW_MASK_WORDS + w) * n + t] = bvs;
         masks[(long)(2 * SW_MASK_WORDS + w) * n + t] = bv1;
         masks[(long)(3 * SW_MASK_WORDS + w) * n + t] = bv2;
+    } else if (valid) {
+        // this block's byte of each word (little-endian: bits 8 sub .. 8 sub + 7)
+        uint8_t *mb = (uint8_t *)masks;
+        mb[((long)(0 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bsh >> (8 * sub));
+        mb[((long)(1 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bvs >> (8 * sub));
+        mb[((long)(2 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bv1 >> (8 * sub));
+        mb[((long)(3 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bv2 >> (8 * sub));
     }
 }
 
@@ -1569,7 +1579,7 @@ int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int
     const long tiles = (long)((rows + SVF_TILE - 1) / SVF_TILE) * ((cols + SVF_TILE - 1) / SVF_TILE);
     dim3 grid((unsigned)(((tiles + 7) / 8) * 8), SW_MASK_WORDS);
     hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float *)hpad, offs, (const int4 *)tab,
-                       counts, masks, stats, rows, cols, (const int32_t *)nullptr, 0);
+                       counts, masks, stats, rows, cols, (const int32_t *)nullptr, 0, 1);
     return hip_check(hipGetLastError(), "k_svf_march");
 }
 
@@ -1595,8 +1605,10 @@ int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, 
     rc = hip_check(hipGetLastError(), "k_pad_heights");
     if (rc) return rc;
     const unsigned g = (unsigned)(((ntiles + 7) / 8) * 8);
-    hipLaunchKernelGGL(k_svf_march, dim3(g, SW_MASK_WORDS), dim3(256), 0, S(stream), (const float *)hpad, offs,
-                       (const int4 *)tab, counts, masks, stats, rows, cols, tiles, ntiles);
+    // short lists (under 8 blocks per CU at one word per block) split each word over four blocks
+    const int split = (long)g * SW_MASK_WORDS < 2048 ? 4 : 1;
+    hipLaunchKernelGGL(k_svf_march, dim3(g, SW_MASK_WORDS * split), dim3(256), 0, S(stream), (const float *)hpad,
+                       offs, (const int4 *)tab, counts, masks, stats, rows, cols, tiles, ntiles, split);
     rc = hip_check(hipGetLastError(), "k_svf_march");
     if (rc) return rc;
     hipLaunchKernelGGL(k_svf_fold_tiles, dim3(g), dim3(256), 0, S(stream), masks, vegdsm2, w_iso, w_aniso, ring,

@@ -42,6 +42,26 @@ def layer_shapes(ch):
             "d1a": (c1, 3, c2 + c1), "d1": (c1, 3, c1), "head": (COUT_HEAD, 1, c1)}
 
 
+#: resolution level of each layer (1 = full window, 4 = 1/8)
+LAYER_LEVEL = {"e1a": 1, "e1": 1, "e2a": 2, "e2": 2, "e3a": 3, "e3": 3, "e4a": 4, "e4": 4,
+               "d3a": 3, "d3": 3, "d2a": 2, "d2": 2, "d1a": 1, "d1": 1, "head": 1}
+
+
+def macs_per_window(ch, hw: int = HW, padded: bool = False) -> int:
+    """Multiply-accumulates of one forward pass over an hw x hw window. ``padded`` counts the
+    zero-padded input and head channels the kernels compute; otherwise the 3 input and 10 output
+    channels of the model."""
+    total = 0
+    for name, (co, ks, ci) in layer_shapes(ch).items():
+        if not padded and name == "e1a":
+            ci = 3
+        if not padded and name == "head":
+            co = N_FIELDS
+        side = hw >> (LAYER_LEVEL[name] - 1)
+        total += side * side * co * ks * ks * ci
+    return total
+
+
 class SurrogateNet:
     """Parameters (f32 master + bf16 copy), gradients and Adam state in flat device buffers."""
 
@@ -134,10 +154,14 @@ class Passes:
     """Activation/gradient buffers and the forward and backward op lists for one batch shape."""
 
     def __init__(self, net: SurrogateNet, batch: int, hw: int = HW, core: int = CORE, c0: int = HALO,
-                 nchunk: int = 64):
+                 nchunk: int = 64, train: bool = True, fuse: bool | None = None):
         if hw % 8:
             raise ValueError("window size must be a multiple of 8")
-        self.net, self.n, self.hw, self.core, self.c0 = net, batch, hw, core, c0
+        self.net, self.n, self.hw, self.core, self.c0, self.train = net, batch, hw, core, c0, train
+        # with 16 first-level channels the pools and the head run in the convolutions' epilogues
+        self.fuse = (net.channels[0] == 16) if fuse is None else bool(fuse)
+        if self.fuse and net.channels[0] != 16:
+            raise ValueError("the fused pool / head epilogues need 16 first-level channels")
         c1, c2, c3, c4 = net.channels
         dev = net.device
         bf = torch.bfloat16
@@ -152,30 +176,36 @@ class Passes:
         self.x = t(1, CIN0)
         for name, lv in lvl.items():
             co = net.shapes[name][0]
-            self.a[name] = t(lv, co)
-            self.d[name] = t(lv, co)
+            if train or not (self.fuse and name == "d1"):
+                self.a[name] = t(lv, co)
+            if train:
+                self.d[name] = t(lv, co)
         for k, lv, c in (("p1", 2, c1), ("p2", 3, c2), ("p3", 4, c3)):
             self.a[k] = t(lv, c)
-            self.d[k] = t(lv, c)
-        # data gradients of the concat layers: upsampled part (full res) and skip part
-        self.d["up4"], self.d["sk3"] = t(3, c4), t(3, c3)
-        self.d["up3"], self.d["sk2"] = t(2, c3), t(2, c2)
-        self.d["up2"], self.d["sk1"] = t(1, c2), t(1, c1)
+            if train:
+                self.d[k] = t(lv, c)
+        if train:
+            # data gradients of the concat layers: upsampled part (full res) and skip part
+            self.d["up4"], self.d["sk3"] = t(3, c4), t(3, c3)
+            self.d["up3"], self.d["sk2"] = t(2, c3), t(2, c2)
+            self.d["up2"], self.d["sk1"] = t(1, c2), t(1, c1)
         self.gz = {}                 # ReLU-gradient scratch per (level, channels), reused layer to layer
         self.out = torch.zeros((batch, hw, hw, COUT_HEAD), dtype=torch.float32, device=dev)
-        self.dhead = t(1, COUT_HEAD)
         self.labels = torch.zeros((batch, core, core, N_FIELDS), dtype=torch.float32, device=dev)
-        nparts = -(-(batch * hw * hw * COUT_HEAD) // 256)
-        self.parts = torch.zeros(nparts, dtype=torch.float32, device=dev)
         self.dims, self.lvl, self.nchunk = dims, lvl, nchunk
         self.fwd_ops = self._forward_ops()
-        self.bwd_ops, ws = self._backward_ops()
+        self.bwd_ops, ws = [], 0
+        if train:
+            self.dhead = t(1, COUT_HEAD)
+            nparts = -(-(batch * hw * hw * COUT_HEAD) // 256)
+            self.parts = torch.zeros(nparts, dtype=torch.float32, device=dev)
+            self.bwd_ops, ws = self._backward_ops()
         self.ws = torch.zeros(max(ws // 4, 1), dtype=torch.float32, device=dev)
         for op in self.bwd_ops:
             if op.kind in (L.CNN_CONV_BWD_DATA, L.CNN_CONV_BWD_WEIGHT):
                 op.ws = L.ptr(self.ws)
         self.fwd_arr = (L.CnnOp * len(self.fwd_ops))(*self.fwd_ops)
-        self.bwd_arr = (L.CnnOp * len(self.bwd_ops))(*self.bwd_ops)
+        self.bwd_arr = (L.CnnOp * max(len(self.bwd_ops), 1))(*self.bwd_ops)
 
     # -- op builders -----------------------------------------------------------
     def _conv(self, name, in0, in1=None, up=False, out=None, act=1, outf=None):
@@ -199,9 +229,12 @@ class Passes:
         ops.append(self._conv("e1", a["e1a"], out=a["e1"]))
         for lv, (e, p) in enumerate((("e1", "p1"), ("e2", "p2"), ("e3", "p3")), start=1):
             s = self.dims[lv]
-            o = L.CnnOp(kind=L.CNN_POOL2, n=self.n, h=s, w=s, c1=a[e].shape[3])
-            o.in0, o.out0 = L.ptr(a[e]), L.ptr(a[p])
-            ops.append(o)
+            if self.fuse:
+                ops[-1].pool_out = L.ptr(a[p])          # the pool runs in the epilogue of the conv producing e
+            else:
+                o = L.CnnOp(kind=L.CNN_POOL2, n=self.n, h=s, w=s, c1=a[e].shape[3])
+                o.in0, o.out0 = L.ptr(a[e]), L.ptr(a[p])
+                ops.append(o)
             nxt = ("e2a", "e2") if lv == 1 else (("e3a", "e3") if lv == 2 else ("e4a", "e4"))
             ops.append(self._conv(nxt[0], a[p], out=a[nxt[0]]))
             ops.append(self._conv(nxt[1], a[nxt[0]], out=a[nxt[1]]))
@@ -210,8 +243,13 @@ class Passes:
         ops.append(self._conv("d2a", a["d3"], a["e2"], up=True, out=a["d2a"]))
         ops.append(self._conv("d2", a["d2a"], out=a["d2"]))
         ops.append(self._conv("d1a", a["d2"], a["e1"], up=True, out=a["d1a"]))
-        ops.append(self._conv("d1", a["d1a"], out=a["d1"]))
-        ops.append(self._conv("head", a["d1"], act=2, outf=self.out))
+        if self.fuse:
+            o = self._conv("d1", a["d1a"], out=a.get("d1"), outf=self.out)
+            o.head_w, o.head_b = L.ptr(self.net.w("head")), L.ptr(self.net.b("head"))
+            ops.append(o)
+        else:
+            ops.append(self._conv("d1", a["d1a"], out=a["d1"]))
+            ops.append(self._conv("head", a["d1"], act=2, outf=self.out))
         return ops
 
     def _backward_ops(self):
@@ -302,6 +340,8 @@ class Passes:
         return self.out
 
     def backward(self, stream=None):
+        if not self.train:
+            raise RuntimeError("an inference Passes has no backward pass")
         L.call("sw_cnn_bwd", self.bwd_arr, len(self.bwd_ops), _sp(stream))
 
     def loss(self) -> float:
@@ -430,6 +470,7 @@ class SvfSurrogate:
         self.net, self.tol, self.samples, self.seed = net, tol, samples, seed
         self._geo = {}
         self.ps = None
+        self._side = None
 
     def _geometry(self, rows, cols):
         key = (rows, cols)
@@ -461,6 +502,8 @@ class SvfSurrogate:
                 pw.append(np.full(idx.size, w, np.int64))
             pix = np.concatenate(pix)
             pw = np.concatenate(pw)
+            seg = np.zeros(len(org) + 1, np.int32)
+            np.cumsum(np.bincount(pw, minlength=len(org)), out=seg[1:])
             win = np.zeros((rows, cols), np.int64)
             for w, (r0, c0) in enumerate(org + HALO):
                 win[r0:r0 + CORE, c0:c0 + CORE] = w
@@ -469,7 +512,9 @@ class SvfSurrogate:
             g = dict(org=org, grid=grid, tiles=tiles, d_tiles=torch.from_numpy(tiles).to(dev),
                      pix=torch.from_numpy(pix).to(dev), pw=torch.from_numpy(pw).to(dev),
                      npix=torch.bincount(torch.from_numpy(pw), minlength=len(org)).to(dev).float(),
-                     win=torch.from_numpy(win.ravel()).to(dev), sampled=torch.from_numpy(sampled).to(dev))
+                     win=torch.from_numpy(win.ravel()).to(dev), sampled=torch.from_numpy(sampled).to(dev),
+                     pix32=torch.from_numpy(pix.astype(np.int32)).to(dev), seg=torch.from_numpy(seg).to(dev),
+                     d_org=torch.from_numpy(np.ascontiguousarray(org, np.int32)).to(dev))
             self._geo[key] = g
         return g
 
@@ -478,7 +523,7 @@ class SvfSurrogate:
         g = self._geometry(ds.rows, ds.cols)
         org = g["org"]
         if self.ps is None or self.ps.n != len(org):
-            self.ps = Passes(self.net, len(org))
+            self.ps = Passes(self.net, len(org), train=False)
         site = SiteField(ds.hpack, fields, ds.rows, ds.cols)
         d_org = load_batch(self.ps, site, org, with_labels=False, stream=stream)
         self.ps.forward(stream)
@@ -493,23 +538,36 @@ class SvfSurrogate:
         t0 = time.perf_counter()
         rows, cols = ds.rows, ds.cols
         g = self._geometry(rows, cols)
-        n = rows * cols
-        surr = torch.empty_like(ds.fields)
+        org = g["org"]
+        dev = self.net.device
+        main = stream if stream is not None else torch.cuda.current_stream(dev)
+        if self._side is None:
+            self._side = torch.cuda.Stream(dev)
+        if self.ps is None or self.ps.n != len(org):
+            self.ps = Passes(self.net, len(org), train=False)
+        ps = self.ps
+        solver.svf_tables(rows, cols, ds.tile.scale)          # built and uploaded on main before the fork
+        # the exact sweep of the sampled tiles runs beside the network (different pixels and buffers)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        self._side.wait_event(ready)
+        self._exact_tiles(solver, ds, g["d_tiles"], len(g["tiles"]), self._side)
+        sampled_done = torch.cuda.Event()
+        sampled_done.record(self._side)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record()
-        org = self.predict(ds, surr, stream)
-        ev[1].record()
-        self._exact_tiles(solver, ds, g["d_tiles"], len(g["tiles"]), stream)
-        f = ds.fields.view(L.NFIELDS, n)
-        sv = surr.view(L.NFIELDS, n)
-        e = (sv[:N_FIELDS, g["pix"]] - f[:N_FIELDS, g["pix"]]).abs().mean(dim=0)
-        werr = torch.zeros(len(org), device=f.device).index_add_(0, g["pw"], e) / g["npix"]
-        acc = werr <= self.tol
-        keep = acc[g["win"]] & ~g["sampled"]
-        sv[15] = sv[0] - (1.0 - sv[5]) * (1.0 - 0.03)
-        for k in list(range(N_FIELDS)) + [15]:
-            f[k] = torch.where(keep, sv[k], f[k])
-        werr_h = werr.cpu().numpy()
+        ev[0].record(main)
+        sp = _sp(main)
+        L.call("sw_cnn_pack_input", L.ptr(ds.hpack), L.ptr(g["d_org"]), L.ptr(ps.x), len(org), ps.hw, rows, cols, sp)
+        ps.forward(main)
+        ev[1].record(main)
+        main.wait_event(sampled_done)
+        fm = _field_map(dev)
+        werr = torch.empty(len(org), dtype=torch.float32, device=dev)
+        L.call("sw_gate_error", L.ptr(ps.out), L.ptr(ds.fields), L.ptr(g["d_org"]), L.ptr(g["pix32"]),
+               L.ptr(g["seg"]), L.ptr(fm), L.ptr(werr), len(org), ps.hw, COUT_HEAD, N_FIELDS, rows, cols, sp)
+        L.call("sw_gate_write", L.ptr(ps.out), L.ptr(g["d_org"]), L.ptr(fm), L.ptr(g["sampled"]), L.ptr(ds.fields),
+               len(org), ps.hw, COUT_HEAD, ps.core, ps.c0, N_FIELDS, rows, cols, sp)
+        werr_h = werr.cpu().numpy() if stream is None else _to_host(werr, main)
         bad = np.nonzero(werr_h > self.tol)[0]
         ex = 0
         if len(bad):
@@ -519,8 +577,8 @@ class SvfSurrogate:
                 m[r0 // 16:min(g["grid"].ty, (r0 + CORE) // 16), c0 // 16:min(g["grid"].tx, (c0 + CORE) // 16)] = True
             ids = TileGrid.ids(m)
             ex = len(ids)
-            self._exact_tiles(solver, ds, torch.from_numpy(ids).to(f.device), ex, stream)
-        torch.cuda.synchronize()
+            self._exact_tiles(solver, ds, torch.from_numpy(ids).to(dev), ex, main)
+        main.synchronize()
         return GateReport(windows=len(org), accepted=len(org) - len(bad), sampled_tiles=len(g["tiles"]),
                           exact_tiles=ex, ms_total=(time.perf_counter() - t0) * 1e3,
                           ms_network=ev[0].elapsed_time(ev[1]), window_errors=werr_h)
@@ -535,6 +593,12 @@ class SvfSurrogate:
         L.call("sw_svf_sweep_tiles", P(ds.hpack), P(ds.stats), P(offs), P(tab), P(ds.vegdsm2), P(c["w_iso"]),
                P(c["w_aniso"]), P(c["ring"]), P(c["nann"]), P(c["dirm"]), P(ds.counts), P(ds.hpad), P(ds.masks),
                P(ds.fields), P(d_ids), int(count), ds.rows, ds.cols, _sp(stream))
+
+
+def _to_host(t, stream):
+    """A device tensor's values on the host after ``stream`` reaches this point."""
+    stream.synchronize()
+    return t.cpu().numpy()
 
 
 DEFAULT_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "svf_surrogate_16_32_64_128.npz"

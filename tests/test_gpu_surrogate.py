@@ -67,6 +67,31 @@ def test_forward_and_gradients_match_oracle(S):
             assert rel <= 3e-2, (name, rel)
 
 
+def test_fused_epilogues_match_separate_ops(S):
+    """The pools fused into e1 / e2 / e3 equal separate pools of the same outputs bit for bit; the head fused
+    into d1 equals the separate 1x1 head within f32 accumulation order; inference mode (no d1 store)
+    gives the same head output."""
+    _say("surrogate fused pool / head vs separate ops")
+    net = S.SurrogateNet(channels=(16, 32, 32, 48), seed=7)
+    rng = np.random.default_rng(7)
+    x = np.zeros((2, 64, 64, 16), np.float32)
+    x[..., :3] = rng.uniform(0, 1, (2, 64, 64, 3))
+    xb = torch.from_numpy(x).to(torch.bfloat16).cuda()
+    runs = {}
+    for key, kw in (("fused", dict(fuse=True)), ("separate", dict(fuse=False)), ("infer", dict(train=False))):
+        ps = S.Passes(net, batch=2, hw=64, core=32, c0=16, nchunk=4, **kw)
+        ps.x.copy_(xb)
+        ps.forward()
+        torch.cuda.synchronize()
+        runs[key] = ps
+    f, u, i = runs["fused"], runs["separate"], runs["infer"]
+    for name in ("e1a", "e1", "p1", "e2", "p2", "e3", "p3", "e4", "d2", "d1a", "d1"):
+        assert torch.equal(f.a[name], u.a[name]), name
+    assert f.fuse and not u.fuse and i.fuse and "d1" not in i.a
+    assert (f.out - u.out).abs().max().item() <= 2e-6
+    assert torch.equal(f.out, i.out)
+
+
 def test_adam_step_matches_formula(S):
     net, ps, _ = _small(S, seed=2)
     ps.forward()
