@@ -381,13 +381,15 @@ def run_surrogate(args):
     untrained = SG.SurrogateNet(ch, dev, seed=args.seed + 7)
     rep_u, wall_u, err_u = gated(untrained)
     exact_ms = float(np.median(ex_ms))
-    # MFMA roofline of the network: 2 x MACs per 384^2 window (model channels) x windows / network time
-    flops = 2.0 * SG.macs_per_window(ch) * rep.windows
+    # MFMA roofline of the network: 2 x MACs per 384^2 window as inference runs it (model channels; the
+    # last two layers over the core) x windows / network time
+    fpw = 2 * SG.macs_per_window(ch, core=SG.CORE)
+    flops = float(fpw) * rep.windows
     ach = flops / (rep.ms_network * 1e-3) / 1e12
     roof = {"kernel": "sw_cnn_fwd (window pack + 15 conv layers + scatter)", "bound": "mfma",
             "achieved": round(ach, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(ach / BF16_PEAK_TFLOPS, 4), "traffic": None,
-            "flop_per_window": 2 * SG.macs_per_window(ch), "windows": rep.windows,
+            "flop_per_window": fpw, "windows": rep.windows,
             "network_ms": round(rep.ms_network, 3)}
     line = {
         "metric": "surrogate vs exact SVF (speedup of the gated surrogate path, 2000² city, 10 SVF fields)",

@@ -387,7 +387,9 @@ typedef struct sw_cnn_op {
     int32_t n, h, w;              /* batch and full-resolution size of the op */
     int32_t c1, c2, up, o1;       /* input channels (in0 half resolution when up), output split */
     int32_t cout, ks, act, nchunk;/* act: 0 none, 1 ReLU, 2 sigmoid; nchunk: weight-gradient split */
-    int32_t core, c0, nl, pad;    /* head loss: core window [c0, c0 + core), nl labelled channels */
+    int32_t core, c0, nl, pad;    /* head loss: core window [c0, c0 + core), nl labelled channels;
+                                   * forward conv: core > 0 computes only the square [c0, c0 + core)^2
+                                   * of each image (inference; the rest of the output is untouched) */
     const void *in0, *in1;
     void *out0, *out1;
     float *outf;

@@ -45,6 +45,7 @@ struct ConvArgs {
     __bf16 *out2;          // output channels [o1, cout) (NHWC with cout - o1 channels), may be null
     float *outf;           // f32 output [pixel][cout] (head), may be null
     int c1, c2, up1, o1, cout, n, h, w, act;   // act: 0 none, 1 ReLU, 2 sigmoid
+    int r0, rn;            // output region [r0, r0 + rn)^2 of every image (k_conv_n16 only); rn 0: all
 };
 
 // Implicit-GEMM convolution, LDS tiled: a block computes BM = 32 * WAVES_M * WM pixels x
@@ -303,10 +304,13 @@ __global__ void __launch_bounds__(256) k_conv_n16(Conv16Args ca) {
     const ConvArgs &a = ca.c;
     __shared__ bf16x8 sX[HH * HW2 * 2];         // [halo cell][k half]
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6, g = l >> 4, r16 = l & 15;
-    const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+    // output rows / columns [ro, ro + rh) x [ro, ro + rw): the image, or the op's square region
+    const int ro = a.rn ? a.r0 : 0, rh = a.rn ? a.rn : a.h, rw = a.rn ? a.rn : a.w;
+    const int tiles_x = (rw + TW - 1) / TW, tiles_y = (rh + TH - 1) / TH;
     const int t = blockIdx.x;
     const int img = t / (tiles_x * tiles_y);
-    const int y0 = ((t / tiles_x) % tiles_y) * TH, x0 = (t % tiles_x) * TW;
+    const int y0 = ro + ((t / tiles_x) % tiles_y) * TH, x0 = ro + (t % tiles_x) * TW;
+    const int ylim = ro + rh, xlim = ro + rw;     // within the image (checked by the launcher)
     const int cin = a.c1 + a.c2;
     const int hh = a.h >> 1, wh = a.w >> 1;
     const int co0 = blockIdx.y * 16 * NT;         // first output channel of the block
@@ -380,7 +384,7 @@ __global__ void __launch_bounds__(256) k_conv_n16(Conv16Args ca) {
     for (int pt = 0; pt < 4; ++pt) {
         const int ry = wave * 2 + (pt >> 1), cx = 16 * (pt & 1) + r16;
         const int yy = y0 + ry, xx = x0 + cx;
-        const bool ok = yy < a.h && xx < a.w;
+        const bool ok = yy < ylim && xx < xlim;
         const long p = ((long)img * a.h + yy) * a.w + xx;
         __bf16 ob[NT][4];
 #pragma unroll
@@ -783,14 +787,20 @@ static int launch_conv(const sw_cnn_op_t &o, const __bf16 *w, void *stream) {
     a.outf = (float *)o.outf;
     a.c1 = o.c1; a.c2 = o.c2; a.up1 = o.up; a.o1 = o.o1; a.cout = o.cout;
     a.n = o.n; a.h = o.h; a.w = o.w; a.act = o.act;
+    a.r0 = o.kind == SW_CNN_CONV ? o.c0 : 0;
+    a.rn = o.kind == SW_CNN_CONV ? o.core : 0;
     const long M = (long)o.n * o.h * o.w;
     const bool fused = o.pool_out || o.head_w;
-    const long tiles8x32 = (long)o.n * ((o.h + 7) / 8) * ((o.w + 31) / 32);
+    const int th = a.rn ? a.rn : o.h, tw = a.rn ? a.rn : o.w;
+    const long tiles8x32 = (long)o.n * ((th + 7) / 8) * ((tw + 31) / 32);
     // the channel-row kernel stages its input once per 32 output channels: it wins for 16 output
     // channels, for shallow inputs and for grids the 64-channel tiles of k_conv_d leave underfilled
     // (under 512 blocks); deep inputs with many output channels keep k_conv_d (measured per layer)
     const bool rows16 = o.cout == 16 || fused || o.c1 + o.c2 <= 32 || tiles8x32 * ((o.cout + 63) / 64) < 512;
-    if (o.ks == 3 && rows16 && o.cout % 16 == 0 && o.o1 >= o.cout && !o.outf == !o.head_w &&
+    if (a.rn && (a.r0 < 0 || a.r0 + a.rn > o.h || a.r0 + a.rn > o.w || o.pool_out || o.ks != 3 ||
+                 o.cout % 16 || o.o1 < o.cout || (o.outf && !o.head_w)))
+        return set_err(SW_E_ARG, "sw_cnn: an output region needs a 3x3 conv with a whole output, no pool, inside the image");
+    if (o.ks == 3 && (rows16 || a.rn) && o.cout % 16 == 0 && o.o1 >= o.cout && !o.outf == !o.head_w &&
         (!o.head_w || o.cout == 16)) {
         // channels as MFMA rows: 8 x 32 pixel tiles x 16 (cout 16) or 32 output channels per block on
         // 16x16x32 MFMAs, optional pool / head epilogues

@@ -47,10 +47,11 @@ LAYER_LEVEL = {"e1a": 1, "e1": 1, "e2a": 2, "e2": 2, "e3a": 3, "e3": 3, "e4a": 4
                "d3a": 3, "d3": 3, "d2a": 2, "d2": 2, "d1a": 1, "d1": 1, "head": 1}
 
 
-def macs_per_window(ch, hw: int = HW, padded: bool = False) -> int:
+def macs_per_window(ch, hw: int = HW, padded: bool = False, core: int | None = None) -> int:
     """Multiply-accumulates of one forward pass over an hw x hw window. ``padded`` counts the
     zero-padded input and head channels the kernels compute; otherwise the 3 input and 10 output
-    channels of the model."""
+    channels of the model. ``core``: inference, where d1 and the head run over the core x core
+    square only and d1a over the core plus one pixel."""
     total = 0
     for name, (co, ks, ci) in layer_shapes(ch).items():
         if not padded and name == "e1a":
@@ -58,6 +59,10 @@ def macs_per_window(ch, hw: int = HW, padded: bool = False) -> int:
         if not padded and name == "head":
             co = N_FIELDS
         side = hw >> (LAYER_LEVEL[name] - 1)
+        if core is not None and name in ("d1", "head"):
+            side = core
+        elif core is not None and name == "d1a":
+            side = core + 2
         total += side * side * co * ks * ks * ci
     return total
 
@@ -246,6 +251,10 @@ class Passes:
         if self.fuse:
             o = self._conv("d1", a["d1a"], out=a.get("d1"), outf=self.out)
             o.head_w, o.head_b = L.ptr(self.net.w("head")), L.ptr(self.net.b("head"))
+            if not self.train and self.c0 >= 1 and self.c0 + self.core < self.hw:
+                # inference reads the core only: d1 (and the head) over the core, d1a one pixel beyond it
+                o.c0, o.core = self.c0, self.core
+                ops[-1].c0, ops[-1].core = self.c0 - 1, self.core + 2
             ops.append(o)
         else:
             ops.append(self._conv("d1", a["d1a"], out=a["d1"]))

@@ -69,8 +69,8 @@ def test_forward_and_gradients_match_oracle(S):
 
 def test_fused_epilogues_match_separate_ops(S):
     """The pools fused into e1 / e2 / e3 equal separate pools of the same outputs bit for bit; the head fused
-    into d1 equals the separate 1x1 head within f32 accumulation order; inference mode (no d1 store)
-    gives the same head output."""
+    into d1 equals the separate 1x1 head within f32 accumulation order; inference mode (no d1 store,
+    the last two layers over the core) gives the same head output there."""
     _say("surrogate fused pool / head vs separate ops")
     net = S.SurrogateNet(channels=(16, 32, 32, 48), seed=7)
     rng = np.random.default_rng(7)
@@ -89,7 +89,10 @@ def test_fused_epilogues_match_separate_ops(S):
         assert torch.equal(f.a[name], u.a[name]), name
     assert f.fuse and not u.fuse and i.fuse and "d1" not in i.a
     assert (f.out - u.out).abs().max().item() <= 2e-6
-    assert torch.equal(f.out, i.out)
+    # inference computes d1 and the head over the core only (d1a one pixel beyond it)
+    core = slice(16, 48)
+    assert torch.equal(f.out[:, core, core], i.out[:, core, core])
+    assert torch.equal(f.a["d1a"][:, 15:49, 15:49], i.a["d1a"][:, 15:49, 15:49])
 
 
 def test_adam_step_matches_formula(S):
