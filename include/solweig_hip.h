@@ -404,6 +404,13 @@ typedef struct sw_cnn_op {
     void *pool_out;
     const void *head_w;
     const float *head_b;
+    /* forward 3x3 conv with c1 = 16: when site is set, in0 is unused and the input is the window at
+     * site_origins[img] (int32 row, col) of the site's composed heights (float4 {a, vegdsm, vegdsm2,
+     * -} per cell, site_rows x site_cols): channels 0-2 = height * 0.02 in bf16, the rest and cells
+     * outside the site zero (the values sw_cnn_pack_input writes) */
+    const void *site;
+    const int32_t *site_origins;
+    int32_t site_rows, site_cols;
 } sw_cnn_op_t;
 
 int sw_cnn_fwd(const sw_cnn_op_t *ops, int nops, void *stream);

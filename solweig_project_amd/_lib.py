@@ -106,7 +106,8 @@ class CnnOp(ctypes.Structure):
     _fields_ = [(k, _c_i32) for k in ("kind", "n", "h", "w", "c1", "c2", "up", "o1", "cout", "ks", "act", "nchunk",
                                       "core", "c0", "nl", "pad")] + \
         [(k, ctypes.c_void_p) for k in ("in0", "in1", "out0", "out1", "outf", "weight", "bias", "dz", "gw", "gb",
-                                        "ws", "pool_out", "head_w", "head_b")]
+                                        "ws", "pool_out", "head_w", "head_b", "site", "site_origins")] + \
+        [(k, _c_i32) for k in ("site_rows", "site_cols")]
 
 
 CNN_CONV, CNN_CONV_BWD_DATA, CNN_CONV_BWD_WEIGHT, CNN_POOL2, CNN_POOL2_BWD, CNN_UP2_BWD, CNN_RELU_BWD, \

@@ -46,6 +46,9 @@ struct ConvArgs {
     float *outf;           // f32 output [pixel][cout] (head), may be null
     int c1, c2, up1, o1, cout, n, h, w, act;   // act: 0 none, 1 ReLU, 2 sigmoid
     int r0, rn;            // output region [r0, r0 + rn)^2 of every image (k_conv_n16 only); rn 0: all
+    const float4 *site;    // k_conv_n16: input = window org[img] of the site's composed heights (c1 == 16)
+    const int2 *org;
+    int srows, scols;
 };
 
 // Implicit-GEMM convolution, LDS tiled: a block computes BM = 32 * WAVES_M * WM pixels x
@@ -326,7 +329,16 @@ __global__ void __launch_bounds__(256) k_conv_n16(Conv16Args ca) {
             bf16x8 v = (bf16x8){};
             if ((unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w) {
                 const int ch = cb + 8 * half;
-                if (ch < a.c1) {
+                if (a.site) {
+                    // the window's composed heights / 50 m in channels 0-2 (k_pack_input's values)
+                    const int r = a.org[img].x + yy, c = a.org[img].y + xx;
+                    if (half == 0 && (unsigned)r < (unsigned)a.srows && (unsigned)c < (unsigned)a.scols) {
+                        const float4 hq = a.site[(long)r * a.scols + c];
+                        v[0] = f2bf(hq.x * 0.02f);
+                        v[1] = f2bf(hq.y * 0.02f);
+                        v[2] = f2bf(hq.z * 0.02f);
+                    }
+                } else if (ch < a.c1) {
                     const long p = a.up1 ? (((long)img * hh + (yy >> 1)) * wh + (xx >> 1))
                                          : (((long)img * a.h + yy) * a.w + xx);
                     v = *(const bf16x8 *)(a.in1 + p * a.c1 + ch);
@@ -789,6 +801,10 @@ static int launch_conv(const sw_cnn_op_t &o, const __bf16 *w, void *stream) {
     a.n = o.n; a.h = o.h; a.w = o.w; a.act = o.act;
     a.r0 = o.kind == SW_CNN_CONV ? o.c0 : 0;
     a.rn = o.kind == SW_CNN_CONV ? o.core : 0;
+    a.site = o.kind == SW_CNN_CONV ? (const float4 *)o.site : nullptr;
+    a.org = (const int2 *)o.site_origins;
+    a.srows = o.site_rows;
+    a.scols = o.site_cols;
     const long M = (long)o.n * o.h * o.w;
     const bool fused = o.pool_out || o.head_w;
     const int th = a.rn ? a.rn : o.h, tw = a.rn ? a.rn : o.w;
@@ -796,7 +812,11 @@ static int launch_conv(const sw_cnn_op_t &o, const __bf16 *w, void *stream) {
     // the channel-row kernel stages its input once per 32 output channels: it wins for 16 output
     // channels, for shallow inputs and for grids the 64-channel tiles of k_conv_d leave underfilled
     // (under 512 blocks); deep inputs with many output channels keep k_conv_d (measured per layer)
-    const bool rows16 = o.cout == 16 || fused || o.c1 + o.c2 <= 32 || tiles8x32 * ((o.cout + 63) / 64) < 512;
+    const bool rows16 = o.cout == 16 || fused || a.site || o.c1 + o.c2 <= 32 ||
+                        tiles8x32 * ((o.cout + 63) / 64) < 512;
+    if (a.site && (o.c1 != 16 || o.c2 || o.up || !a.org || o.site_rows <= 0 || o.site_cols <= 0 || o.ks != 3 ||
+                   o.cout % 16 || o.o1 < o.cout || o.outf))
+        return set_err(SW_E_ARG, "sw_cnn: a site-input conv needs 16 input channels, origins, the site size and a 3x3 kernel");
     if (a.rn && (a.r0 < 0 || a.r0 + a.rn > o.h || a.r0 + a.rn > o.w || o.pool_out || o.ks != 3 ||
                  o.cout % 16 || o.o1 < o.cout || (o.outf && !o.head_w)))
         return set_err(SW_E_ARG, "sw_cnn: an output region needs a 3x3 conv with a whole output, no pool, inside the image");
@@ -856,7 +876,7 @@ static int check_conv(const sw_cnn_op_t &o, const char *who) {
     if (o.c1 <= 0 || o.c1 % 16 || o.c2 < 0 || o.c2 % 16 || o.cout <= 0 || o.n <= 0 || o.h <= 0 || o.w <= 0)
         return set_err(SW_E_ARG, who);
     if (o.up && (o.h % 2 || o.w % 2)) return set_err(SW_E_ARG, who);
-    if (!o.in0 || (o.c2 && !o.in1) || !o.weight) return set_err(SW_E_ARG, who);
+    if ((!o.in0 && !(o.kind == SW_CNN_CONV && o.site)) || (o.c2 && !o.in1) || !o.weight) return set_err(SW_E_ARG, who);
     if (o.pool_out && (o.h % 2 || o.w % 2)) return set_err(SW_E_ARG, who);
     if (o.head_w && !o.outf) return set_err(SW_E_ARG, who);
     return SW_OK;

@@ -15,6 +15,10 @@ requests overlaps on the device (each slot has its own HIP stream).
 * ``full`` - a full re-solve of the site's current inputs (compose, SVF
   sweep, static planes, day) in a slot.
 
+Edits go first: queued edits are applied before slot requests are admitted
+or advanced, and slots enqueue their day a few steps at a time (one chunk in
+flight per slot), so an edit shares the GPU with at most one chunk per slot.
+
 Every request's result equals the same request served alone on the site
 version it was admitted against; ``checksum`` fingerprints results for that
 comparison.
@@ -115,7 +119,9 @@ class Slot:
         self.busy = None                  # the request in flight
         self.event = torch.cuda.Event()
         self.pending_plan = None
+        self.plan = None
         self.dplan = None
+        self.next_step = 0                # next day step to enqueue
 
     def footprint(self) -> int:
         s = self.scene
@@ -129,9 +135,10 @@ class Server:
     """One site, a FIFO of arrived requests, ``slots`` execution slots (admitted while HBM allows)."""
 
     def __init__(self, solver: HipSolver, st: I.SiteState, met_pool: list, sun, latitude: float,
-                 pipeline: PlanPipeline, slots: int = 4, hbm_reserve: float = 0.1):
+                 pipeline: PlanPipeline, slots: int = 4, hbm_reserve: float = 0.1, chunk_steps: int = 4):
         self.solver, self.st, self.met_pool, self.sun, self.lat = solver, st, met_pool, sun, latitude
         self.pipeline = pipeline
+        self.chunk_steps = max(1, int(chunk_steps))
         self.slots = []
         free, total = torch.cuda.mem_get_info(solver.device)
         for _ in range(slots):
@@ -180,27 +187,44 @@ class Server:
         return self.met_pool[0]
 
     def _advance(self, slot: Slot, now: float):
-        """Move a slot's request on; returns True when it finished."""
+        """Move a slot's request on; returns True when it finished.
+
+        The day is enqueued a few steps at a time (one chunk in flight per slot), so an edit that
+        arrives shares the GPU with at most one chunk per slot."""
         req = slot.busy
         if slot.pending_plan is not None:
             if not slot.pending_plan.ready():
                 return False
             plan = slot.pending_plan.get()
             slot.pending_plan = None
+            slot.plan, slot.next_step = plan, 0
             with torch.cuda.stream(slot.stream):
                 slot.dplan = _owned_plan(plan, self.solver.device)
                 if req.kind == "full":
                     self.solver.tile_static(slot.scene, slot.dplan, slot.stream)
-                self.solver.run_day(slot.scene, plan, slot.dplan, self.st.outputs, out=slot.out, stream=slot.stream)
-                slot.event.record(slot.stream)
+            self._feed(slot)
             return False
         if not slot.event.query():
+            return False
+        if slot.plan is not None and slot.next_step < slot.plan.n_steps:
+            self._feed(slot)
             return False
         req.finish = time.perf_counter() - self.t0
         if self.checksums:
             req.result = checksum(slot.out)
         slot.busy = None
+        slot.plan = None
         return True
+
+    def _feed(self, slot: Slot):
+        """Enqueue the slot's next chunk of day steps and mark its end."""
+        t0 = slot.next_step
+        t1 = min(slot.plan.n_steps, t0 + self.chunk_steps)
+        with torch.cuda.stream(slot.stream):
+            self.solver.run_day(slot.scene, slot.plan, slot.dplan, self.st.outputs, out=slot.out,
+                                stream=slot.stream, steps=(t0, t1))
+            slot.event.record(slot.stream)
+        slot.next_step = t1
 
     def _edit(self, req: Request):
         p = req.payload
@@ -228,27 +252,29 @@ class Server:
             now = time.perf_counter() - self.t0
             while todo and todo[0].arrival <= now:
                 queue.append(todo.popleft())
+            progressed = False
+            # edits first, in arrival order among themselves, on the master stream; requests waiting
+            # for a slot do not hold them back (each is served on the site version it is admitted at)
+            edits = [r for r in queue if r.kind == "edit"]
+            for req in edits:
+                queue.remove(req)
+                req.start = time.perf_counter() - self.t0
+                self._edit(req)
+                req.finish = time.perf_counter() - self.t0
+                done.append(req)
+                progressed = True
+            if edits:
+                continue                    # new arrivals may be edits too
             for s in self.slots:
                 req = s.busy
                 if req is not None and self._advance(s, now):
                     done.append(req)
-            # edits apply in order on the master stream; slot work is admitted as slots free up
-            progressed = False
+            # slot work is admitted as slots free up, in arrival order
             while queue:
-                req = queue[0]
-                if req.kind == "edit":
-                    queue.popleft()
-                    req.start = now
-                    self._edit(req)
-                    req.finish = time.perf_counter() - self.t0
-                    done.append(req)
-                    progressed = True
-                    break
                 free = next((s for s in self.slots if s.busy is None), None)
                 if free is None:
                     break
-                queue.popleft()
-                self._start(free, req, now)
+                self._start(free, queue.popleft(), now)
                 progressed = True
             if not progressed and not any(s.busy for s in self.slots) and todo and not queue:
                 time.sleep(max(0.0, min(0.002, todo[0].arrival - now)))

@@ -17,6 +17,7 @@ The reference entry points this replaces: ``svf_calculator``
 from __future__ import annotations
 
 import ctypes
+from contextlib import nullcontext as _nullcontext
 import math
 import time
 from dataclasses import dataclass, field
@@ -596,8 +597,11 @@ class HipSolver:
                P(ds.tan_asvf), P(ds.tan_sa), P(ds.walkface), ds.rows, ds.cols, _stream_ptr(stream))
 
     def run_day(self, ds: DeviceScene, plan: DayPlan, dplan, outputs=("tmrt", "utci", "shadow"),
-                out=None, stream=None, reset_state=True, per_launch=False):
-        """Enqueue the time loop; returns the device output tensor (T, k, rows, cols)."""
+                out=None, stream=None, reset_state=True, per_launch=False, steps=None):
+        """Enqueue the time loop; returns the device output tensor (T, k, rows, cols).
+
+        ``steps`` = (t0, t1) enqueues only those steps (their planes of ``out``); the state carries
+        over from the previous range, and is cleared only for a range starting at 0."""
         mask = 0
         for name in outputs:
             mask |= OUT_BITS[name]
@@ -607,6 +611,19 @@ class HipSolver:
             out = torch.empty((T, k, ds.rows, ds.cols), dtype=torch.float32, device=self.device)
         sp = _stream_ptr(stream)
         P = L.ptr
+        if steps is not None:
+            t0, t1 = (int(v) for v in steps)
+            if not 0 <= t0 < t1 <= T or per_launch:
+                raise ValueError(f"step range {steps} of a {T}-step day")
+            if reset_state and t0 == 0:
+                with torch.cuda.stream(stream) if stream is not None else _nullcontext():
+                    ds.state.zero_()
+            # the day flags of the range: sw_solve_day reads is_day[i - t0]
+            is_day = (ctypes.c_int32 * (t1 - t0))(*[int(x) for x in plan.is_day[t0:t1]])
+            base = P(out) + t0 * k * ds.n * 4
+            L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
+                   P(dplan["sun_tab"]), is_day, t0, t1, base, mask, plan.walk_halo, 0, sp)
+            return out
         if reset_state:
             ds.state.zero_()
         is_day = (ctypes.c_int32 * T)(*[int(x) for x in plan.is_day])

@@ -344,6 +344,12 @@ class Passes:
         return ops, ws_need
 
     # -- passes ----------------------------------------------------------------
+    def use_site(self, hpack, d_org, rows: int, cols: int):
+        """Inference: the first convolution reads the windows straight from a site's composed heights
+        (no packed input); ``d_org`` holds the window origins (int32 pairs) on the device."""
+        op = self.fwd_arr[0]
+        op.site, op.site_origins, op.site_rows, op.site_cols = L.ptr(hpack), L.ptr(d_org), int(rows), int(cols)
+
     def forward(self, stream=None):
         L.call("sw_cnn_fwd", self.fwd_arr, len(self.fwd_ops), _sp(stream))
         return self.out
@@ -534,6 +540,7 @@ class SvfSurrogate:
         if self.ps is None or self.ps.n != len(org):
             self.ps = Passes(self.net, len(org), train=False)
         site = SiteField(ds.hpack, fields, ds.rows, ds.cols)
+        self.ps.fwd_arr[0].site = None
         d_org = load_batch(self.ps, site, org, with_labels=False, stream=stream)
         self.ps.forward(stream)
         L.call("sw_cnn_scatter_cores", L.ptr(self.ps.out), L.ptr(d_org), L.ptr(_field_map(self.net.device)),
@@ -566,7 +573,7 @@ class SvfSurrogate:
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record(main)
         sp = _sp(main)
-        L.call("sw_cnn_pack_input", L.ptr(ds.hpack), L.ptr(g["d_org"]), L.ptr(ps.x), len(org), ps.hw, rows, cols, sp)
+        ps.use_site(ds.hpack, g["d_org"], rows, cols)
         ps.forward(main)
         ev[1].record(main)
         main.wait_event(sampled_done)

@@ -95,6 +95,28 @@ def test_fused_epilogues_match_separate_ops(S):
     assert torch.equal(f.a["d1a"][:, 15:49, 15:49], i.a["d1a"][:, 15:49, 15:49])
 
 
+def test_site_input_equals_packed_input(S):
+    """The first convolution reading a site's heights directly gives the same activations as the
+    packed input, windows partly outside the site included."""
+    from solweig_project_amd import _lib as L
+    net = S.SurrogateNet(channels=(16, 32, 32, 48), seed=9)
+    rows, cols = 90, 110
+    rng = np.random.default_rng(9)
+    hp = torch.from_numpy(rng.uniform(0, 40, (rows * cols, 4)).astype(np.float32)).cuda()
+    org = np.array([[-16, -16], [10, 30], [40, 60]], np.int32)
+    d_org = torch.from_numpy(org).cuda()
+    a = S.Passes(net, batch=3, hw=64, core=32, c0=16, nchunk=4, train=False)
+    b = S.Passes(net, batch=3, hw=64, core=32, c0=16, nchunk=4, train=False)
+    L.call("sw_cnn_pack_input", L.ptr(hp), L.ptr(d_org), L.ptr(a.x), 3, 64, rows, cols, None)
+    a.forward()
+    b.use_site(hp, d_org, rows, cols)
+    b.forward()
+    torch.cuda.synchronize()
+    for name in ("e1a", "e1", "p1", "e4", "d1a"):
+        assert torch.equal(a.a[name], b.a[name]), name
+    assert torch.equal(a.out, b.out)
+
+
 def test_adam_step_matches_formula(S):
     net, ps, _ = _small(S, seed=2)
     ps.forward()
@@ -176,7 +198,8 @@ def test_window_copies(S):
 
 def test_gated_surrogate_on_a_city(S):
     """The shipped network on an unseen 768 x 640 city: every rejected core is exact, accepted cores
-    hold the gate's mean error, and an untrained network is rejected everywhere (the gate falls back)."""
+    hold the gate's mean error, the derived day's Tmrt stays within 0.5 K per core (mean), and an
+    untrained network is rejected everywhere (the gate falls back)."""
     from solweig_project_amd import synth
     from solweig_project_amd.solver import HipSolver, TileInputs
     _say("gated surrogate on a 768 x 640 city")
@@ -201,6 +224,26 @@ def test_gated_surrogate_on_a_city(S):
         else:
             assert float(win.mean()) <= 0.03, (w, float(win.mean()))
     assert float(err.mean()) <= 0.02
+    # derived Tmrt (fields 0-9 and SVFtotal from the gate, masks and aveg fields exact): every core's
+    # mean |dTmrt| over the day within 0.5 K
+    from solweig_project_amd import sunpos
+    from solweig_project_amd.solver import plan_day
+    sun = sunpos.day_sun_table(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    plan = plan_day(ds.tile, synth.make_met(sun.altitude, seed=778), sun, site.latitude)
+    dplan = solver.upload_plan(plan)
+
+    def day_tmrt():
+        solver.tile_static(ds, dplan)
+        o = solver.run_day(ds, plan, dplan, ("tmrt",))
+        torch.cuda.synchronize()
+        return o[:, 0].clone()
+    gated_t = day_tmrt()
+    solver.svf(ds)
+    exact_t = day_tmrt()
+    per_px = (gated_t - exact_t).abs().mean(dim=0)
+    core_means = [float(per_px[r0:r0 + 256, c0:c0 + 256].mean()) for r0, c0 in org.tolist()]
+    _say(f"derived Tmrt: core means max {max(core_means):.3f} K, overall {float(per_px.mean()):.4f} K")
+    assert max(core_means) <= 0.5 and float(per_px.mean()) <= 0.1
     untrained = S.SurrogateNet(net.channels, seed=5)
     solver.svf(ds)
     rep_u = S.SvfSurrogate(untrained, tol=0.02, samples=4).svf(solver, ds)
