@@ -629,6 +629,31 @@ def main():
                            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
                            "avg_launch_ms": round(avg_ms, 4), "launches_per_solve": launches // args.steps,
                            "algorithmic_bytes_per_launch": int(alg[name])}
+    # UTCI on the vector ALUs: the standalone kernel (the function the flux kernel fuses) over this
+    # workload's pixel-steps, 450 FLOP per pixel-step (210 polynomial terms as FMAs + the saturation
+    # vapour pressure chain, log and exp counted once each)
+    from solweig_project_amd import _lib as LU
+    m = rows * cols * met.shape[0]
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    u_in = [torch.rand(m, device=dev, generator=g) * (hi - lo) + lo
+            for lo, hi in ((22.0, 36.0), (30.0, 70.0), (20.0, 70.0), (0.5, 4.0))]
+    u_out = torch.empty(m, device=dev)
+    us = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        LU.call("sw_utci", *[LU.ptr(x) for x in u_in], LU.ptr(u_out), m, us)
+    ue = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ue[0].record()
+    for _ in range(5):
+        LU.call("sw_utci", *[LU.ptr(x) for x in u_in], LU.ptr(u_out), m, us)
+    ue[1].record()
+    torch.cuda.synchronize()
+    u_ms = ue[0].elapsed_time(ue[1]) / 5
+    u_tf = 450.0 * m / (u_ms / 1e3) / 1e12
+    rooflines["utci"] = {"bound": "valu", "achieved": round(u_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(u_tf / FP32_PEAK_TFLOPS, 4), "traffic": None, "avg_launch_ms": round(u_ms, 4),
+                         "pixel_steps": m, "flop_per_pixel_step": 450,
+                         "note": "standalone k_utci over the day's pixel-steps; the flux kernel fuses the same function"}
+    del u_in, u_out
     # dominant kernel = largest total time per solve
     dom = max(ksum, key=lambda k: ksum[k][1])
     roof = dict(kernel=dom, **rooflines[dom]) if dom in rooflines else None
