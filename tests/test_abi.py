@@ -1,6 +1,7 @@
 """The C ABI: header declarations, exported symbols, ctypes signatures, struct layout.
 
-Runs on the CPU: loads libsolweig_hip.so without calling any compute entry point.
+Runs on the CPU: loads libsolweig_hip.so and calls entry points only with arguments they refuse
+before any device work.
 """
 import ctypes
 import re
@@ -80,3 +81,27 @@ def test_struct_layout_matches_c(tmp_path):
 def test_loader_fails_loudly_without_library(tmp_path):
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         L.load(tmp_path / "missing.so")
+
+
+def test_argument_refusals_return_before_any_launch():
+    """Entry points refuse bad arguments with a message and an error code before touching the
+    device (pointers here are never dereferenced): oversized tiles for the padded march buffer, an
+    empty output mask, a malformed convolution op, bad gate sizes."""
+    L.load()
+    fake = ctypes.c_void_p(256)
+    with pytest.raises(L.HipError, match="too large"):
+        L.call("sw_svf_march", fake, fake, fake, fake, fake, fake, fake, 14000, 14000, None)
+    tile = L.TileStruct()
+    tile.rows, tile.cols = 8, 8
+    day = (ctypes.c_int32 * 2)(0, 1)
+    with pytest.raises(L.HipError, match="selects nothing"):
+        L.call("sw_solve_day", ctypes.byref(tile), fake, fake, fake, day, 0, 2, fake, 0, 0, 0, None)
+    with pytest.raises(L.HipError, match="step range"):
+        L.call("sw_solve_day", ctypes.byref(tile), fake, fake, fake, day, 2, 1, fake, 1, 0, 0, None)
+    op = L.CnnOp(kind=L.CNN_CONV, n=1, h=8, w=8, c1=8, c2=0, o1=16, cout=16, ks=3)
+    op.in0, op.out0, op.weight = fake, fake, fake
+    arr = (L.CnnOp * 1)(op)
+    with pytest.raises(L.HipError, match="multiple of 16"):
+        L.call("sw_cnn_fwd", arr, 1, None)
+    with pytest.raises(L.HipError, match="bad sizes"):
+        L.call("sw_gate_write", fake, fake, fake, fake, fake, 1, 64, 16, 32, 16, 4, 8, 8, None)
