@@ -48,6 +48,8 @@ def parse():
                          "re-solves of a 2000x2000 site at --rate requests/s")
     ap.add_argument("--requests", type=int, default=1000, help="requests served by --workload serve")
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="c2: consecutive solves alternate over this many scene copies on their own streams")
     ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
     ap.add_argument("--chunk-steps", type=int, default=4, help="day steps a serving slot enqueues at a time")
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
@@ -564,19 +566,28 @@ def main():
     ds = DeviceScene(tile, dev, inputs)
     out = torch.empty((T, 3, rows, cols), dtype=torch.float32, device=dev)
     job = Job(met, sun, site.latitude)
+    # lanes: consecutive solves alternate over scene copies on their own streams, so one solve's
+    # kernel tails overlap the next solve's kernels (every solve is still a complete SVF + day)
+    lanes = None
+    if args.lanes > 1:
+        lanes = [(ds, torch.cuda.Stream(dev), out)]
+        for _ in range(args.lanes - 1):
+            lanes.append((DeviceScene(tile, dev, upload_inputs(tile, dev)), torch.cuda.Stream(dev),
+                          torch.empty_like(out)))
     log("warmup")
     pipeline.warm(ds.tile, job)
-    solve_many(solver, ds, [job] * max(args.warmup, 1), pipeline, out=out)
+    n_warm = max(args.warmup, 1, len(lanes) if lanes else 1)
+    solve_many(solver, ds, [job] * n_warm, pipeline, out=out, lanes=lanes)
     torch.cuda.synchronize()
     ds.check_flags()
     timer = KernelTimer(reserve=args.steps * (6 + 3 * T))
     solver.timer = timer
-    log(f"timing {args.steps} steps")
+    log(f"timing {args.steps} steps over {max(args.lanes, 1)} lane(s)")
     solver.plan_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     # every solve plans its own day (in the planner process, one solve ahead)
-    solve_many(solver, ds, [job] * args.steps, pipeline, out=out)
+    solve_many(solver, ds, [job] * args.steps, pipeline, out=out, lanes=lanes)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     solver.timer = _NoTimer()
@@ -706,7 +717,7 @@ def main():
         "unit": "Mpixel-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": n_warm,
         "ms_per_step": round(ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -715,7 +726,7 @@ def main():
         "data": "synthetic (seeded city tile generator; inputs resident in HBM, outputs left in HBM)",
         "config": {"workload": f"full SOLWEIG solve {rows}x{cols}x{T}h (SVF sweep + sun march + GVF + "
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
-                   "time_steps": T, "pixel_size_m": site.pixel_size},
+                   "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": max(args.lanes, 1)},
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,

@@ -85,11 +85,15 @@ class PlanPipeline:
 
 
 def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci", "shadow"), out=None,
-               stream=None):
+               stream=None, lanes=None):
     """Solve every job on the scene ``ds`` (SVF sweep + day), planning one job ahead.
 
     ``out`` receives each job's outputs in turn (the last job's remain);
     returns the device output tensor. The host never waits for the GPU.
+
+    ``lanes``: a list of (scene, stream, out) triples; job k runs on lane k mod len(lanes), so
+    consecutive independent solves overlap on the device (each lane's buffers, plan staging and
+    outputs are its own; ``ds``, ``stream`` and ``out`` are then unused). Returns the last lane's out.
     """
     import time
     if not jobs:
@@ -97,6 +101,8 @@ def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci"
     marks = []
     pending = pipeline.submit(ds.tile, jobs[0])
     for k, job in enumerate(jobs):
+        if lanes:
+            ds, stream, out = lanes[k % len(lanes)]
         t0 = time.perf_counter()
         solver.svf(ds, stream)
         nxt = pipeline.submit(ds.tile, jobs[k + 1]) if k + 1 < len(jobs) else None

@@ -503,7 +503,7 @@ class HipSolver:
         self._tables = {}
         self.timer = _NoTimer()
         self.plan_ms: list[float] = []      # host planning time of each solve
-        self._staging = None
+        self._staging = {}            # plan staging (pinned host + device), one per stream
 
     # -- SVF ---------------------------------------------------------------
     def upload(self, tile: TileInputs) -> DeviceScene:
@@ -570,20 +570,21 @@ class HipSolver:
         sizes = [p.nbytes for p in parts]
         offs = np.concatenate([[0], np.cumsum([(z + 255) // 256 * 256 for z in sizes])]).astype(int)
         total = int(offs[-1])
-        st = self._staging
+        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        key = s.cuda_stream                # one staging buffer per stream: concurrent solves on two streams
+        st = self._staging.get(key)
         if st is None or st["host"].numel() < total:
             if st is not None:
                 st["event"].synchronize()
             cap = max(total, 1 << 20)
             st = dict(host=torch.empty(cap, dtype=torch.uint8, pin_memory=True),
                       dev=torch.empty(cap, dtype=torch.uint8, device=self.device), event=torch.cuda.Event())
-            st["event"].record()
-            self._staging = st
+            st["event"].record(s)
+            self._staging[key] = st
         st["event"].synchronize()          # the previous copy out of the staging buffer has run
         host = st["host"].numpy()
         for p, o in zip(parts, offs):
             host[o:o + p.nbytes] = p
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
         with torch.cuda.stream(s):
             st["dev"][:total].copy_(st["host"][:total], non_blocking=True)
             st["event"].record(s)
