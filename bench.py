@@ -580,6 +580,16 @@ def main():
     solve_many(solver, ds, [job] * n_warm, pipeline, out=out, lanes=lanes)
     torch.cuda.synchronize()
     ds.check_flags()
+    mt = solver.march_tuner
+    mkey = (rows, cols, float(tile.scale))
+    march_info = {"autotuned": mt.enabled}
+    if mkey in mt.best:
+        ch, sh, spl = mt.best[mkey]
+        tm = mt.timings[mkey]
+        march_info.update(chunk=ch, shape=["8x8", "4x16", "16x4"][sh], split=spl, tuned_ms=tm[mt.best[mkey]],
+                          default_ms=tm[mt.DEFAULT], candidates=len(tm))
+        log(f"march autotuner: chunk {ch}, wave {march_info['shape']}, split {spl}: "
+            f"{tm[mt.best[mkey]]:.3f} ms (default {tm[mt.DEFAULT]:.3f} ms, {len(tm)} candidates)")
     timer = KernelTimer(reserve=args.steps * (6 + 3 * T))
     solver.timer = timer
     log(f"timing {args.steps} steps over {max(args.lanes, 1)} lane(s)")
@@ -726,7 +736,8 @@ def main():
         "data": "synthetic (seeded city tile generator; inputs resident in HBM, outputs left in HBM)",
         "config": {"workload": f"full SOLWEIG solve {rows}x{cols}x{T}h (SVF sweep + sun march + GVF + "
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
-                   "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": max(args.lanes, 1)},
+                   "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": max(args.lanes, 1),
+                   "march": march_info},
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,

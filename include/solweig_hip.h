@@ -199,6 +199,18 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
  * bounds-checked march). */
 int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
                  uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream);
+/* March launch configuration (no counterpart in the reference; chosen per tile shape by the
+ * autotuner, solweig_project_amd.solver.MarchTuner). None of it changes any output:
+ *   chunk  steps per gather chunk of the fast path: 4, 8 (default) or 12;
+ *   shape  a wave's pixels in the 16 x 16 tile: 0 an 8 x 8 square (default), 1 a 4 x 16 row
+ *          strip, 2 a 16 x 4 column strip;
+ *   split  blocks per 32-patch mask word: 1 (default), 2 or 4. */
+typedef struct sw_march_config {
+    int32_t chunk, shape, split, pad;
+} sw_march_config_t;
+int sw_svf_march_cfg(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
+                     uint32_t *masks, uint32_t *stats, int rows, int cols, const sw_march_config_t *cfg,
+                     void *stream);
 /* Fold the masks into the 16 fields [svf, E, S, W, N, veg, vegE.., aveg, avegE.., total]
  * (shadow.py:545-613), in the reference's per-annulus summation order. */
 int sw_svf_fold(const uint32_t *masks, const float *vegdsm2, const float *w_iso,
@@ -276,6 +288,11 @@ int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const 
                  const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
                  const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
                  float *fields, int rows, int cols, void *stream);
+/* sw_svf_sweep with a march configuration (NULL: the defaults) */
+int sw_svf_sweep_cfg(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                     const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                     const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                     float *fields, int rows, int cols, const sw_march_config_t *cfg, void *stream);
 /* Day loop over steps [t0, t1) (the time loop of run_utci_window,
  * utci_process.py:917-1049): per step the sun march when is_day[i - t0]
  * (host array) and the flux step; outputs of step i at

@@ -117,6 +117,12 @@ WALL_ROTATIONS = 180
 WALL_LIST_CAP = 128
 
 
+class MarchConfig(ctypes.Structure):
+    """Mirror of ``sw_march_config_t`` (chunk 4/8/12, wave shape 0-2, split 1/2/4)."""
+
+    _fields_ = [("chunk", _c_i32), ("shape", _c_i32), ("split", _c_i32), ("pad", _c_i32)]
+
+
 class WallFiltersStruct(ctypes.Structure):
     """Mirror of ``sw_wall_filters_t`` (rotated wall-aspect filters)."""
 
@@ -178,6 +184,8 @@ SIGNATURES = {
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
+    "sw_svf_march_cfg": (_I, [_VP] * 7 + [_I, _I, _VP, _VP]),
+    "sw_svf_sweep_cfg": (_I, [_VP] * 14 + [_I, _I, _VP, _VP]),
     "sw_svf_sweep_tiles": (_I, [_VP] * 15 + [_I, _I, _I, _VP]),
     "sw_solve_day_tiles": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP]),
     "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),

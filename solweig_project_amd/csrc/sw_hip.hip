@@ -296,11 +296,12 @@ __device__ __forceinline__ uint32_t svf_out(const SvfLane &st) {
 // ray that left the grid stays out, dz grows along the patch), so testing
 // the last step of a chunk finds the exit the per-step test would have
 // found in that chunk, and the steps run past it change no output.
-#define SVF_CHUNK 8
-static_assert(SVF_CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
+#define SVF_CHUNK 8                 // default steps per gather chunk (autotuned: 4, 8 or 12)
+template <int CHUNK>
 __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, uint32_t tpb, float at,
                                                    float veg2t, bool valid, const int4 *__restrict__ T,
                                                    int count, float hmax) {
+    static_assert(CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
     SvfLane st;
     st.sh = false; st.vs = false; st.vbb = false;
     bool active = valid && count > 0;
@@ -311,19 +312,19 @@ __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, ui
         active = !(st.sh & !st.vs);
     }
     int s = 1;
-    for (; s + SVF_CHUNK <= count; s += SVF_CHUNK) {
+    for (; s + CHUNK <= count; s += CHUNK) {
         if (!wave_any(active)) break;
-        int4 e[SVF_CHUNK];
+        int4 e[CHUNK];
 #pragma unroll
-        for (int k = 0; k < SVF_CHUNK; ++k) e[k] = T[s + k];
+        for (int k = 0; k < CHUNK; ++k) e[k] = T[s + k];
         if (active) {
-            H3 h[SVF_CHUNK];
+            H3 h[CHUNK];
 #pragma unroll
-            for (int k = 0; k < SVF_CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
+            for (int k = 0; k < CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
 #pragma unroll
-            for (int k = 0; k < SVF_CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
-            const float dl = __int_as_float(e[SVF_CHUNK - 1].z);
-            active = !((st.sh & !st.vs) | outside(h[SVF_CHUNK - 1]) | (hmax - dl <= at));
+            for (int k = 0; k < CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
+            const float dl = __int_as_float(e[CHUNK - 1].z);
+            active = !((st.sh & !st.vs) | outside(h[CHUNK - 1]) | (hmax - dl <= at));
         }
     }
     for (; s < count; ++s) {
@@ -386,6 +387,10 @@ __device__ __forceinline__ uint32_t march_svf_checked(__amdgpu_buffer_rsrc_t rs,
 // square, which keeps more lanes busy than a row segment because exit
 // steps are spatially correlated), y = 32-patch word group.
 #define SVF_TILE 16
+// SHAPE: the pixels of a wave within the 16 x 16 tile: 0 an 8 x 8 square, 1 a 4 x 16 row strip, 2 a
+// 16 x 4 column strip (exits are spatially correlated, so the shape sets how long a wave keeps lanes
+// marching). CHUNK: steps per gather chunk of the fast path. Neither changes any output.
+template <int CHUNK, int SHAPE>
 __global__ void __launch_bounds__(256)
 k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, const int4 *__restrict__ tab,
             const int32_t *__restrict__ counts, uint32_t *__restrict__ masks,
@@ -402,8 +407,10 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         if (tile < 0 || tile >= tiles_x * ((rows + SVF_TILE - 1) / SVF_TILE)) return;
     }
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    const int r = (tile / tiles_x) * SVF_TILE + (wv >> 1) * 8 + (ln >> 3);
-    const int c = (tile % tiles_x) * SVF_TILE + (wv & 1) * 8 + (ln & 7);
+    const int ry = SHAPE == 0 ? (wv >> 1) * 8 + (ln >> 3) : (SHAPE == 1 ? wv * 4 + (ln >> 4) : (ln >> 2));
+    const int cx = SHAPE == 0 ? (wv & 1) * 8 + (ln & 7) : (SHAPE == 1 ? (ln & 15) : wv * 4 + (ln & 3));
+    const int r = (tile / tiles_x) * SVF_TILE + ry;
+    const int c = (tile % tiles_x) * SVF_TILE + cx;
     const bool valid = (r < rows) && (c < cols);
     const int W = cols + 2 * SW_SVF_PAD;
     const uint32_t nbytes = (uint32_t)((long)(rows + 2 * SW_SVF_PAD) * W * 12);
@@ -430,7 +437,7 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         const bool chk = checked || (counts[p] > 0 && __int_as_float(tab[offs[p]].z) < 0.f);
         const uint32_t m = chk
             ? march_svf_checked(rs, tpb, r, c, rows, cols, at, veg2t, valid, tab + offs[p], counts[p], hmax)
-            : march_svf_fast(rs, tpb, at, veg2t, valid, tab + offs[p], counts[p], hmax);
+            : march_svf_fast<CHUNK>(rs, tpb, at, veg2t, valid, tab + offs[p], counts[p], hmax);
         const uint32_t bit = 1u << (p - p0);
         if (m & 1u) bsh |= bit;
         if (m & 2u) bvs |= bit;
@@ -443,6 +450,13 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         masks[(long)(1 * SW_MASK_WORDS + w) * n + t] = bvs;
         masks[(long)(2 * SW_MASK_WORDS + w) * n + t] = bv1;
         masks[(long)(3 * SW_MASK_WORDS + w) * n + t] = bv2;
+    } else if (valid && split == 2) {
+        // this block's half of each word (little-endian: bits 16 sub .. 16 sub + 15)
+        uint16_t *mh = (uint16_t *)masks;
+        mh[((long)(0 * SW_MASK_WORDS + w) * n + t) * 2 + sub] = (uint16_t)(bsh >> (16 * sub));
+        mh[((long)(1 * SW_MASK_WORDS + w) * n + t) * 2 + sub] = (uint16_t)(bvs >> (16 * sub));
+        mh[((long)(2 * SW_MASK_WORDS + w) * n + t) * 2 + sub] = (uint16_t)(bv1 >> (16 * sub));
+        mh[((long)(3 * SW_MASK_WORDS + w) * n + t) * 2 + sub] = (uint16_t)(bv2 >> (16 * sub));
     } else if (valid) {
         // this block's byte of each word (little-endian: bits 8 sub .. 8 sub + 7)
         uint8_t *mb = (uint8_t *)masks;
@@ -451,6 +465,29 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         mb[((long)(2 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bv1 >> (8 * sub));
         mb[((long)(3 * SW_MASK_WORDS + w) * n + t) * 4 + sub] = (uint8_t)(bv2 >> (8 * sub));
     }
+}
+
+// launch of the march with a configuration (chunk 4 / 8 / 12, shape 0 / 1 / 2, split 1 / 2 / 4)
+static void launch_march(dim3 grid, hipStream_t st, int chunk, int shape, const float *hpad, const int32_t *offs,
+                         const int4 *tab, const int32_t *counts, uint32_t *masks, const uint32_t *stats, int rows,
+                         int cols, const int32_t *tiles, int ntiles, int split) {
+#define SW_MARCH_CASE(C, SH)                                                                                  \
+    if (chunk == C && shape == SH) {                                                                         \
+        hipLaunchKernelGGL((k_svf_march<C, SH>), grid, dim3(256), 0, st, hpad, offs, tab, counts, masks, stats, \
+                           rows, cols, tiles, ntiles, split);                                                \
+        return;                                                                                              \
+    }
+    SW_MARCH_CASE(8, 0) SW_MARCH_CASE(4, 0) SW_MARCH_CASE(12, 0)
+    SW_MARCH_CASE(8, 1) SW_MARCH_CASE(4, 1) SW_MARCH_CASE(12, 1)
+    SW_MARCH_CASE(8, 2) SW_MARCH_CASE(4, 2) SW_MARCH_CASE(12, 2)
+#undef SW_MARCH_CASE
+}
+
+static int check_march_config(const sw_march_config_t *cfg, const char *who) {
+    if (!cfg) return SW_OK;
+    const bool ok = (cfg->chunk == 4 || cfg->chunk == 8 || cfg->chunk == 12) && cfg->shape >= 0 && cfg->shape <= 2 &&
+                    (cfg->split == 1 || cfg->split == 2 || cfg->split == 4);
+    return ok ? SW_OK : set_err(SW_E_ARG, who);
 }
 
 __device__ __forceinline__ void load_masks(const uint32_t *__restrict__ masks, long n, long t,
@@ -1562,7 +1599,16 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
 
 int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
                  uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream) {
+    return sw_svf_march_cfg(hpack, offs, tab, counts, hpad, masks, stats, rows, cols, nullptr, stream);
+}
+
+int sw_svf_march_cfg(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
+                     uint32_t *masks, uint32_t *stats, int rows, int cols, const sw_march_config_t *cfg,
+                     void *stream) {
     int rc = check_shape(rows, cols, "sw_svf_march");
+    if (rc) return rc;
+    if ((rc = check_march_config(cfg, "sw_svf_march: march configuration (chunk 4/8/12, shape 0-2, split 1/2/4)")))
+        return rc;
     if (rc) return rc;
     if (!hpack || !offs || !tab || !counts || !hpad || !masks || !stats)
         return set_err(SW_E_ARG, "sw_svf_march: NULL pointer");
@@ -1577,9 +1623,10 @@ int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int
     rc = hip_check(hipGetLastError(), "k_pad_heights");
     if (rc) return rc;
     const long tiles = (long)((rows + SVF_TILE - 1) / SVF_TILE) * ((cols + SVF_TILE - 1) / SVF_TILE);
-    dim3 grid((unsigned)(((tiles + 7) / 8) * 8), SW_MASK_WORDS);
-    hipLaunchKernelGGL(k_svf_march, grid, dim3(256), 0, S(stream), (const float *)hpad, offs, (const int4 *)tab,
-                       counts, masks, stats, rows, cols, (const int32_t *)nullptr, 0, 1);
+    const int chunk = cfg ? cfg->chunk : SVF_CHUNK, shape = cfg ? cfg->shape : 0, split = cfg ? cfg->split : 1;
+    dim3 grid((unsigned)(((tiles + 7) / 8) * 8), SW_MASK_WORDS * split);
+    launch_march(grid, S(stream), chunk, shape, (const float *)hpad, offs, (const int4 *)tab, counts, masks, stats,
+                 rows, cols, (const int32_t *)nullptr, 0, split);
     return hip_check(hipGetLastError(), "k_svf_march");
 }
 
@@ -1607,8 +1654,8 @@ int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, 
     const unsigned g = (unsigned)(((ntiles + 7) / 8) * 8);
     // short lists (under 8 blocks per CU at one word per block) split each word over four blocks
     const int split = (long)g * SW_MASK_WORDS < 2048 ? 4 : 1;
-    hipLaunchKernelGGL(k_svf_march, dim3(g, SW_MASK_WORDS * split), dim3(256), 0, S(stream), (const float *)hpad,
-                       offs, (const int4 *)tab, counts, masks, stats, rows, cols, tiles, ntiles, split);
+    launch_march(dim3(g, SW_MASK_WORDS * split), S(stream), SVF_CHUNK, 0, (const float *)hpad, offs,
+                 (const int4 *)tab, counts, masks, stats, rows, cols, tiles, ntiles, split);
     rc = hip_check(hipGetLastError(), "k_svf_march");
     if (rc) return rc;
     hipLaunchKernelGGL(k_svf_fold_tiles, dim3(g), dim3(256), 0, S(stream), masks, vegdsm2, w_iso, w_aniso, ring,

@@ -214,7 +214,15 @@ int sw_svf_sweep(const void *hpack, uint32_t *stats, const int32_t *offs, const 
                  const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
                  const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
                  float *fields, int rows, int cols, void *stream) {
-    int rc = sw_svf_march(hpack, offs, tab, counts, hpad, masks, stats, rows, cols, stream);
+    return sw_svf_sweep_cfg(hpack, stats, offs, tab, vegdsm2, w_iso, w_aniso, ring, nann, dirm, counts, hpad, masks,
+                            fields, rows, cols, nullptr, stream);
+}
+
+int sw_svf_sweep_cfg(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                     const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                     const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
+                     float *fields, int rows, int cols, const sw_march_config_t *cfg, void *stream) {
+    int rc = sw_svf_march_cfg(hpack, offs, tab, counts, hpad, masks, stats, rows, cols, cfg, stream);
     if (rc) return rc;
     return sw_svf_fold(masks, vegdsm2, w_iso, w_aniso, ring, nann, dirm, fields, rows, cols, stream);
 }

@@ -412,6 +412,57 @@ def upload_inputs(tile: TileInputs, device):
     return d
 
 
+class MarchTuner:
+    """Autotuner of the SVF march launch (``sw_march_config_t``): the gather chunk length (steps per
+    wave between exit tests), the wave's pixel shape in the 16 x 16 tile and the blocks per 32-patch
+    mask word. On the first sweep of a tile shape it times every candidate on the scene itself
+    (HIP events on the sweep's stream, best of ``reps``) and keeps the fastest for that
+    (rows, cols, scale); every candidate writes the same masks. ``enabled`` False (or the
+    environment variable SW_MARCH_TUNE=0) keeps the defaults (chunk 8, 8 x 8 squares, split 1)."""
+
+    CANDIDATES = tuple((c, sh, sp) for c in (8, 4, 12) for sh in (0, 1, 2) for sp in (1, 2))
+    DEFAULT = (8, 0, 1)
+
+    def __init__(self, enabled: bool = True, reps: int = 2):
+        import os
+        self.enabled = enabled and os.environ.get("SW_MARCH_TUNE", "1") != "0"
+        self.reps = reps
+        self.best = {}        # (rows, cols, scale) -> (chunk, shape, split)
+        self.timings = {}     # (rows, cols, scale) -> {candidate: ms}
+
+    @staticmethod
+    def struct(c) -> L.MarchConfig:
+        return L.MarchConfig(chunk=c[0], shape=c[1], split=c[2])
+
+    def config(self, solver, ds, offs, tab, stream=None) -> L.MarchConfig:
+        key = (ds.rows, ds.cols, float(ds.tile.scale))
+        if not self.enabled:
+            return self.struct(self.DEFAULT)
+        if key not in self.best:
+            self.best[key], self.timings[key] = self._tune(solver, ds, offs, tab, stream)
+        return self.struct(self.best[key])
+
+    def _tune(self, solver, ds, offs, tab, stream):
+        P = L.ptr
+        sp = _stream_ptr(stream)
+        s = torch.cuda.current_stream(solver.device) if stream is None else stream
+        args = (P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.hpad), P(ds.masks), P(ds.stats), ds.rows, ds.cols)
+        L.call("sw_svf_march_cfg", *args, ctypes.byref(self.struct(self.DEFAULT)), sp)        # warm
+        times = {}
+        for c in self.CANDIDATES:
+            cs = self.struct(c)
+            best = float("inf")
+            for _ in range(self.reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                L.call("sw_svf_march_cfg", *args, ctypes.byref(cs), sp)
+                e1.record(s)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            times[c] = round(best, 4)
+        return min(times, key=times.get), times
+
+
 class KernelTimer:
     """Per-launch kernel times from HIP events recorded on the launch stream.
 
@@ -490,10 +541,11 @@ class _NoTimer:
 class HipSolver:
     """Drives the HIP kernels for one tile at a time on one GPU."""
 
-    def __init__(self, device: str | torch.device = "cuda"):
+    def __init__(self, device: str | torch.device = "cuda", tune_march: bool = True):
         if not torch.cuda.is_available():
             raise RuntimeError("HipSolver needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device)
+        self.march_tuner = MarchTuner(enabled=tune_march)
         self.lib = L.load()
         self.vault = SV.build_skyvault()
         v = self.vault
@@ -536,15 +588,19 @@ class HipSolver:
                    P(ds.vegdsm2), P(ds.buildings), P(ds.cls), P(ds.aspect), P(ds.stats), ds.rows, ds.cols, sp)
 
     def svf(self, ds: DeviceScene, stream=None, tables=None, compose=True):
-        """Compose (unless ``compose`` is False) and the SVF sweep; returns the field planes."""
+        """Compose (unless ``compose`` is False) and the SVF sweep; returns the field planes.
+
+        The march runs with the configuration the autotuner chose for this tile shape (tuned on the
+        first sweep of the shape when ``self.march_tuner`` is enabled)."""
         sp = _stream_ptr(stream)
         P = L.ptr
         if compose:
             self.compose(ds, stream)
         offs, tab = tables or self.svf_tables(ds.rows, ds.cols, ds.tile.scale)
+        cfg = self.march_tuner.config(self, ds, offs, tab, stream)
         with self.timer.span("svf_march", stream):
-            L.call("sw_svf_march", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.hpad), P(ds.masks),
-                   P(ds.stats), ds.rows, ds.cols, sp)
+            L.call("sw_svf_march_cfg", P(ds.hpack), P(offs), P(tab), P(ds.counts), P(ds.hpad), P(ds.masks),
+                   P(ds.stats), ds.rows, ds.cols, ctypes.byref(cfg), sp)
         c = self._fold_consts
         with self.timer.span("svf_fold", stream):
             L.call("sw_svf_fold", P(ds.masks), P(ds.vegdsm2), P(c["w_iso"]), P(c["w_aniso"]), P(c["ring"]),

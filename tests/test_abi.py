@@ -67,6 +67,8 @@ def test_struct_layout_matches_c(tmp_path):
                    ' offsetof(sw_wall_filters_t, off));'
                    'printf("%zu %zu %zu\\n", sizeof(sw_batch_item_t), offsetof(sw_batch_item_t, out),'
                    ' offsetof(sw_batch_item_t, flux_block0));'
+                   'printf("%zu %zu %zu %zu\\n", sizeof(sw_march_config_t), offsetof(sw_march_config_t, split),'
+                   ' sizeof(sw_cnn_op_t), offsetof(sw_cnn_op_t, site_cols));'
                    'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
@@ -75,7 +77,8 @@ def test_struct_layout_matches_c(tmp_path):
                     L.StepStruct.pl.offset, L.ModelStruct.walk_hi.offset, ctypes.sizeof(L.TileStruct),
                     L.TileStruct.state.offset, ctypes.sizeof(L.WallFiltersStruct), L.WallFiltersStruct.off.offset,
                     ctypes.sizeof(L.BatchItemStruct), L.BatchItemStruct.out.offset,
-                    L.BatchItemStruct.flux_block0.offset]
+                    L.BatchItemStruct.flux_block0.offset, ctypes.sizeof(L.MarchConfig), L.MarchConfig.split.offset,
+                    ctypes.sizeof(L.CnnOp), L.CnnOp.site_cols.offset]
 
 
 def test_loader_fails_loudly_without_library(tmp_path):
