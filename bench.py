@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
     ap.add_argument("--lanes", type=int, default=2,
                     help="c2: consecutive solves alternate over this many scene copies on their own streams")
+    ap.add_argument("--march", default="",
+                    help="c2: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
     ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
     ap.add_argument("--chunk-steps", type=int, default=4, help="day steps a serving slot enqueues at a time")
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
@@ -561,7 +563,7 @@ def main():
     T = met.shape[0]
     tile = TileInputs.from_site(site)
     dev = torch.device("cuda:0")
-    solver = HipSolver(dev)
+    solver = HipSolver(dev, march_config=tuple(int(x) for x in args.march.split(",")) if args.march else None)
     inputs = upload_inputs(tile, dev)
     ds = DeviceScene(tile, dev, inputs)
     out = torch.empty((T, 3, rows, cols), dtype=torch.float32, device=dev)
@@ -583,7 +585,9 @@ def main():
     mt = solver.march_tuner
     mkey = (rows, cols, float(tile.scale))
     march_info = {"autotuned": mt.enabled}
-    if mkey in mt.best:
+    if mt.fixed is not None:
+        march_info.update(chunk=mt.fixed[0], shape=["8x8", "4x16", "16x4"][mt.fixed[1]], split=mt.fixed[2])
+    elif mkey in mt.best:
         ch, sh, spl = mt.best[mkey]
         tm = mt.timings[mkey]
         march_info.update(chunk=ch, shape=["8x8", "4x16", "16x4"][sh], split=spl, tuned_ms=tm[mt.best[mkey]],

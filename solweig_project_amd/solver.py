@@ -418,14 +418,18 @@ class MarchTuner:
     mask word. On the first sweep of a tile shape it times every candidate on the scene itself
     (HIP events on the sweep's stream, best of ``reps``) and keeps the fastest for that
     (rows, cols, scale); every candidate writes the same masks. ``enabled`` False (or the
-    environment variable SW_MARCH_TUNE=0) keeps the defaults (chunk 8, 8 x 8 squares, split 1)."""
+    environment variable SW_MARCH_TUNE=0) keeps the defaults (chunk 8, 8 x 8 squares, split 1);
+    ``fixed`` = (chunk, shape, split) pins one candidate for every tile without timing."""
 
     CANDIDATES = tuple((c, sh, sp) for c in (8, 4, 12) for sh in (0, 1, 2) for sp in (1, 2))
     DEFAULT = (8, 0, 1)
 
-    def __init__(self, enabled: bool = True, reps: int = 2):
+    def __init__(self, enabled: bool = True, reps: int = 2, fixed: tuple | None = None):
         import os
-        self.enabled = enabled and os.environ.get("SW_MARCH_TUNE", "1") != "0"
+        if fixed is not None and tuple(fixed) not in self.CANDIDATES:
+            raise ValueError(f"march config {fixed} is not one of {self.CANDIDATES}")
+        self.fixed = None if fixed is None else tuple(fixed)
+        self.enabled = enabled and self.fixed is None and os.environ.get("SW_MARCH_TUNE", "1") != "0"
         self.reps = reps
         self.best = {}        # (rows, cols, scale) -> (chunk, shape, split)
         self.timings = {}     # (rows, cols, scale) -> {candidate: ms}
@@ -435,9 +439,11 @@ class MarchTuner:
         return L.MarchConfig(chunk=c[0], shape=c[1], split=c[2])
 
     def config(self, solver, ds, offs, tab, stream=None) -> L.MarchConfig:
-        key = (ds.rows, ds.cols, float(ds.tile.scale))
+        if self.fixed is not None:
+            return self.struct(self.fixed)
         if not self.enabled:
             return self.struct(self.DEFAULT)
+        key = (ds.rows, ds.cols, float(ds.tile.scale))
         if key not in self.best:
             self.best[key], self.timings[key] = self._tune(solver, ds, offs, tab, stream)
         return self.struct(self.best[key])
@@ -541,11 +547,12 @@ class _NoTimer:
 class HipSolver:
     """Drives the HIP kernels for one tile at a time on one GPU."""
 
-    def __init__(self, device: str | torch.device = "cuda", tune_march: bool = True):
+    def __init__(self, device: str | torch.device = "cuda", tune_march: bool = True,
+                 march_config: tuple | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("HipSolver needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device)
-        self.march_tuner = MarchTuner(enabled=tune_march)
+        self.march_tuner = MarchTuner(enabled=tune_march, fixed=march_config)
         self.lib = L.load()
         self.vault = SV.build_skyvault()
         v = self.vault

@@ -108,3 +108,16 @@ def test_argument_refusals_return_before_any_launch():
         L.call("sw_cnn_fwd", arr, 1, None)
     with pytest.raises(L.HipError, match="bad sizes"):
         L.call("sw_gate_write", fake, fake, fake, fake, fake, 1, 64, 16, 32, 16, 4, 8, 8, None)
+
+
+def test_march_tuner_pinned_configuration():
+    """A pinned march configuration is used for every tile without timing; one outside the
+    candidates is refused."""
+    pytest.importorskip("torch")
+    from solweig_project_amd.solver import MarchTuner
+    t = MarchTuner(fixed=(8, 0, 2))
+    assert not t.enabled
+    c = t.config(None, None, None, None)
+    assert (c.chunk, c.shape, c.split) == (8, 0, 2) and t.best == {}
+    with pytest.raises(ValueError):
+        MarchTuner(fixed=(16, 0, 1))
