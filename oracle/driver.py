@@ -58,7 +58,8 @@ def _declare(L):
     L.or_march_sun.argtypes = [vp, vp, vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, ci, cf, cf, vp, vp]
     L.or_gvf.argtypes = [vp, vp, vp, vp, vp, vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp, cf, cf, cf, cf, vp, vp]
     L.or_utci.argtypes = [vp, vp, vp, vp, cl, vp]
-    L.or_flux.argtypes = [ctypes.POINTER(OrStep), ctypes.POINTER(OrPatch), cl] + [vp] * 21
+    L.or_flux.argtypes = [ctypes.POINTER(OrStep), ctypes.POINTER(OrPatch), cl] + [vp] * 21 + [
+        ctypes.POINTER(OrOverride), vp]
     L.or_static.argtypes = [vp, vp, cl, vp, vp, vp]
     L.or_walk_sources.argtypes = [vp, vp, vp, vp, vp, vp, vp, cl, cf, cf, cf, cf, vp, vp, vp]
 
@@ -76,6 +77,19 @@ class OrPatch(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("lv", "lumchi", "yi_k", "yi_l", "lsky_down",
                                                "lsky_side", "l_sun", "alt", "ster", "cos_alt",
                                                "sin_alt", "angl")]
+
+
+class OrOverride(ctypes.Structure):
+    """Values given in place of the oracle's derivations (``or_override_t``): F_sh plane, packed
+    sunlit/shaded decisions (``stride`` bytes per pixel; Kside patch p at bit p, guarded longwave
+    patch p at bit ``kside_n + rank[p]``), night Lup plane. NULL members keep the derivation."""
+    _fields_ = [("fsh", ctypes.c_void_p), ("sun_pb", ctypes.c_void_p), ("shd_pb", ctypes.c_void_p),
+                ("stride", ctypes.c_int), ("kside_n", ctypes.c_int), ("rank", ctypes.c_void_p),
+                ("night_lup", ctypes.c_void_p)]
+
+
+#: planes of or_flux's optional o_extra output
+EXTRA_NAMES = ("lside", "kside", "ksidei", "ksided", "drad", "tgout", "ke", "ks", "kw", "kn")
 
 
 @dataclass
@@ -225,7 +239,7 @@ def day(sc: Scene, fields, shm, vm, bm, walls, aspect_deg, landcover, met, sun: 
                   _p(bu), _p(emis), _p(alb), _p(tg), _p(shm), _p(vm), _p(bm),
                   _p(np.ascontiguousarray(sh_sun)), _p(np.ascontiguousarray(vegsh_sun)), _p(gvf),
                   _p(tgmap), _p(o["tmrt"]), _p(o["utci"]), _p(o["kdown"]), _p(o["kup"]),
-                  _p(o["ldown"]), _p(o["lup"]), _p(o["shadow"]))
+                  _p(o["ldown"]), _p(o["lup"]), _p(o["shadow"]), None, None)
         for k in OUT_NAMES:
             outs[k][i] = o[k].reshape(rows, cols)
         outs["wbgt"][i] = WB.wbgt(outs["tmrt"][i], outs["shadow"][i], F32(sp.ta_mat), sp.wbgt_w, F32(sp.wbgt_a))

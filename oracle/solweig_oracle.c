@@ -433,11 +433,37 @@ typedef struct {
 } or_patch_t;
 
 /*
+ * Values a caller may supply in place of the oracle's own derivations. The reference's flat
+ * radiation-step bundle (solweig_core/numba_cpu/radiation.py:1266-1818, its pinned form in
+ * native/cuda/data/rad_pins.json) carries them precomputed:
+ *   fsh        F_sh plane (cylindric_wedge, solweig.py:404-449) instead of tan_svfalfa;
+ *   sun_pb/shd_pb  the shaded_or_sunlit decisions (solweig.py:524-562), `stride` bytes per
+ *              pixel, bit i of byte i / 8: Kside patch p at bit p, a guarded define_patch
+ *              patch p at bit kside_n + rank[p];
+ *   night_lup  night Lup plane instead of SBC emis (Ta + 273.15)^4.
+ * NULL members (or a NULL struct) keep the oracle's derivations.
+ */
+typedef struct {
+    const float *fsh;
+    const uint8_t *sun_pb, *shd_pb;
+    int stride, kside_n;
+    const int *rank;
+    const float *night_lup;
+} or_override_t;
+
+static inline float pbitf(const uint8_t *pb, long t, int stride, int i)
+{
+    return ((pb[t * stride + (i >> 3)] >> (i & 7)) & 1) ? 1.f : 0.f;
+}
+
+/*
  * Static per-pixel inputs: svfbuveg, tan_asvf, tan_svfalfa, buildings, emis, alb, tg (per-pixel
  * Tg of this step), masks sh/veg/vbsh cubes (NP, n).
  * day inputs: vegsh_sun/sh_sun (march), gvf (15 summed planes from or_gvf).
  * state (in/out): tgmap (6 planes: Lup, E, S, W, N, TgOut).
- * outputs: tmrt, utci, kdown, kup, ldown, lup, shadow.
+ * outputs: tmrt, utci, kdown, kup, ldown, lup, shadow; o_extra (NULL or 10 planes): Lside,
+ * Kside, KsideI, KsideD, dRad, TgOut, Keast, Ksouth, Kwest, Knorth (zero K terms and TgOut = Ta
+ * at night, as solweig.py:2183-2227 leaves them).
  */
 void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
              const float *svfbuveg, const float *tan_asvf, const float *tan_svfalfa,
@@ -445,8 +471,11 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
              const float *shm, const float *vegm, const float *vbm,
              const float *sh_sun, const float *vegsh_sun, const float *gvf,
              float *tgmap, float *o_tmrt, float *o_utci, float *o_kdown, float *o_kup,
-             float *o_ldown, float *o_lup, float *o_shadow)
+             float *o_ldown, float *o_lup, float *o_shadow, const or_override_t *ov, float *o_extra)
 {
+    const int given_fsh = ov && ov->fsh;
+    const int given_bits = ov && ov->sun_pb && ov->shd_pb;
+    const int given_lup = ov && ov->night_lup;
     const float rad2deg = (float)(180.0 / 3.14159265358979323846);
     const float pi_f = (float)3.14159265358979323846;
     const float two_pi = (float)(2.0 * 3.14159265358979323846);
@@ -454,6 +483,7 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
     for (long t = 0; t < n; ++t) {
         float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f, Kside = 0.f;
         float Lup, LupE, LupS, LupW, LupN, shadow = 0.f;
+        float KsideI = 0.f, KsideD = 0.f, dRad = 0.f, TgOut = sp->ta;
         const float sbc_emis = sp->sbc * emis[t];
         if (sp->is_day) {
             shadow = sh_sun[t] - (1.f - vegsh_sun[t]) * sp->one_m_psi;
@@ -481,25 +511,31 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
                 tgmap[(long)q * n + t] = T1;
             }
             Lup = L[0]; LupE = L[1]; LupS = L[2]; LupW = L[3]; LupN = L[4];
+            TgOut = L[5];
             /* cylinder wedge F_sh */
-            const float ta_ = tan_svfalfa[t], tb = sp->tan_zen;
-            const float xa = 1.f - 2.f / (ta_ * tb);
-            const float ha = 2.f / (ta_ * tb);
-            const float ba = 1.f / ta_;
-            const float hkil = 2.f * ba * ha;
-            float ukil = 0.f;
-            if (xa < 0.f) {
-                const float qa = tb / 2.f;
-                const float Za = f_sqrt(ba * ba - (qa * qa) / 4.f);
-                const float phi = f_atan(Za / qa);
-                const float A = (f_sin(phi) - phi * f_cos(phi)) / (1.f - f_cos(phi));
-                ukil = 2.f * ba * xa * A;
+            float F_sh;
+            if (given_fsh) {
+                F_sh = ov->fsh[t];
+            } else {
+                const float ta_ = tan_svfalfa[t], tb = sp->tan_zen;
+                const float xa = 1.f - 2.f / (ta_ * tb);
+                const float ha = 2.f / (ta_ * tb);
+                const float ba = 1.f / ta_;
+                const float hkil = 2.f * ba * ha;
+                float ukil = 0.f;
+                if (xa < 0.f) {
+                    const float qa = tb / 2.f;
+                    const float Za = f_sqrt(ba * ba - (qa * qa) / 4.f);
+                    const float phi = f_atan(Za / qa);
+                    const float A = (f_sin(phi) - phi * f_cos(phi)) / (1.f - f_cos(phi));
+                    ukil = 2.f * ba * xa * A;
+                }
+                const float Ssurf = hkil + ukil;
+                F_sh = (two_pi * ba - Ssurf) / (two_pi * ba);
+                if (F_sh != F_sh) F_sh = 0.5f;
             }
-            const float Ssurf = hkil + ukil;
-            float F_sh = (two_pi * ba - Ssurf) / (two_pi * ba);
-            if (F_sh != F_sh) F_sh = 0.5f;
             /* diffuse sky + Kside patch loop */
-            float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
+            float aniLum = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
             const float hsvf = tan_asvf[t];
             for (int p = 0; p < NP; ++p) {
                 const float sh = shm[(long)p * n + t], vs = vegm[(long)p * n + t], vb = vbm[(long)p * n + t];
@@ -509,13 +545,19 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
                 const float tvs = (vs == 0.f || vb == 0.f) ? 1.f : 0.f;
                 Kref_veg = Kref_veg + sp->shaded_k * tvs * pt->ster[p] * pt->cos_alt[p];
                 const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
-                const float sd = f_atan(hsvf + pt->yi_k[p]) * rad2deg;
-                const float sunlit = (sd < pt->alt[p]) ? 1.f : 0.f;
-                const float shaded = (sd > pt->alt[p]) ? 1.f : 0.f;
+                float sunlit, shaded;
+                if (given_bits) {
+                    sunlit = pbitf(ov->sun_pb, t, ov->stride, p);
+                    shaded = pbitf(ov->shd_pb, t, ov->stride, p);
+                } else {
+                    const float sd = f_atan(hsvf + pt->yi_k[p]) * rad2deg;
+                    sunlit = (sd < pt->alt[p]) ? 1.f : 0.f;
+                    shaded = (sd > pt->alt[p]) ? 1.f : 0.f;
+                }
                 Kref_sun = Kref_sun + sp->sunlit_k * sunlit * tsh * pt->ster[p] * pt->cos_alt[p];
                 Kref_sh = Kref_sh + sp->shaded_k * shaded * tsh * pt->ster[p] * pt->cos_alt[p];
             }
-            const float dRad = aniLum * sp->radD;
+            dRad = aniLum * sp->radD;
             const float sb = 1.f - svfbuveg[t];
             const float mix = sp->radG * (1.f - F_sh) + sp->radD * F_sh;
             Kdown = sp->radI * shadow * sp->sinalt + dRad + sp->albedo_b * sb * mix;
@@ -525,11 +567,11 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
             KupS = galbS * sp->radI * sp->sinalt + kdiff * gnoshS;
             KupW = galbW * sp->radI * sp->sinalt + kdiff * gnoshW;
             KupN = galbN * sp->radI * sp->sinalt + kdiff * gnoshN;
-            const float KsideI = shadow * sp->radI * sp->cosalt_k;
+            KsideI = shadow * sp->radI * sp->cosalt_k;
             Kside = KsideI + KsideD + Kref_sun + Kref_sh + Kref_veg;
         } else {
             const float x = sp->ta + 273.15f;
-            Lup = sbc_emis * t_powi(x, 4);
+            Lup = given_lup ? ov->night_lup[t] : sbc_emis * t_powi(x, 4);
             LupE = LupS = LupW = LupN = Lup;
         }
         /* longwave: define_patch_characteristics, two passes */
@@ -546,9 +588,15 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
             Ldown_veg = Ldown_veg + sp->veg_l * pt->ster[p] * pt->sin_alt[p] * tvs;
             const float tsh = ((1.f - sh) * vb == 1.f) ? 1.f : 0.f;
             if (pt->l_sun[p]) {
-                const float sd = f_atan(hsvf + pt->yi_l[p]) * rad2deg;
-                const float sunlit = (sd < pt->alt[p]) ? 1.f : 0.f;
-                const float shaded = (sd > pt->alt[p]) ? 1.f : 0.f;
+                float sunlit, shaded;
+                if (given_bits) {
+                    sunlit = pbitf(ov->sun_pb, t, ov->stride, ov->kside_n + ov->rank[p]);
+                    shaded = pbitf(ov->shd_pb, t, ov->stride, ov->kside_n + ov->rank[p]);
+                } else {
+                    const float sd = f_atan(hsvf + pt->yi_l[p]) * rad2deg;
+                    sunlit = (sd < pt->alt[p]) ? 1.f : 0.f;
+                    shaded = (sd > pt->alt[p]) ? 1.f : 0.f;
+                }
                 Lside_sun = Lside_sun + sp->sunlit_l * sunlit * pt->ster[p] * pt->cos_alt[p] * tsh;
                 Lside_sh = Lside_sh + sp->shaded_l * shaded * pt->ster[p] * pt->cos_alt[p] * tsh;
                 Ldown_sun = Ldown_sun + sp->sunlit_l * sunlit * pt->ster[p] * pt->sin_alt[p] * tsh;
@@ -575,6 +623,10 @@ void or_flux(const or_step_t *sp, const or_patch_t *pt, long n,
         const float tmrt = f_sqrt(f_sqrt(Sstr / (sp->absL * sp->sbc))) - 273.2f;
         o_tmrt[t] = tmrt;
         o_kdown[t] = Kdown; o_kup[t] = Kup; o_ldown[t] = Ldown; o_lup[t] = Lup; o_shadow[t] = shadow;
+        if (o_extra) {
+            const float ex[10] = {Lside, Kside, KsideI, KsideD, dRad, TgOut, Keast, Ksouth, Kwest, Knorth};
+            for (int q = 0; q < 10; ++q) o_extra[(long)q * n + t] = ex[q];
+        }
         if (bu[t] == 1.f) o_utci[t] = utci_point(sp->ta_mat, sp->rh, tmrt, sp->va);
         else o_utci[t] = NAN;
     }
