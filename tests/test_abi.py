@@ -121,3 +121,18 @@ def test_march_tuner_pinned_configuration():
     assert (c.chunk, c.shape, c.split) == (8, 0, 2) and t.best == {}
     with pytest.raises(ValueError):
         MarchTuner(fixed=(16, 0, 1))
+
+
+def test_build_flags_keep_one_rounding_per_operation():
+    """The device library and the oracle are built without FMA contraction, fast math or
+    flush-to-zero (the reference gates the same build mutations), and without packed-FP32
+    instruction selection."""
+    from solweig_project_amd import build as B
+    flags = B.HIPCC_FLAGS
+    assert "-ffp-contract=off" in flags and "-fno-fast-math" in flags
+    assert "-packed-fp32-ops" in flags and flags[flags.index("-packed-fp32-ops") - 1] == "-Xclang"
+    for bad in ("-ffast-math", "-ffp-contract=fast", "-ffp-contract=on", "-fgpu-flush-denormals-to-zero",
+                "-funsafe-math-optimizations", "-ffinite-math-only", "-O3 -ffast-math"):
+        assert bad not in flags, bad
+    make = (Path(__file__).resolve().parents[1] / "oracle" / "Makefile").read_text()
+    assert "-ffp-contract=off" in make and "-ffast-math" not in make and "-Ofast" not in make
