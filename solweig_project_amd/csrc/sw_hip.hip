@@ -1043,7 +1043,7 @@ struct PassA {
     float Lside_sky, Lside_veg, Lside_sun, Lside_sh;
 };
 
-template <bool DAY>
+template <bool DAY, bool ROLLED>
 __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, const uint32_t *msh,
                                              const uint32_t *mvs, const uint32_t *mv1, const uint32_t *mv2,
                                              float hsvf, PassA &o) {
@@ -1061,8 +1061,7 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
         uint32_t btvs = ~avs | ~(av1 | av2), btsh = ~ash & av1, btsky = ash & avs;
         uint32_t bsh = ash, bvs = avs;
         const int pend = min(w * 32 + 32, SW_NPATCH);
-#pragma unroll 4
-        for (int p = w * 32; p < pend; ++p) {
+        auto patch = [&](int p) {
             const float tvs = (float)(btvs & 1u), tsky = (float)(btsky & 1u);
             const bool tsh = btsh & 1u;
             const bool sh = bsh & 1u, vs = bvs & 1u;
@@ -1099,6 +1098,18 @@ __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, c
                 Ldown_sun = __builtin_fmaf(sunlit, u.ls_down, Ldown_sun);
                 Ldown_sh = __builtin_fmaf(shaded, u.lh_down, Ldown_sh);
             }
+        };
+        // ROLLED (the walking kernel, which holds both passes): no unrolling. Unrolled, its
+        // iterations keep several patches' scalar constants live at once; the register
+        // allocator then spills SGPRs to VGPR lanes and holds 114 VGPRs (4 waves per SIMD).
+        // Rolled, it needs 68 VGPRs (7 waves per SIMD), spills nothing and runs 7 % faster.
+        // The night-only kernel keeps 4 (its rolled form is slower).
+        if (ROLLED) {
+#pragma unroll 1
+            for (int p = w * 32; p < pend; ++p) patch(p);
+        } else {
+#pragma unroll 4
+            for (int p = w * 32; p < pend; ++p) patch(p);
         }
     }
     o.aniLum = aniLum; o.KsideD = KsideD; o.Kref_sun = Kref_sun; o.Kref_sh = Kref_sh; o.Kref_veg = Kref_veg;
@@ -1368,8 +1379,8 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     load_masks(masks, n, t, msh, mvs, mv1, mv2);
     const float hsvf = tan_asvf[t];
     PassA pa;
-    if (day) patch_pass_a<true>(sp, msh, mvs, mv1, mv2, hsvf, pa);
-    else patch_pass_a<false>(sp, msh, mvs, mv1, mv2, hsvf, pa);
+    if (day) patch_pass_a<true, WALK>(sp, msh, mvs, mv1, mv2, hsvf, pa);
+    else patch_pass_a<false, WALK>(sp, msh, mvs, mv1, mv2, hsvf, pa);
     const float aniLum = pa.aniLum, KsideD = pa.KsideD, Kref_sun = pa.Kref_sun, Kref_sh = pa.Kref_sh,
                 Kref_veg = pa.Kref_veg;
     const float Ldown_sky = pa.Ldown_sky, Ldown_veg = pa.Ldown_veg, Ldown_sun = pa.Ldown_sun, Ldown_sh = pa.Ldown_sh;
