@@ -108,6 +108,12 @@ def test_argument_refusals_return_before_any_launch():
         L.call("sw_cnn_fwd", arr, 1, None)
     with pytest.raises(L.HipError, match="bad sizes"):
         L.call("sw_gate_write", fake, fake, fake, fake, fake, 1, 64, 16, 32, 16, 4, 8, 8, None)
+    for bad in ((5, 0, 1), (8, 3, 1), (8, 0, 3), (12, -1, 2)):
+        cfg = L.MarchConfig(chunk=bad[0], shape=bad[1], split=bad[2])
+        with pytest.raises(L.HipError, match="march configuration"):
+            L.call("sw_svf_march_cfg", *[fake] * 7, 8, 8, ctypes.byref(cfg), None)
+        with pytest.raises(L.HipError, match="march configuration"):
+            L.call("sw_svf_sweep_cfg", *[fake] * 14, 8, 8, ctypes.byref(cfg), None)
 
 
 def test_march_tuner_pinned_configuration():
