@@ -3,7 +3,8 @@
 One-row and one-column strips, 2 x 2 and 3 x 3 tiles, a 17 x 33 tile (no side a
 multiple of the 16 x 16 work tiles) and a flat tile with no buildings or trees
 (amax 0): the march, fold, sun march, ground walk and flux kernels all run
-their edge paths. Bars as for the parity tests: masks and shadow bit-exact,
+their edge paths. Two polar days at 69 N: the midnight sun (24 day steps down to
+2.7 degrees altitude) and the polar night (no day step). Bars as for the parity tests: masks and shadow bit-exact,
 SVF within 1e-5, Tmrt within 0.01 K, UTCI within 0.01 C with the same NaN
 pattern, fluxes within 0.01 W/m^2.
 """
@@ -58,5 +59,37 @@ def test_edge_tiles_match_oracle(shape):
     assert np.array_equal(np.isnan(out["utci"]), ~m)
     if m.any():
         assert np.abs(out["utci"][m] - o["utci"][m]).max() <= 0.01
+    for k in ("kdown", "kup", "ldown", "lup"):
+        assert np.abs(out[k] - o[k]).max() <= 0.01, k
+
+
+@pytest.mark.parametrize("doy", [172, 355], ids=["midnight_sun", "polar_night"])
+def test_polar_days_match_oracle(doy):
+    """69 N: on DOY 172 the sun stays up for all 24 steps (down to 2.7 degrees, the longest sun
+    rays and walk reach); on DOY 355 it never rises (every step takes the night branch)."""
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    from oracle import driver as D
+    from solweig_project_amd import sunpos, synth
+    from solweig_project_amd.solver import HipSolver, compose_scene
+    print(f"[gpu-test] polar day DOY {doy}", flush=True)
+    lat = 69.0
+    site = synth.make_site(48, 44, seed=5)
+    sun = sunpos.day_sun_table(lat, 18.9, 1, 2021, doy)
+    assert int((sun.altitude > 0).sum()) == (24 if doy == 172 else 0)
+    met = synth.make_met(sun.altitude, seed=6, doy=doy)
+    sc = D.compose(site.dsm, site.dem, site.cdsm, site.scale)
+    fields_o, shm, vm, bm = D.svf(sc)
+    o = D.day(sc, fields_o, shm, vm, bm, site.walls, site.aspect, site.landcover, met, sun, lat)
+    solver = HipSolver("cuda:0")
+    scene = compose_scene(site.dsm, site.dem, site.cdsm, site.walls, site.aspect, site.landcover, scale=site.scale)
+    outs = ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup")
+    fields, out, _ = solver.solve(scene, met, sun, lat, outputs=outs)
+    assert np.abs(fields - fields_o).max() <= 1e-5
+    assert np.array_equal(out["shadow"], o["shadow"])
+    assert np.abs(out["tmrt"] - o["tmrt"]).max() <= 0.01
+    m = ~np.isnan(o["utci"])
+    assert np.array_equal(np.isnan(out["utci"]), ~m)
+    assert np.abs(out["utci"][m] - o["utci"][m]).max() <= 0.01
     for k in ("kdown", "kup", "ldown", "lup"):
         assert np.abs(out[k] - o[k]).max() <= 0.01, k
