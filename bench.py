@@ -40,8 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="side of the CPU-oracle tile (0: the benchmark tile itself)")
-    ap.add_argument("--workload", choices=("c2", "c3", "edits", "surrogate", "serve"), default="c2",
-                    help="c2: one 1000x1000 tile (default); c3: batched tiles 300/700/1200/2000 squared plus "
+    ap.add_argument("--workload", choices=("tile", "batch", "edits", "surrogate", "serve"), default="tile",
+                    help="tile: one 1000x1000 tile (default); batch: batched tiles 300/700/1200/2000 squared plus "
                          "64 edit scenarios of a 500x500 base; edits: single-tree edits of a 2000x2000 site "
                          "(p50/p99 latency); surrogate: train the SVF surrogate, then gated surrogate vs "
                          "exact SVF on a 2000x2000 city; serve: Poisson mix of edits, met-only and full "
@@ -49,9 +49,9 @@ def parse():
     ap.add_argument("--requests", type=int, default=1000, help="requests served by --workload serve")
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
     ap.add_argument("--lanes", type=int, default=2,
-                    help="c2: consecutive solves alternate over this many scene copies on their own streams")
+                    help="tile workload: consecutive solves alternate over this many scene copies on their own streams")
     ap.add_argument("--march", default="",
-                    help="c2: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
+                    help="tile workload: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
     ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
     ap.add_argument("--chunk-steps", type=int, default=4, help="day steps a serving slot enqueues at a time")
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
@@ -134,7 +134,7 @@ def c3_inputs(seed, dev):
     return tiles, edits, base
 
 
-def run_c3(args):
+def run_batch(args):
     """Batched exact solves: every group runs its SVF sweeps, then one sw_solve_batch day loop."""
     import numpy as np
     import torch
@@ -143,7 +143,7 @@ def run_c3(args):
     from solweig_project_amd.solver import DayBatch, HipSolver
 
     dev = torch.device("cuda:0")
-    log("generating C3 inputs (walls/aspect on the device)")
+    log("generating the batch inputs (walls/aspect on the device)")
     tiles, edits, base = c3_inputs(args.seed, dev)
     sun = sun_for_site(base.latitude, base.longitude, base.utc_offset, 2021, 180)
     met = synth.make_met(sun.altitude, seed=args.seed + 1)
@@ -224,13 +224,13 @@ def run_c3(args):
     tot_px = sum(px.values())
     tot_ms = sum(r["ms"] for r in res.values())
     line = {
-        "metric": "Mpixel-steps/s (C3: batched exact solves, 4 uneven tiles + 64 edit scenarios of 500², 24 h)",
+        "metric": "Mpixel-steps/s (batched exact solves, 4 uneven tiles + 64 edit scenarios of 500², 24 h)",
         "value": round(tot_px * T / (tot_ms / 1e3) / 1e6, 3),
         "unit": "Mpixel-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(tot_ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded city tiles + edit scenarios; inputs resident in HBM, outputs left in HBM)",
-        "config": {"workload": "C3 batch: tiles 300², 700², 1200², 2000² and 64 edit scenarios (1-20 tree or "
+        "config": {"workload": "batch: tiles 300², 700², 1200², 2000² and 64 edit scenarios (1-20 tree or "
                                "building edits of 10-400 px) of a 500² base, 24 hourly steps, one plan per tile "
                                "shape, SVF sweep per item, one sun-march and one flux launch per step for all "
                                "items", "time_steps": T},
@@ -461,7 +461,7 @@ def run_surrogate(args):
 
 
 def run_serve(args, pipeline):
-    """C5: Poisson arrivals (60 % single-tree edits, 30 % met-only, 10 % full re-solves) against a
+    """Serving mix: Poisson arrivals (60 % single-tree edits, 30 % met-only, 10 % full re-solves) against a
     device-resident 2000 x 2000 x 24 h site; latency = completion - arrival, queueing included."""
     import numpy as np
     import torch
@@ -500,7 +500,7 @@ def run_serve(args, pipeline):
     T = st.plan.n_steps
     px_steps = sum(n * n * T for r in done if r.kind in ("met", "full"))
     line = {
-        "metric": "p99 request latency (ms), C5 serving mix on a 2000² x 24 h site",
+        "metric": "p99 request latency (ms), serving mix on a 2000² x 24 h site",
         "value": allp["p99_ms"], "unit": "ms",
         "n_gpus": 1, "steps": len(done), "warmup": len(warm), "ms_per_step": allp["mean_ms"],
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
@@ -529,8 +529,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
               "design-only (tiles with read-only halos, no collectives)", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "c3":
-        run_c3(args)
+    if args.workload == "batch":
+        run_batch(args)
         return
     if args.workload == "edits":
         run_edits(args)

@@ -119,7 +119,7 @@ def test_utci_kernel_matches_reference(solver, golden_dir):
 
 
 @pytest.fixture(scope="module")
-def c1_case():
+def tile200_case():
     """The seeded 200 x 200 x 24 h generator tile (30 N, DOY 180) with the oracle's answer."""
     from oracle import driver as D
     from solweig_project_amd import synth
@@ -135,19 +135,19 @@ def c1_case():
     return dict(site=site, sun=sun, met=met, fields=fields, cubes=(shm, vm, bm), outs=outs)
 
 
-def test_c1_matches_oracle(solver, c1_case):
+def test_200x200_day_matches_oracle(solver, tile200_case):
     from solweig_project_amd.solver import compose_scene
-    _say("c1 200x200x24 vs oracle")
-    s = c1_case["site"]
+    _say("200x200x24 day vs oracle")
+    s = tile200_case["site"]
     scene = compose_scene(s.dsm, s.dem, s.cdsm, s.walls, s.aspect, s.landcover, scale=s.scale)
-    fields, out, ds = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude,
+    fields, out, ds = solver.solve(scene, tile200_case["met"], tile200_case["sun"], s.latitude,
                                    outputs=("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup", "wbgt"))
     cubes = [c.cpu().numpy() for c in solver.unpack_masks(ds)]
     rows, cols = scene.shape
-    for got, ref in zip(cubes, c1_case["cubes"]):
+    for got, ref in zip(cubes, tile200_case["cubes"]):
         assert np.array_equal(got, ref.reshape(153, rows, cols).transpose(1, 2, 0))
-    assert np.abs(fields - c1_case["fields"]).max() <= 1e-5
-    o = c1_case["outs"]
+    assert np.abs(fields - tile200_case["fields"]).max() <= 1e-5
+    o = tile200_case["outs"]
     assert np.array_equal(out["shadow"], o["shadow"])
     assert np.abs(out["tmrt"] - o["tmrt"]).max() <= 0.01
     m = ~np.isnan(o["utci"])
@@ -157,12 +157,12 @@ def test_c1_matches_oracle(solver, c1_case):
     assert np.abs(out["wbgt"] - o["wbgt"]).max() <= 0.01
 
 
-def test_repeat_runs_bitwise_identical(solver, c1_case):
+def test_repeat_runs_bitwise_identical(solver, tile200_case):
     from solweig_project_amd.solver import compose_scene
-    s = c1_case["site"]
+    s = tile200_case["site"]
     scene = compose_scene(s.dsm, s.dem, s.cdsm, s.walls, s.aspect, s.landcover, scale=s.scale)
-    f1, o1, _ = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude)
-    f2, o2, _ = solver.solve(scene, c1_case["met"], c1_case["sun"], s.latitude)
+    f1, o1, _ = solver.solve(scene, tile200_case["met"], tile200_case["sun"], s.latitude)
+    f2, o2, _ = solver.solve(scene, tile200_case["met"], tile200_case["sun"], s.latitude)
     assert np.array_equal(f1, f2)
     for k in o1:
         assert np.array_equal(o1[k], o2[k], equal_nan=True), k
