@@ -86,7 +86,8 @@ class PlanPipeline:
 
 def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci", "shadow"), out=None,
                stream=None, lanes=None):
-    """Solve every job on the scene ``ds`` (SVF sweep + day), planning one job ahead.
+    """Solve every job on the scene ``ds`` (SVF sweep + day), planning as many jobs ahead as the
+    pipeline has planner processes (a plan takes longer on the host than a solve on the device).
 
     ``out`` receives each job's outputs in turn (the last job's remain);
     returns the device output tensor. The host never waits for the GPU.
@@ -99,15 +100,17 @@ def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci"
     if not jobs:
         return out
     marks = []
-    pending = pipeline.submit(ds.tile, jobs[0])
+    ahead = max(1, int(getattr(pipeline, "workers", 1)))
+    pending = [pipeline.submit(ds.tile, j) for j in jobs[:ahead]]
     for k, job in enumerate(jobs):
         if lanes:
             ds, stream, out = lanes[k % len(lanes)]
         t0 = time.perf_counter()
         solver.svf(ds, stream)
-        nxt = pipeline.submit(ds.tile, jobs[k + 1]) if k + 1 < len(jobs) else None
+        if k + ahead < len(jobs):
+            pending.append(pipeline.submit(ds.tile, jobs[k + ahead]))
         t1 = time.perf_counter()
-        plan = pending.get()
+        plan = pending.pop(0).get()
         t2 = time.perf_counter()
         if hasattr(solver, "plan_ms") and hasattr(plan, "plan_ms"):
             solver.plan_ms.append(plan.plan_ms)
@@ -117,6 +120,5 @@ def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci"
         out = solver.run_day(ds, plan, dplan, outputs, out=out, stream=stream)
         t4 = time.perf_counter()
         marks.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3))
-        pending = nxt
     solver.pipeline_marks = marks
     return out
