@@ -7,7 +7,7 @@ view factors, flux assembly, Tmrt and UTCI) with the input rasters already
 resident in HBM and the outputs (16 SVF fields, per-step Tmrt, UTCI and
 shadow) left in HBM. The host scalar chain of each solve's day (clearness
 index, Perez luminance, sun tables, step records) is computed inside the
-timed region, in a planner process one solve ahead of the GPU. Prints one
+timed region, in planner processes two solves ahead of the GPU. Prints one
 JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -600,7 +600,7 @@ def main():
     solver.plan_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    # every solve plans its own day (in the planner process, one solve ahead)
+    # every solve plans its own day (in the planner processes, two solves ahead)
     solve_many(solver, ds, [job] * args.steps, pipeline, out=out, lanes=lanes)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
