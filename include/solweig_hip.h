@@ -264,6 +264,18 @@ int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad,
                 const uint8_t *cls, const float *buildings, const sw_step_t *steps,
                 const sw_model_t *model, const int32_t *sun_tab, const uint32_t *stats,
                 uint32_t *src, int step, int rows, int cols, void *stream);
+/* sw_sun_step that also writes the reference's eight output planes of the
+ * march and wall tail, the return tuple of shadowingfunction_wallheight_23
+ * (solweig.py:1187-1224): planes[q * n + pixel] for q = 0 vegsh, 1 sh,
+ * 2 vbshvegsh, 3 wallsh, 4 wallsun, 5 wallshve, 6 facesh, 7 facesun
+ * (float32, 8 * n floats). The march, its exits and the tail are the ones
+ * the day loop runs (same kernel body); src is written as by sw_sun_step.
+ * Replaces the native lane's sw_run_march_wallheight23 + wall tail
+ * (native/cuda/src/sw_api.cu:219-228). */
+int sw_sun_step_planes(const void *hpack, const float *walls, const float *aspect_rad,
+                       const uint8_t *cls, const float *buildings, const sw_step_t *steps,
+                       const sw_model_t *model, const int32_t *sun_tab, const uint32_t *stats,
+                       uint32_t *src, float *planes, int step, int rows, int cols, void *stream);
 
 /* Flux assembly -> Tmrt -> UTCI for one step (Solweig_2022a_calc,
  * solweig.py:2101-2311; utci_calculator, calculate_utci.py:259-330).

@@ -182,6 +182,7 @@ SIGNATURES = {
     "sw_run_march_wallheight23": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
     "sw_tile_static": (_I, [_VP] * 8 + [_I, _I, _VP]),
     "sw_sun_step": (_I, [_VP] * 10 + [_I, _I, _I, _VP]),
+    "sw_sun_step_planes": (_I, [_VP] * 11 + [_I, _I, _I, _VP]),
     "sw_flux_step": (_I, [_VP] * 12 + [_I, _I, _I, _I, _I, _VP]),
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
     "sw_svf_march_cfg": (_I, [_VP] * 7 + [_I, _I, _VP, _VP]),

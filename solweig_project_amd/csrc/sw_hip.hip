@@ -730,15 +730,17 @@ __global__ void k_march_svf_single(const float *__restrict__ a, const float *__r
 struct SunOut {
     int sh, vs, vb;
     float shvo, shvoveg;  // f - a and the vegetation shadow volume (before * vegsh)
+    int stop;             // TRACK: the step after which the lane left the march (count if it never left)
 };
 
-template <bool EXITS>
+template <bool EXITS, bool TRACK = false>
 __device__ __forceinline__ SunOut march_sun_pixel(const float4 *__restrict__ hp, int rows, int cols, int r,
                                                   int c, float at, float vegt, bool valid, bool wall,
                                                   const int4 *__restrict__ tab, int count, float amax,
                                                   float hmax) {
     float f = at, svv = vegt;
     int sh = 0, vs = 0, vb = 0;
+    int stop = count;
     bool active = valid;
     const bool exit_ok = EXITS && (at >= 0.f) && (vegt >= 0.f);
     for (int s = 0; s < count; ++s) {
@@ -772,13 +774,33 @@ __device__ __forceinline__ SunOut march_sun_pixel(const float4 *__restrict__ hp,
                 if (exit_ok && !inb) active = false;
                 else if (!wall && sh == 1 && vs == 0) active = false;
                 else if (!wall && exit_ok && (hmax - dzp) <= at) active = false;
+                if (TRACK && !active) stop = s;
             }
         }
         if (!(amax >= __int_as_float(tab[s].z))) break;
     }
     SunOut o;
-    o.sh = sh; o.vs = vs; o.vb = vb; o.shvo = f - at; o.shvoveg = svv;
+    o.sh = sh; o.vs = vs; o.vb = vb; o.shvo = f - at; o.shvoveg = svv; o.stop = stop;
     return o;
+}
+
+// The vegetation shadow volume over the steps a lane skipped after its exit (steps stop + 1 ..
+// under the same amax rule). The exits are exact for every plane the day loop reads; the
+// reference's wallshve plane on a pixel without a wall is (shvoveg - a) * 0, a zero whose sign
+// still records whether the complete running max reached the pixel's height.
+__device__ __forceinline__ float sun_veg_volume_rest(const float4 *__restrict__ hp, int rows, int cols, int r,
+                                                     int c, float svv, const int4 *__restrict__ tab, int count,
+                                                     int stop, float amax) {
+    for (int s = stop + 1; s < count; ++s) {
+        if (!(amax >= __int_as_float(tab[s - 1].z))) break;
+        const int4 e = tab[s];
+        const int sr = r + e.x, sc = c + e.y;
+        float tv = 0.f;
+        if (((unsigned)sr < (unsigned)rows) && ((unsigned)sc < (unsigned)cols))
+            tv = hp[(long)sr * cols + sc].y - __int_as_float(e.z);
+        svv = fmaxf(svv, tv);
+    }
+    return svv;
 }
 
 // ---------------------------------------------------------------------------
@@ -811,14 +833,16 @@ __device__ __forceinline__ float pow4(float x) {
 #define FLUX_TX 16
 #define FLUX_TY 16
 
-// one 256-pixel block of the sun step; `tile` is the block's index within its tile
-template <bool TILED>
+// one 256-pixel block of the sun step; `tile` is the block's index within its tile.
+// PLANES also writes the reference's eight output planes of the march and wall tail
+// (vegsh, sh, vbshvegsh, wallsh, wallsun, wallshve, facesh, facesun) to planes[q * n + t].
+template <bool TILED, bool PLANES = false>
 __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, const float *__restrict__ walls,
                                               const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
                                               const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
                                               const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
                                               const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step,
-                                              int rows, int cols, int tile) {
+                                              int rows, int cols, int tile, float *__restrict__ planes = nullptr) {
     const long n = (long)rows * cols;
     const float amax = funkey(stats[0]);
     const float hmax = funkey(stats[1]);
@@ -849,13 +873,17 @@ __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, con
         vegt = h.y;
         w = walls[t];
     }
-    const SunOut o = march_sun_pixel<true>(hp, rows, cols, r, c, at, vegt, valid, w > 0.f,
-                                           suntab + sp->sun_off, sp->sun_count, amax, hmax);
+    const SunOut o = march_sun_pixel<true, PLANES>(hp, rows, cols, r, c, at, vegt, valid, w > 0.f,
+                                                   suntab + sp->sun_off, sp->sun_count, amax, hmax);
     if (!valid) return;
     // finalisation (solweig.py:1198-1219)
     const int shv = 1 - o.sh;
     const int vsf = o.vs > 0 ? 1 : o.vs;
-    const float svv = (o.shvoveg - at) * (float)vsf;
+    float shvoveg = o.shvoveg;
+    if (PLANES && o.stop < sp->sun_count)
+        shvoveg = sun_veg_volume_rest(hp, rows, cols, r, c, shvoveg, suntab + sp->sun_off, sp->sun_count, o.stop,
+                                      amax);
+    const float svv = (shvoveg - at) * (float)vsf;
     const int vso = 1 - vsf;
     const float wallbol = (w > 0.f) ? 1.f : 0.f;
     const float asp = aspect[t];
@@ -871,6 +899,21 @@ __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, con
     if (wallshve < 0.f) wallshve = 0.f;
     wallsun = wallsun - wallshve;
     if (wallsun < 0.f) wallsun = 0.f;
+    if (PLANES) {
+        // the tail's remaining outputs (solweig.py:1207, 1221); the walk reads only wallsun
+        const float facesun = ((facesh + wallbol == 1.f) && (w > 0.f)) ? 1.f : 0.f;
+        const float wallshve_out = (wallshve > w) ? w : wallshve;
+        int vb = o.vb > 0 ? 1 : o.vb;
+        vb = vb - o.vs;
+        planes[t] = (float)vso;
+        planes[n + t] = (float)shv;
+        planes[2 * n + t] = (float)(1 - vb);
+        planes[3 * n + t] = wallsh;
+        planes[4 * n + t] = wallsun;
+        planes[5 * n + t] = wallshve_out;
+        planes[6 * n + t] = facesh;
+        planes[7 * n + t] = facesun;
+    }
     // walk sources (sunonsurface_2018a prologue)
     const float b = bu[t];
     const int k = cls[t];
@@ -887,14 +930,16 @@ __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, con
     src[t] = make_uint2(__float_as_uint(lup), flags);
 }
 
+template <bool PLANES>
 __global__ void __launch_bounds__(256)
 k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
            const float *__restrict__ aspect, const uint8_t *__restrict__ cls,
            const float *__restrict__ bu, const sw_step_t *__restrict__ steps,
            const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
-           const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols) {
-    sun_step_body<false>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
-                         xcd_tile(blockIdx.x, gridDim.x));
+           const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols,
+           float *__restrict__ planes) {
+    sun_step_body<false, PLANES>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
+                                 xcd_tile(blockIdx.x, gridDim.x), planes);
 }
 
 // the sun step over listed 16 x 16 tiles
@@ -1827,19 +1872,48 @@ int sw_tile_static(const float *fields, const float *walls, const float *aspect_
     return hip_check(hipGetLastError(), "k_walk_face");
 }
 
+static int sun_step(const void *hpack, const float *walls, const float *aspect_rad, const uint8_t *cls,
+                    const float *buildings, const sw_step_t *steps, const sw_model_t *model, const int32_t *sun_tab,
+                    const uint32_t *stats, uint32_t *src, float *planes, int step, int rows, int cols, void *stream,
+                    const char *who) {
+    int rc = check_shape(rows, cols, who);
+    if (rc) return rc;
+    char buf[96];
+    if (!hpack || !walls || !aspect_rad || !cls || !buildings || !steps || !model || !sun_tab || !stats || !src) {
+        snprintf(buf, sizeof(buf), "%s: NULL pointer", who);
+        return set_err(SW_E_ARG, buf);
+    }
+    if (step < 0) {
+        snprintf(buf, sizeof(buf), "%s: negative step", who);
+        return set_err(SW_E_ARG, buf);
+    }
+    const long n = (long)rows * cols;
+    if (planes) {
+        hipLaunchKernelGGL(k_sun_step<true>, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack,
+                           walls, aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src,
+                           step, rows, cols, planes);
+        return hip_check(hipGetLastError(), "k_sun_step<planes>");
+    }
+    hipLaunchKernelGGL(k_sun_step<false>, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack, walls,
+                       aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src, step,
+                       rows, cols, (float *)nullptr);
+    return hip_check(hipGetLastError(), "k_sun_step");
+}
+
 int sw_sun_step(const void *hpack, const float *walls, const float *aspect_rad, const uint8_t *cls,
                 const float *buildings, const sw_step_t *steps, const sw_model_t *model, const int32_t *sun_tab,
                 const uint32_t *stats, uint32_t *src, int step, int rows, int cols, void *stream) {
-    int rc = check_shape(rows, cols, "sw_sun_step");
-    if (rc) return rc;
-    if (!hpack || !walls || !aspect_rad || !cls || !buildings || !steps || !model || !sun_tab || !stats || !src)
-        return set_err(SW_E_ARG, "sw_sun_step: NULL pointer");
-    if (step < 0) return set_err(SW_E_ARG, "sw_sun_step: negative step");
-    const long n = (long)rows * cols;
-    hipLaunchKernelGGL(k_sun_step, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack, walls,
-                       aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src, step,
-                       rows, cols);
-    return hip_check(hipGetLastError(), "k_sun_step");
+    return sun_step(hpack, walls, aspect_rad, cls, buildings, steps, model, sun_tab, stats, src, nullptr, step, rows,
+                    cols, stream, "sw_sun_step");
+}
+
+int sw_sun_step_planes(const void *hpack, const float *walls, const float *aspect_rad, const uint8_t *cls,
+                       const float *buildings, const sw_step_t *steps, const sw_model_t *model,
+                       const int32_t *sun_tab, const uint32_t *stats, uint32_t *src, float *planes, int step,
+                       int rows, int cols, void *stream) {
+    if (!planes) return set_err(SW_E_ARG, "sw_sun_step_planes: NULL planes");
+    return sun_step(hpack, walls, aspect_rad, cls, buildings, steps, model, sun_tab, stats, src, planes, step, rows,
+                    cols, stream, "sw_sun_step_planes");
 }
 
 int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuveg, const float *tan_asvf,

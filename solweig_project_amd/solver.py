@@ -34,6 +34,8 @@ F32 = np.float32
 #: output plane bits of sw_flux_step, in packing order
 OUT_BITS = {"tmrt": 1, "utci": 2, "shadow": 4, "kdown": 8, "kup": 16, "ldown": 32, "lup": 64, "wbgt": 128}
 OUT_ORDER = ("tmrt", "utci", "shadow", "kdown", "kup", "ldown", "lup", "wbgt")
+#: planes of HipSolver.sun_planes, in the order shadowingfunction_wallheight_23 returns them
+SUN_PLANES = ("vegsh", "sh", "vbsh", "wallsh", "wallsun", "wallshve", "facesh", "facesun")
 
 
 class UnsupportedInput(ValueError):
@@ -659,6 +661,20 @@ class HipSolver:
         P = L.ptr
         L.call("sw_tile_static", P(ds.fields), P(ds.walls), P(ds.aspect), P(dplan["model"]), P(ds.svfbuveg),
                P(ds.tan_asvf), P(ds.tan_sa), P(ds.walkface), ds.rows, ds.cols, _stream_ptr(stream))
+
+    def sun_planes(self, ds: DeviceScene, plan: DayPlan, dplan, step: int, stream=None):
+        """The day loop's sun march and wall tail of daytime ``step`` with the reference's eight
+        output planes (``shadowingfunction_wallheight_23``'s return tuple, ``solweig.py:1187-1224``):
+        a (8, rows, cols) device tensor in the order :data:`SUN_PLANES`. Also rewrites the scene's
+        walk sources as the day loop's sun step does."""
+        if not plan.is_day[step]:
+            raise ValueError(f"step {step} is a night step: no sun march")
+        planes = torch.empty((len(SUN_PLANES), ds.rows, ds.cols), dtype=torch.float32, device=self.device)
+        P = L.ptr
+        L.call("sw_sun_step_planes", P(ds.hpack), P(ds.walls), P(ds.aspect), P(ds.cls), P(ds.buildings),
+               P(dplan["steps"]), P(dplan["model"]), P(dplan["sun_tab"]), P(ds.stats), P(ds.src), P(planes),
+               int(step), ds.rows, ds.cols, _stream_ptr(stream))
+        return planes
 
     def run_day(self, ds: DeviceScene, plan: DayPlan, dplan, outputs=("tmrt", "utci", "shadow"),
                 out=None, stream=None, reset_state=True, per_launch=False, steps=None):

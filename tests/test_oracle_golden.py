@@ -53,7 +53,8 @@ def test_sun_march_planes_bit_exact(solved):
     assert len(solved["rec"]) == d["m_sh"].shape[0]
     for j, r in enumerate(solved["rec"]):
         for q, nm in enumerate(names):
-            assert np.array_equal(r["sun"][q], d["m_" + nm][j]), (j, nm)
+            # bit for bit, signed zeros included
+            assert np.array_equal(r["sun"][q].view(np.uint32), d["m_" + nm][j].view(np.uint32)), (j, nm)
 
 
 def test_day_outputs(solved):
@@ -95,3 +96,42 @@ def test_planted_dz_error_is_caught(golden_dir):
         moved = D.march_svf_patch(sc, tab)
         flipped += sum(int((x != y).sum()) for x, y in zip(base, moved))
     assert flipped > 0
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_recorded_sun_tables_reproduce_the_reference_planes(golden_dir, name):
+    """The day-loop tables of ``sun_tabs_day.npz`` (grid-edge tables, amax stop applied by the
+    march) give the reference's eight planes of every daytime step through the oracle, bit for bit."""
+    g, d = _load(golden_dir, name)
+    z = np.load(golden_dir / "sun_tabs_day.npz")
+    sc = D.compose(g["dsm"], g["dem"], g["cdsm"], float(g["scale"]))
+    rows, cols = sc.a.shape
+    n = rows * cols
+    walls = np.ascontiguousarray(g["walls"], np.float32)
+    asp = ((np.ascontiguousarray(g["aspect"], np.float32) * np.float32(np.pi)) / np.float32(180.0)).astype(np.float32)
+    sun = M.SunTable(d["altitude"][0], d["azimuth"][0], d["zen"][0], d["jday"][0], d["dectime"], d["altmax"][0])
+    loop = M.DayLoop(d["met"], sun, latitude=float(d["latitude"]), landcover=M.landcover_constants(True),
+                     scale=sc.scale, amax=sc.amax, rows=rows, cols=cols)
+    tab = z[f"{name}_sun_tab"]
+    names = ["vegsh", "sh", "vbsh", "wallsh", "wallsun", "wallshve", "facesh", "facesun"]
+    j = 0
+    out = np.empty(8 * n, np.float32)
+    work = np.empty(25 * n, np.float32)
+    for i in np.nonzero(z[f"{name}_is_day"])[0]:
+        sp = loop.step(int(i))
+        o, c = int(z[f"{name}_sun_off"][i]), int(z[f"{name}_sun_count"][i])
+        t = tab[o:o + c]
+        dz = np.ascontiguousarray(t[:, 2]).view(np.float32)
+        stop = np.nonzero(~(np.float32(sc.amax) >= dz))[0]
+        cnt = int(stop[0]) + 1 if stop.size else c
+        dx, dy = (np.ascontiguousarray(t[:cnt, k], np.int32) for k in (0, 1))
+        dzc = np.ascontiguousarray(dz[:cnt])
+        dzp = np.ascontiguousarray(np.ascontiguousarray(t[:cnt, 3]).view(np.float32))
+        D.lib().or_march_sun(D._p(sc.a), D._p(sc.vegdsm), D._p(sc.vegdsm2), rows, cols, D._p(dx), D._p(dy),
+                             D._p(dzc), D._p(dzp), cnt, D._p(walls), D._p(asp.ravel()), sp.facesh_branch,
+                             sp.azilow, sp.azihigh, D._p(out), D._p(work))
+        got = out.reshape(8, rows, cols)
+        for q, nm in enumerate(names):
+            assert np.array_equal(got[q].view(np.uint32), d["m_" + nm][j].view(np.uint32)), (int(i), nm)
+        j += 1
+    assert j == d["m_sh"].shape[0]
