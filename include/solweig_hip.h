@@ -305,6 +305,23 @@ int sw_svf_sweep_cfg(const void *hpack, uint32_t *stats, const int32_t *offs, co
                      const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
                      const int32_t *nann, const int32_t *dirm, int32_t *counts, void *hpad, uint32_t *masks,
                      float *fields, int rows, int cols, const sw_march_config_t *cfg, void *stream);
+/* Scratch-free SVF sweep. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc). sw_svf_sweep_cfg without the
+ * counts and hpad workspace arguments: both live in a static device array of
+ * SW_SVF_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends. Limit: (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 12
+ * + 1024 <= SW_SVF_SCRATCH_BYTES (tiles up to about 2300 x 2300); larger tiles
+ * are refused with SW_E_ARG before anything is enqueued. Calls on one GPU
+ * share its array: calls that overlap on one GPU (several streams or host
+ * threads) race; serialise them on one stream. Allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine). Outputs equal
+ * sw_svf_sweep_cfg's. */
+#define SW_SVF_SCRATCH_BYTES (64LL << 20)
+int sw_svf_sweep_noalloc(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                         const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                         const int32_t *nann, const int32_t *dirm, uint32_t *masks, float *fields, int rows,
+                         int cols, const sw_march_config_t *cfg, void *stream);
 /* Day loop over steps [t0, t1) (the time loop of run_utci_window,
  * utci_process.py:917-1049): per step the sun march when is_day[i - t0]
  * (host array) and the flux step; outputs of step i at

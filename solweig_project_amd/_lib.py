@@ -187,6 +187,7 @@ SIGNATURES = {
     "sw_utci": (_I, [_VP] * 5 + [ctypes.c_int64, _VP]),
     "sw_svf_march_cfg": (_I, [_VP] * 7 + [_I, _I, _VP, _VP]),
     "sw_svf_sweep_cfg": (_I, [_VP] * 14 + [_I, _I, _VP, _VP]),
+    "sw_svf_sweep_noalloc": (_I, [_VP] * 12 + [_I, _I, _VP, _VP]),
     "sw_svf_sweep_tiles": (_I, [_VP] * 15 + [_I, _I, _I, _VP]),
     "sw_solve_day_tiles": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP]),
     "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),

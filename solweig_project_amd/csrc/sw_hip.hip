@@ -20,6 +20,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "sw_internal.h"
 
 #define SW_BUILD_FLAGS "hipcc -O3 --offload-arch=gfx950 -ffp-contract=off -fno-fast-math -packed-fp32-ops(off)"
@@ -1684,6 +1686,66 @@ int sw_svf_march_cfg(const void *hpack, const int32_t *offs, const int32_t *tab,
     launch_march(grid, S(stream), chunk, shape, (const float *)hpad, offs, (const int4 *)tab, counts, masks, stats,
                  rows, cols, (const int32_t *)nullptr, 0, split);
     return hip_check(hipGetLastError(), "k_svf_march");
+}
+
+// ---------------------------------------------------------------------------
+// scratch-free SVF sweep: the march workspace (step counts, padded heights) in
+// a static device array instead of caller buffers. Each GPU holds its own copy
+// of the array; host code addresses it only through the pointer that
+// hipGetSymbolAddress returns with that GPU current, kept per device.
+// ---------------------------------------------------------------------------
+#define SW_SCRATCH_COUNTS_BYTES 1024   // counts[153] at the start, padded heights after
+#define SW_SCRATCH_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[SW_SVF_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SW_SCRATCH_MAX_DEVICES];
+
+// the current device's address of `scratch`, looked up once per device
+static int scratch_ptr(void **out) {
+    int dev = -1;
+    int rc = hip_check(hipGetDevice(&dev), "sw_svf_sweep_noalloc: hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= SW_SCRATCH_MAX_DEVICES)
+        return set_err(SW_E_UNSUPPORTED, "sw_svf_sweep_noalloc: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "sw_svf_sweep_noalloc: hipGetSymbolAddress");
+        if (rc) return rc;
+        if (!p) return set_err(SW_E_HIP, "sw_svf_sweep_noalloc: hipGetSymbolAddress gave a NULL address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SW_OK;
+}
+
+int sw_svf_sweep_noalloc(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
+                         const float *vegdsm2, const float *w_iso, const float *w_aniso, const int32_t *ring,
+                         const int32_t *nann, const int32_t *dirm, uint32_t *masks, float *fields, int rows,
+                         int cols, const sw_march_config_t *cfg, void *stream) {
+    int rc = check_shape(rows, cols, "sw_svf_sweep_noalloc");
+    if (rc) return rc;
+    if (!hpack || !stats || !offs || !tab || !vegdsm2 || !w_iso || !w_aniso || !ring || !nann || !dirm || !masks ||
+        !fields)
+        return set_err(SW_E_ARG, "sw_svf_sweep_noalloc: NULL pointer");
+    const long long need =
+        (long long)(rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 12 + SW_SCRATCH_COUNTS_BYTES;
+    if (need > (long long)SW_SVF_SCRATCH_BYTES) {
+        char buf[200];
+        snprintf(buf, sizeof(buf),
+                 "sw_svf_sweep_noalloc: a %d x %d tile needs %lld bytes of march workspace, above the %lld-byte "
+                 "static scratch (use sw_svf_sweep with caller buffers)",
+                 rows, cols, need, (long long)SW_SVF_SCRATCH_BYTES);
+        return set_err(SW_E_ARG, buf);
+    }
+    void *base = nullptr;
+    if ((rc = scratch_ptr(&base))) return rc;
+    // both parts are rewritten in full before they are read: k_svf_counts writes all 153 counts and
+    // k_pad_heights every padded cell, so the previous call's contents never leak into this one
+    int32_t *counts = (int32_t *)base;
+    void *hpad = (unsigned char *)base + SW_SCRATCH_COUNTS_BYTES;
+    return sw_svf_sweep_cfg(hpack, stats, offs, tab, vegdsm2, w_iso, w_aniso, ring, nann, dirm, counts, hpad, masks,
+                            fields, rows, cols, cfg, stream);
 }
 
 int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, const int32_t *tab,
