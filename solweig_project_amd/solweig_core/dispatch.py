@@ -1,11 +1,11 @@
-"""Explicit device/backend resolution (mirror of ``solweig_core/dispatch.py:51-191``).
+"""Explicit device/backend resolution (the interface of ``solweig_core/dispatch.py:51-191``).
 
-The request DECLARES its device; :func:`resolve` answers READY with a plan
-or BLOCKED with a typed refusal. No availability probing, no silent
-re-targeting. Backends per declared device: ``'cuda'`` (a ROCm/HIP device
-under PyTorch-ROCm's naming) resolves to the MI355X backend; this package
-carries no CPU solver, so ``'cpu'`` is refused with a typed reason (a
-caller keeps its own CPU backends in its own table).
+A request declares its device; :func:`resolve` answers READY with a plan or BLOCKED with a typed
+refusal, never probing availability and never re-targeting. Kept from the reference: the record
+types ``DispatchPlan`` / ``DispatchResult``, their ``describe()`` keys (run-record format) and the
+refusal reasons. The registry is ours: the declared device ``'cuda'`` (PyTorch-ROCm's name for a HIP
+device) maps to the MI355X backend ``'hip-gfx950'``. This package carries no CPU solver, so
+``'cpu'`` has no backend here and is refused (a caller keeps its own CPU backends in its own table).
 """
 from __future__ import annotations
 
@@ -13,10 +13,11 @@ from dataclasses import dataclass
 from typing import Any
 
 from .request import SolveRequestView
-from .status import DispatchStatus, FallbackReason, RefusalReason, TypedRefusal
+from .status import DispatchBlockedError, DispatchStatus, FallbackReason, RefusalReason, TypedRefusal
 
 HIP_BACKEND = "hip-gfx950"
 
+#: declared device -> backends registered for it, first = default
 _BACKENDS: dict[str, tuple[str, ...]] = {
     "cpu": (),
     "cuda": (HIP_BACKEND,),
@@ -33,10 +34,10 @@ class DispatchPlan:
     status: DispatchStatus = DispatchStatus.READY
 
     def describe(self) -> dict[str, Any]:
-        return {"device": self.device, "backend": self.backend, "profile_id": self.profile_id,
-                "logical_domain_id": self.logical_domain_id,
-                "fallback_reason": self.fallback_reason.value if self.fallback_reason else None,
-                "status": self.status.value}
+        record = {key: getattr(self, key) for key in ("device", "backend", "profile_id", "logical_domain_id")}
+        record["fallback_reason"] = None if self.fallback_reason is None else self.fallback_reason.value
+        record["status"] = self.status.value
+        return record
 
 
 @dataclass(frozen=True)
@@ -47,48 +48,48 @@ class DispatchResult:
 
     @property
     def ready(self) -> bool:
-        return self.status is DispatchStatus.READY
+        return self.status is DispatchStatus.READY and self.plan is not None
 
     def raise_if_blocked(self) -> DispatchPlan:
-        if self.status is not DispatchStatus.READY or self.plan is None:
-            from .status import DispatchBlockedError
-            raise DispatchBlockedError(self.refusal or TypedRefusal(RefusalReason.UNKNOWN_BACKEND,
-                                                                    "dispatch blocked"))
-        return self.plan
+        """The plan of a READY result; a BLOCKED one raises its refusal."""
+        if self.ready:
+            return self.plan
+        why = self.refusal if self.refusal is not None else TypedRefusal(
+            RefusalReason.UNKNOWN_BACKEND, "the dispatch result carries no plan")
+        raise DispatchBlockedError(why)
 
     def describe(self) -> dict[str, Any]:
-        return {"status": self.status.value, "plan": self.plan.describe() if self.plan else None,
-                "refusal": self.refusal.to_dict() if self.refusal else None}
+        return {"status": self.status.value,
+                "plan": None if self.plan is None else self.plan.describe(),
+                "refusal": None if self.refusal is None else self.refusal.to_dict()}
 
 
 def supported_devices() -> tuple[str, ...]:
     return tuple(sorted(_BACKENDS))
 
 
+def _blocked(reason: RefusalReason, detail: str, **context) -> DispatchResult:
+    return DispatchResult(DispatchStatus.BLOCKED, refusal=TypedRefusal(reason, detail, context=context))
+
+
 def resolve(request: SolveRequestView, prefer_backend: str | None = None) -> DispatchResult:
-    """Resolve the DECLARED device of a request (validated here again)."""
+    """Resolve the declared device of a request (validating it first)."""
     request.validate()
-    if request.device not in _BACKENDS:
-        return DispatchResult(DispatchStatus.BLOCKED, refusal=TypedRefusal(
-            RefusalReason.UNSUPPORTED_DEVICE,
-            f"device {request.device!r} is outside the dispatch vocabulary {sorted(_BACKENDS)}",
-            context={"device": request.device}))
-    backends = _BACKENDS[request.device]
-    if not backends:
-        return DispatchResult(DispatchStatus.BLOCKED, refusal=TypedRefusal(
-            RefusalReason.UNSUPPORTED_DEVICE,
-            f"device {request.device!r}: this package provides only the MI355X backend "
-            f"({HIP_BACKEND!r} under device 'cuda'); a CPU request is never run on the GPU",
-            context={"device": request.device, "logical_domain_id": request.logical_domain_id}))
-    backend = backends[0]
-    if prefer_backend is not None:
-        if prefer_backend not in backends:
-            return DispatchResult(DispatchStatus.BLOCKED, refusal=TypedRefusal(
-                RefusalReason.UNKNOWN_BACKEND,
-                f"preferred backend {prefer_backend!r} is not registered for device {request.device!r} "
-                f"(registered: {list(backends)})",
-                context={"device": request.device, "prefer_backend": prefer_backend}))
-        backend = prefer_backend
+    device = request.device
+    registered = _BACKENDS.get(device)
+    if registered is None:
+        return _blocked(RefusalReason.UNSUPPORTED_DEVICE,
+                        f"device {device!r} is outside the dispatch vocabulary {supported_devices()}", device=device)
+    if not registered:
+        return _blocked(RefusalReason.UNSUPPORTED_DEVICE,
+                        f"no backend of this package serves device {device!r}: it provides {HIP_BACKEND!r} under "
+                        "'cuda' only, and a CPU request is never moved to the GPU",
+                        device=device, logical_domain_id=request.logical_domain_id)
+    chosen = registered[0] if prefer_backend is None else prefer_backend
+    if chosen not in registered:
+        return _blocked(RefusalReason.UNKNOWN_BACKEND,
+                        f"backend {chosen!r} is not registered for device {device!r} (registered: {list(registered)})",
+                        device=device, prefer_backend=chosen)
     return DispatchResult(DispatchStatus.READY, plan=DispatchPlan(
-        device=request.device, backend=backend, profile_id=request.profile_id,
-        logical_domain_id=request.logical_domain_id, fallback_reason=None))
+        device=device, backend=chosen, profile_id=request.profile_id,
+        logical_domain_id=request.logical_domain_id))

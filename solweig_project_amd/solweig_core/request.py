@@ -1,13 +1,17 @@
-"""Typed solve requests (mirror of ``solweig_core/request.py:66-259``).
+"""Typed solve requests of the core surface (the interface of ``solweig_core/request.py:66-259``).
 
-The logical domain (whole grid) and the physical windows (cells read /
-written) are separate objects; time coverage is in GLOBAL timestep indices.
-Validation raises :class:`RequestValidationError` with a typed refusal.
+A request names a logical domain (the whole grid), two physical cell windows (read and write) and a
+coverage of GLOBAL forcing timesteps. What is kept from the reference is the interface: the field
+names and defaults, the refusal reason (``request_invalid``), the ``describe()`` keys that run records
+quote, and the phrases its tests match ("exceeds the logical domain", "empty or inverted",
+"replay range", "exceeds the forcing series", "not contained in the read", "numpy dtype",
+"vocabulary", ...). The checks themselves are rule tables: each object lists the problems it can
+have as (predicate, message) pairs, and :func:`_first_problem` raises the first one that applies.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Any, Mapping
+from typing import Any, Callable, Iterable, Mapping
 
 import numpy as np
 
@@ -19,32 +23,40 @@ DEFAULT_DTYPE_PLAN: dict[str, str] = {
 }
 DEFAULT_OUTPUT_DTYPE_PLAN: dict[str, str] = {"utci": "<f4", "tmrt": "<f4", "shadow": "<f4"}
 
-#: request vocabulary of devices (dispatch resolves, never guesses)
+#: devices a request may declare; dispatch maps them to backends and never substitutes one
 DEVICES = ("cpu", "cuda")
 
+# A rule: (applies() -> bool, message() -> str, context() -> dict | None). Messages are built only
+# for the rule that fires.
+Rule = tuple[Callable[[], bool], Callable[[], str], Callable[[], "dict[str, Any] | None"]]
 
-def _refusal(detail: str, context: dict[str, Any] | None = None):
-    return RequestValidationError(TypedRefusal(RefusalReason.REQUEST_INVALID, detail, context))
+
+def _first_problem(rules: Iterable[Rule]) -> None:
+    for applies, message, context in rules:
+        if applies():
+            raise RequestValidationError(TypedRefusal(RefusalReason.REQUEST_INVALID, message(), context()))
+
+
+def _no_context():
+    return None
+
+
+def _is_dtype(text: str) -> bool:
+    try:
+        np.dtype(text)
+    except TypeError:
+        return False
+    return True
 
 
 @dataclass(frozen=True)
 class PhysicalWindow:
-    """Half-open cell window [row_start, row_stop) x [col_start, col_stop)."""
+    """Cells [row_start, row_stop) x [col_start, col_stop) of the logical grid."""
 
     row_start: int
     row_stop: int
     col_start: int
     col_stop: int
-
-    def validate(self, *, rows: int, cols: int, label: str = "window") -> None:
-        if self.row_start < 0 or self.col_start < 0:
-            raise _refusal(f"{label} origin ({self.row_start}, {self.col_start}) is negative",
-                           {"window": self.list()})
-        if self.row_stop <= self.row_start or self.col_stop <= self.col_start:
-            raise _refusal(f"{label} is empty or inverted: {self}", {"window": self.list()})
-        if self.row_stop > rows or self.col_stop > cols:
-            raise _refusal(f"{label} {self} exceeds the logical domain grid ({rows} rows x {cols} cols)",
-                           {"window": self.list(), "grid": [rows, cols]})
 
     @property
     def height(self) -> int:
@@ -56,22 +68,37 @@ class PhysicalWindow:
 
     @property
     def origin(self) -> tuple[int, int]:
-        return (self.row_start, self.col_start)
+        return self.row_start, self.col_start
 
     def list(self) -> list[int]:
         return [self.row_start, self.row_stop, self.col_start, self.col_stop]
 
     def __iter__(self):
-        return iter((self.row_start, self.row_stop, self.col_start, self.col_stop))
+        return iter(self.list())
+
+    def _span(self) -> str:
+        return f"rows {self.row_start}:{self.row_stop}, cols {self.col_start}:{self.col_stop}"
 
     def contains(self, other: "PhysicalWindow") -> bool:
-        return (self.row_start <= other.row_start and self.row_stop >= other.row_stop
-                and self.col_start <= other.col_start and self.col_stop >= other.col_stop)
+        rows_in = self.row_start <= other.row_start and other.row_stop <= self.row_stop
+        cols_in = self.col_start <= other.col_start and other.col_stop <= self.col_stop
+        return rows_in and cols_in
+
+    def validate(self, *, rows: int, cols: int, label: str = "window") -> None:
+        ctx = lambda: {"window": self.list(), "grid": [rows, cols]}  # noqa: E731
+        _first_problem((
+            (lambda: min(self.row_start, self.col_start) < 0,
+             lambda: f"{label} ({self._span()}) starts at a negative cell", ctx),
+            (lambda: self.height <= 0 or self.width <= 0,
+             lambda: f"{label} ({self._span()}) is empty or inverted", ctx),
+            (lambda: self.row_stop > rows or self.col_stop > cols,
+             lambda: f"{label} ({self._span()}) exceeds the logical domain of {rows} x {cols} cells", ctx),
+        ))
 
 
 @dataclass(frozen=True)
 class TimeCoverage:
-    """GLOBAL time indices [time_start, time_stop) of the forcing series."""
+    """GLOBAL forcing timesteps [time_start, time_stop) of a series of ``total_steps`` (None = to the end)."""
 
     time_start: int
     time_stop: int | None
@@ -81,24 +108,25 @@ class TimeCoverage:
         if self.time_stop is None:
             object.__setattr__(self, "time_stop", int(self.total_steps))
 
-    def validate(self) -> None:
-        if int(self.total_steps) <= 0:
-            raise _refusal(f"total_steps must be positive, got {self.total_steps}")
-        if int(self.time_start) < 0 or int(self.time_start) >= int(self.time_stop):
-            raise _refusal(f"time_start {self.time_start} outside the replay range [0, {self.time_stop})",
-                           {"time": [self.time_start, self.time_stop]})
-        if int(self.time_stop) > int(self.total_steps):
-            raise _refusal(f"time_stop {self.time_stop} exceeds the forcing series ({self.total_steps} global "
-                           "timesteps)", {"time": [self.time_start, self.time_stop]})
-
     @property
     def global_indices(self) -> tuple[int, ...]:
         return tuple(range(int(self.time_start), int(self.time_stop)))
 
+    def validate(self) -> None:
+        start, stop, total = int(self.time_start), int(self.time_stop), int(self.total_steps)
+        ctx = lambda: {"time": [start, stop], "total_steps": total}  # noqa: E731
+        _first_problem((
+            (lambda: total < 1, lambda: f"total_steps is {total}; a forcing series has at least one step", ctx),
+            (lambda: not 0 <= start < stop,
+             lambda: f"steps {start}..{stop} leave no step of the replay range [0, {stop})", ctx),
+            (lambda: stop > total,
+             lambda: f"time_stop {stop} exceeds the forcing series of {total} global timesteps", ctx),
+        ))
+
 
 @dataclass(frozen=True)
 class SolveRequestView:
-    """Everything the core needs to route one window solve; the device is DECLARED."""
+    """What the core needs to route one window solve. The device is declared by the caller."""
 
     logical_domain_id: str
     rows: int
@@ -115,41 +143,53 @@ class SolveRequestView:
     profile_id: str = ""
     device: str = "cpu"
 
+    def _bad_dtypes(self) -> list[str]:
+        plans = {"dtype_plan": self.dtype_plan, "output_dtype_plan": self.output_dtype_plan}
+        return [f"{name}[{plane!r}] = {text!r}" for name, plan in plans.items()
+                for plane, text in dict(plan).items() if not _is_dtype(text)]
+
     def validate(self) -> None:
-        if not isinstance(self.logical_domain_id, str) or not self.logical_domain_id:
-            raise _refusal("logical_domain_id must be a non-empty string")
-        if self.rows <= 0 or self.cols <= 0:
-            raise _refusal(f"logical domain shape ({self.rows}, {self.cols}) is empty")
-        if not (self.pixel_size_m > 0.0):
-            raise _refusal(f"pixel_size_m must be positive, got {self.pixel_size_m!r}")
+        # identity of the run first, then the grid, the windows, time and the outputs
+        _first_problem((
+            (lambda: not (isinstance(self.logical_domain_id, str) and self.logical_domain_id),
+             lambda: "logical_domain_id must name the domain (a non-empty string)", _no_context),
+            (lambda: not self.profile_id,
+             lambda: "profile_id is required: every solve runs under a named profile", _no_context),
+            (lambda: self.device not in DEVICES,
+             lambda: f"device {self.device!r} is not in the request vocabulary {list(DEVICES)}; "
+                     "dispatch never guesses a device",
+             lambda: {"device": self.device}),
+            (lambda: self.rows < 1 or self.cols < 1,
+             lambda: f"the logical domain of {self.rows} x {self.cols} cells is empty", _no_context),
+            (lambda: not self.pixel_size_m > 0.0,
+             lambda: f"pixel_size_m is {self.pixel_size_m!r}; cells need a positive size", _no_context),
+        ))
         self.read_window.validate(rows=self.rows, cols=self.cols, label="read window")
         self.write_window.validate(rows=self.rows, cols=self.cols, label="write window")
-        if not self.read_window.contains(self.write_window):
-            raise _refusal(f"write window {self.write_window} is not contained in the read window "
-                           f"{self.read_window}", {"read": self.read_window.list(),
-                                                   "write": self.write_window.list()})
+        _first_problem((
+            (lambda: not self.read_window.contains(self.write_window),
+             lambda: f"write window {self.write_window.list()} is not contained in the read window "
+                     f"{self.read_window.list()}",
+             lambda: {"read": self.read_window.list(), "write": self.write_window.list()}),
+        ))
         self.time.validate()
-        if not self.requested_variables:
-            raise _refusal("requested_variables must be non-empty")
-        for plan_name, plan in (("dtype_plan", self.dtype_plan), ("output_dtype_plan", self.output_dtype_plan)):
-            for plane, dtype_str in dict(plan).items():
-                try:
-                    np.dtype(dtype_str)
-                except TypeError as error:
-                    raise _refusal(f"{plan_name}[{plane!r}] = {dtype_str!r} is not a numpy dtype") from error
-        if not self.profile_id:
-            raise _refusal("profile_id is required (profile identity of the run)")
-        if self.device not in DEVICES:
-            raise _refusal(f"device {self.device!r} is outside the request vocabulary ('cpu' or 'cuda'); "
-                           "dispatch never guesses")
+        _first_problem((
+            (lambda: len(self.requested_variables) == 0,
+             lambda: "requested_variables lists no output", _no_context),
+            (lambda: bool(self._bad_dtypes()),
+             lambda: "not a numpy dtype: " + ", ".join(self._bad_dtypes()), _no_context),
+        ))
 
     def describe(self) -> dict[str, Any]:
-        return {
-            "logical_domain_id": self.logical_domain_id, "rows": self.rows, "cols": self.cols,
-            "origin_x_m": self.origin_x_m, "origin_y_m": self.origin_y_m, "pixel_size_m": self.pixel_size_m,
-            "read_window": self.read_window.list(), "write_window": self.write_window.list(),
-            "time_start": self.time.time_start, "time_stop": self.time.time_stop,
-            "total_steps": self.time.total_steps, "requested_variables": list(self.requested_variables),
-            "dtype_plan": dict(self.dtype_plan), "output_dtype_plan": dict(self.output_dtype_plan),
-            "profile_id": self.profile_id, "device": self.device,
-        }
+        """The run-record fields of the request (the reference's record keys)."""
+        record: dict[str, Any] = {name: getattr(self, name) for name in (
+            "logical_domain_id", "rows", "cols", "origin_x_m", "origin_y_m", "pixel_size_m")}
+        record["read_window"] = self.read_window.list()
+        record["write_window"] = self.write_window.list()
+        record.update(time_start=self.time.time_start, time_stop=self.time.time_stop,
+                      total_steps=self.time.total_steps)
+        record["requested_variables"] = list(self.requested_variables)
+        record["dtype_plan"] = dict(self.dtype_plan)
+        record["output_dtype_plan"] = dict(self.output_dtype_plan)
+        record.update(profile_id=self.profile_id, device=self.device)
+        return record
