@@ -3,12 +3,15 @@
 
 One "step" is one full solve of the 1000 x 1000 x 24 h synthetic tile
 (scene compose, 153-patch SVF sweep, 24 hourly steps of sun march, ground
-view factors, flux assembly, Tmrt and UTCI) with the input rasters already
-resident in HBM and the outputs (16 SVF fields, per-step Tmrt, UTCI and
-shadow) left in HBM. The host scalar chain of each solve's day (clearness
-index, Perez luminance, sun tables, step records) is computed inside the
-timed region, in planner processes two solves ahead of the GPU. Prints one
-JSON line.
+view factors, flux assembly, Tmrt and UTCI), timed from host-resident
+inputs to host-resident outputs: each solve uploads the six input rasters
+from pinned host memory and delivers its 16 SVF fields and every step's
+Tmrt, UTCI and shadow planes into pinned host buffers of its own (the
+copies stream out while the GPU computes). The host scalar chain of each
+solve's day (clearness index, Perez luminance, sun tables, step records) is
+computed inside the timed region, in planner processes ahead of the GPU.
+The device-resident rate (inputs in HBM, outputs left in HBM) is reported
+beside it. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,6 +30,11 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA
+# VALU issue: one wave64 VALU instruction per CU per clock (4 SIMDs, each taking 4 clocks per wave64
+# instruction on 16 lanes; the same figure as the non-packed FP32 FMA rate, 78.6 TFLOP/s)
+N_CU = 256
+CLOCK_GHZ = 2.4
+VALU_PEAK_GINSTR = N_CU * CLOCK_GHZ
 
 
 def parse():
@@ -50,6 +58,11 @@ def parse():
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
     ap.add_argument("--lanes", type=int, default=2,
                     help="tile workload: consecutive solves alternate over this many scene copies on their own streams")
+    ap.add_argument("--h2h-lanes", type=int, default=3,
+                    help="tile workload: lanes of the host-to-host timed region (each copies on its own stream)")
+    ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane"), default="own",
+                    help="tile workload: where the host-to-host copies run: on each lane's stream, on one "
+                         "shared copy stream, or on a copy stream per lane")
     ap.add_argument("--march", default="",
                     help="tile workload: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
     ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
@@ -77,6 +90,20 @@ PROFILE_KEYS = {"svf_march": "k_svf_march", "svf_fold": "k_svf_fold", "compose":
                 "flux_day": "k_flux_step<true> (day)", "flux_night": "k_flux_step<false> (night)",
                 "sun_step": "k_sun_step"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_full_solve_1000x1000x24.md"
+VALU_FILE = ROOT / "profiles" / "valu_issue_full_solve_1000x1000x24.md"
+
+
+def load_valu(rows, cols, seed):
+    """VALU wave-instructions per launch from the committed rocprofv3 SQ_INSTS_VALU summary ({} if the
+    summary is absent or was measured on another workload)."""
+    if (rows, cols, seed) != (1000, 1000, 20261203) or not VALU_FILE.exists():
+        return {}
+    text = VALU_FILE.read_text()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return {k: int(v) for k, v in json.loads(block)["valu_insts_per_launch"].items()}
+    except (IndexError, KeyError, ValueError):
+        return {}
 
 
 def load_traffic(rows, cols, seed):
@@ -547,7 +574,7 @@ def main():
             pipeline.close()
         return
     from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
-    pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used
+    pipeline = PlanPipeline(workers=3)      # CPU-only planner processes, started before the GPU is used
     import numpy as np
     import torch
     from solweig_project_amd import skyvault as SV
@@ -608,25 +635,76 @@ def main():
     marks = np.array(getattr(solver, "pipeline_marks", [[0, 0, 0, 0]])) * 1e3
     log("host ms per solve (mean): svf enqueue %.2f, wait for plan %.2f, upload %.2f, day enqueue %.2f"
         % tuple(marks.mean(axis=0)))
-    pipeline.close()
     plan_ms = sum(solver.plan_ms) / max(len(solver.plan_ms), 1)   # per solve, in the planner process
-    ms = (t1 - t0) / args.steps * 1e3
-    value = rows * cols * T / (ms / 1e3) / 1e6
+    dev_ms = (t1 - t0) / args.steps * 1e3
+    dev_value = rows * cols * T / (dev_ms / 1e3) / 1e6
     ksum = timer.summary()
     log("kernels: " + ", ".join(f"{k}: {n} x {t / n:.3f} ms" for k, (n, t) in ksum.items()))
+    log(f"device-resident: {dev_ms:.3f} ms per solve, {dev_value:.1f} Mpixel-steps/s")
 
-    # host -> host rate (input upload + download of all outputs), reported beside value
+    # ---- the timed region of `value`: host-resident inputs -> host-resident outputs ----
+    from solweig_project_amd.pipeline import HostBuffers, pinned_inputs, solve_many_host
+    if lanes is None:
+        lanes = [(ds, torch.cuda.Stream(dev), out)]
+    host_in = pinned_inputs(tile)
+    # host-to-host lanes: each copies its outputs in order on its own stream (lanes + the null stream
+    # stay within the 4 hardware queues); scene copies beyond the device-resident lanes are added
+    h2h_lanes = [(sc, st, None) for sc, st, _ in lanes]
+    while len(h2h_lanes) < args.h2h_lanes:
+        h2h_lanes.append((DeviceScene(tile, dev, upload_inputs(tile, dev)), torch.cuda.Stream(dev), None))
+    h2h_lanes = h2h_lanes[:args.h2h_lanes]
+    if args.h2h_copy == "shared":
+        cs = torch.cuda.Stream(dev)
+        h2h_lanes = [(sc, st, cs) for sc, st, _ in h2h_lanes]
+    elif args.h2h_copy == "per-lane":
+        h2h_lanes = [(sc, st, torch.cuda.Stream(dev)) for sc, st, _ in h2h_lanes]
+    n_spare = len(h2h_lanes) + 1              # host buffers the warm-up solves write in turn
+    n_h2h = args.steps + n_spare
+    gb = n_h2h * (16 + 3 * T) * rows * cols * 4 / 1e9
+    log(f"pinned host outputs for {n_h2h} solves ({gb:.1f} GB)")
+    hbufs = [HostBuffers(torch.empty((16, rows, cols), dtype=torch.float32, pin_memory=True),
+                         torch.empty((T, 3, rows, cols), dtype=torch.float32, pin_memory=True))
+             for _ in range(n_h2h)]
+    dev_outs = [lanes[0][2]] + [torch.empty_like(lanes[0][2]) for _ in range(2)]   # ring of day outputs
+    # warm-up: every lane and every plan staging buffer once (first-use allocations stay out of the timing)
+    n_warm_h2h = max(n_warm, len(h2h_lanes) * solver.STAGING_DEPTH)
+    spare = hbufs[args.steps:]
+    solve_many_host(solver, h2h_lanes, [job] * n_warm_h2h, pipeline, host_in,
+                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs)
     torch.cuda.synchronize()
+    log(f"timing {args.steps} host-to-host solves over {len(h2h_lanes)} lane(s)")
     h0 = time.perf_counter()
-    ds_h = DeviceScene(tile, dev, upload_inputs(tile, dev))
-    f_dev, o_dev = solver.solve_device(ds_h, met, sun, site.latitude, out=out)
-    f_host = f_dev.cpu().numpy()
-    o_host = o_dev.cpu().numpy()
+    done = solve_many_host(solver, h2h_lanes, [job] * args.steps, pipeline, host_in, hbufs[:args.steps], dev_outs)
+    torch.cuda.synchronize()
     h1 = time.perf_counter()
-    host_value = rows * cols * T / (h1 - h0) / 1e6
-    del f_host, o_host
+    hm = np.array(solver.pipeline_marks) * 1e3
+    log("host ms per host-to-host solve (mean): upload + svf + fields copy enqueue %.2f, wait for plan %.2f, "
+        "plan upload %.2f, day + copies enqueue %.2f" % tuple(hm.mean(axis=0)))
+    ms = (h1 - h0) / args.steps * 1e3
+    value = rows * cols * T / (ms / 1e3) / 1e6
+    gaps = [done[i - 1].elapsed_time(done[i]) for i in range(1, len(done))]
+    gap_ms = float(np.median(gaps)) if gaps else ms
+    # every timed solve delivered the same (deterministic) outputs to its own host buffers
+    f_ref = ds.fields.reshape(16, rows, cols).cpu().view(torch.int32)
+    o_ref = dev_outs[(args.steps - 1) % len(dev_outs)].cpu().view(torch.int32)   # bits: UTCI NaN off ground
+    delivered = all(torch.equal(hb.fields.view(torch.int32), f_ref) and torch.equal(hb.out.view(torch.int32), o_ref)
+                    for hb in hbufs[:args.steps])
+    if not delivered:
+        raise RuntimeError("a host-to-host solve delivered outputs different from the device-resident solve")
+    log(f"host-to-host: {ms:.3f} ms per solve ({value:.1f} Mpixel-steps/s), median gap between solve "
+        f"completions {gap_ms:.3f} ms; outputs of all {args.steps} solves checked on the host")
+    del hbufs
 
-    # rooflines: algorithmic bytes per launch / average launch time (HIP events on the launch stream)
+    # rooflines: kernels timed alone (one lane, HIP events on the launch stream, 3 solves), each against
+    # VALU issue (their limiter) with the measured HBM fraction beside it
+    alone = KernelTimer(reserve=3 * (6 + 3 * T))
+    solver.timer = alone
+    solve_many(solver, ds, [job] * 3, pipeline, out=out)
+    pipeline.close()
+    torch.cuda.synchronize()
+    solver.timer = _NoTimer()
+    ksum_alone = alone.summary()
+    log("kernels alone: " + ", ".join(f"{k}: {n} x {t / n:.3f} ms" for k, (n, t) in ksum_alone.items()))
     amax, hmax = ds.amax_hmax()
     offs, dx, dy, dz = SV.svf_tables(solver.vault, tile.scale, np.float32(amax), rows, cols)
     n = rows * cols
@@ -643,16 +721,20 @@ def main():
         "compose": (24 + 16 + 4 + 4 + 4 + 1) * n,
     }
     traffic = load_traffic(rows, cols, args.seed)
+    valu = load_valu(rows, cols, args.seed)
     rooflines = {}
-    for name, (launches, tot) in ksum.items():
-        if name not in alg:
+    for name, (launches, tot) in ksum_alone.items():
+        key = PROFILE_KEYS.get(name, "")
+        if name not in alg or key not in valu:
             continue
         avg_ms = tot / launches
-        ach = alg[name] / (avg_ms / 1e3) / 1e9
-        tb = traffic.get(PROFILE_KEYS.get(name, ""))
-        rooflines[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
-                           "avg_launch_ms": round(avg_ms, 4), "launches_per_solve": launches // args.steps,
+        ach = valu[key] / (avg_ms / 1e3) / 1e9
+        tb = traffic.get(key)
+        rooflines[name] = {"bound": "valu", "achieved": round(ach, 1), "peak": VALU_PEAK_GINSTR,
+                           "unit": "G wave64 VALU instr/s", "frac": round(ach / VALU_PEAK_GINSTR, 4), "traffic": tb,
+                           "hbm_frac": round(tb / (avg_ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4) if tb else None,
+                           "valu_insts_per_launch": valu[key], "avg_launch_ms": round(avg_ms, 4),
+                           "launches_per_solve": launches // 3,
                            "algorithmic_bytes_per_launch": int(alg[name])}
     # UTCI on the vector ALUs: the standalone kernel (the function the flux kernel fuses) over this
     # workload's pixel-steps, 450 FLOP per pixel-step (210 polynomial terms as FMAs + the saturation
@@ -680,7 +762,7 @@ def main():
                          "note": "standalone k_utci over the day's pixel-steps; the flux kernel fuses the same function"}
     del u_in, u_out
     # dominant kernel = largest total time per solve
-    dom = max(ksum, key=lambda k: ksum[k][1])
+    dom = max(ksum_alone, key=lambda k: ksum_alone[k][1])
     roof = dict(kernel=dom, **rooflines[dom]) if dom in rooflines else None
     if args.calibrate_copy:
         from solweig_project_amd import _lib as L
@@ -737,16 +819,23 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded city tile generator; inputs resident in HBM, outputs left in HBM)",
+        "data": "synthetic (seeded city tile generator)",
         "config": {"workload": f"full SOLWEIG solve {rows}x{cols}x{T}h (SVF sweep + sun march + GVF + "
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
-                   "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": max(args.lanes, 1),
+                   "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": len(h2h_lanes),
+                   "lanes_device_resident": len(lanes),
+                   "timed_region": "host-resident inputs (6 rasters, pinned) -> host-resident outputs (16 SVF "
+                                   "fields + 24 x {tmrt, utci, shadow}, pinned, one buffer set per solve): "
+                                   "upload, SVF sweep, day loop, download; day plans in planner processes",
                    "march": march_info},
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
-        "value_host_to_host": round(host_value, 3),
-        "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
+        "value_device_resident": round(dev_value, 3),
+        "ms_per_step_device_resident": round(dev_ms, 3),
+        "median_solve_gap_ms": round(gap_ms, 3),
+        "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum_alone.items()},
+        "kernel_ms_lanes": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
         "host_plan_ms": round(plan_ms, 3),
         "walls_aspect": walls_info,
     }

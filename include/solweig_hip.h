@@ -154,6 +154,12 @@ int sw_copy(void *dst, const void *src, int64_t bytes);
 int sw_copy_async(void *dst, const void *src, int64_t bytes, void *stream);
 int sw_copy2d_async(void *dst, int64_t dpitch, const void *src, int64_t spitch, int64_t width_bytes,
                     int64_t height, void *stream);
+/* Device -> pinned host copy on the DMA engines, using no compute units (hipMemcpyAsync with
+ * hipMemcpyDeviceToDeviceNoCU: pinned host memory is device-addressable), so output downloads
+ * overlap solves on other streams at full PCIe rate. dst must be pinned host memory
+ * (sw_host_alloc). No counterpart in the reference (its residency ABI copies with
+ * cudaMemcpyAsync, sw_residency.cu:53-331). */
+int sw_copy_to_host(void *dst, const void *src, int64_t bytes, void *stream);
 int sw_fill_f32(float *dst, float value, int64_t n, void *stream);
 int sw_stream_create(void **stream);
 int sw_stream_destroy(void *stream);

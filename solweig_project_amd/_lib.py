@@ -157,6 +157,7 @@ SIGNATURES = {
     "sw_fill_f32": (_I, [_VP, _F, _I64, _VP]),
     "sw_stream_create": (_I, [_VP]),
     "sw_stream_destroy": (_I, [_VP]),
+    "sw_copy_to_host": (_I, [_VP, _VP, _I64, _VP]),
     "sw_event_create": (_I, [_VP]),
     "sw_event_destroy": (_I, [_VP]),
     "sw_event_record": (_I, [_VP, _VP]),

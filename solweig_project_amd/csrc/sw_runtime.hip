@@ -126,6 +126,15 @@ int sw_copy2d_async(void *dst, int64_t dpitch, const void *src, int64_t spitch, 
                          "hipMemcpy2DAsync");
 }
 
+int sw_copy_to_host(void *dst, const void *src, int64_t bytes, void *stream) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) return swi_set_err(SW_E_ARG, "sw_copy_to_host: invalid arguments");
+    if (bytes == 0) return SW_OK;
+    // "device to device without compute units": pinned host memory is device-addressable, so this
+    // runs on the DMA engines and leaves every CU to the solves on other streams
+    return swi_hip_check(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDeviceNoCU, swi_stream(stream)),
+                         "sw_copy_to_host: hipMemcpyAsync");
+}
+
 int sw_fill_f32(float *dst, float value, int64_t n, void *stream) {
     if (n < 0 || (n > 0 && !dst)) return swi_set_err(SW_E_ARG, "sw_fill_f32: invalid arguments");
     if (n == 0) return SW_OK;

@@ -122,3 +122,101 @@ def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci"
         marks.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3))
     solver.pipeline_marks = marks
     return out
+
+
+@dataclass
+class HostBuffers:
+    """Pinned host buffers of one solve: input rasters in, SVF fields and day outputs out."""
+
+    fields: "object"          # torch uint8/float32 pinned (16, rows, cols)
+    out: "object"             # torch float32 pinned (T, k, rows, cols)
+
+
+def pinned_inputs(tile):
+    """The tile's input rasters as pinned host tensors (what a host-to-host solve uploads)."""
+    import torch
+    names = ["dsm", "dem", "cdsm", "walls", "aspect_deg"] + (["landcover"] if tile.landcover is not None else [])
+    return {k: torch.from_numpy(np.ascontiguousarray(getattr(tile, k), np.float32)).pin_memory() for k in names}
+
+
+def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, host_outs, dev_outs,
+                    outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 6):
+    """Host-to-host solves: for job k, upload the pinned input rasters to lane k mod len(lanes),
+    run the SVF sweep and the day, and deliver the 16 SVF fields and every step's output planes
+    into ``host_outs[k]`` (pinned :class:`HostBuffers`).
+
+    ``lanes``: (scene, stream, copy stream) triples; a copy stream of None copies on the lane's own
+    stream (in order, no events: the lane waits for its copies while the other lanes compute, and
+    no stream shares a hardware queue when lanes + 1 <= GPU_MAX_HW_QUEUES). The day of job k writes
+    into device buffer
+    ``dev_outs[k mod len(dev_outs)]`` in chunks of ``chunk_steps`` steps. Each chunk's planes go
+    out on the lane's copy stream as soon as the chunk is computed, through ``sw_copy_to_host``
+    (the DMA engines: no compute units, so the downloads overlap the solves at full PCIe rate);
+    the SVF fields go out right after the sweep. Ordering is
+    stream-ordered with events, never a host wait: a lane's next sweep waits for the copy of its
+    previous fields, and a day waits for the copies out of the device buffer it is about to reuse
+    (``len(dev_outs)`` jobs back). Plans come from the planner processes, ``pipeline.workers`` jobs
+    ahead. Returns each job's completion event (its last copy), in job order."""
+    import torch
+
+    from . import _lib as L
+    ahead = max(1, int(getattr(pipeline, "workers", 1)))
+    pending = [pipeline.submit(lanes[0][0].tile, j) for j in jobs[:ahead]]
+    fields_out = [None] * len(lanes)        # event: the copy of a lane's SVF fields is done
+    buf_free = [None] * len(dev_outs)       # event: the copies out of a device output buffer are done
+    done = []
+
+    def copy_out(dst, src, cstream):
+        L.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), src.numel() * src.element_size(), cstream.cuda_stream)
+
+    import time
+    marks = []
+    for k, job in enumerate(jobs):
+        tm = [time.perf_counter()]
+        li, bi = k % len(lanes), k % len(dev_outs)
+        ds, stream, cstream = lanes[li]
+        own = cstream is None
+        cstream = stream if own else cstream
+        out = dev_outs[bi]
+        hb = host_outs[k]
+        with torch.cuda.stream(stream):
+            if fields_out[li] is not None:
+                stream.wait_event(fields_out[li])
+            for name, t in host_inputs.items():
+                ds.inputs[name].copy_(t, non_blocking=True)
+            solver.svf(ds, stream)
+            ev_fields = torch.cuda.Event()
+            ev_fields.record(stream)
+        if not own:
+            cstream.wait_event(ev_fields)
+        copy_out(hb.fields, ds.fields, cstream)
+        if not own:
+            fields_out[li] = torch.cuda.Event()
+            fields_out[li].record(cstream)
+        if k + ahead < len(jobs):
+            pending.append(pipeline.submit(ds.tile, jobs[k + ahead]))
+        tm.append(time.perf_counter())
+        plan = pending.pop(0).get()
+        tm.append(time.perf_counter())
+        dplan = solver.upload_plan(plan, stream)
+        tm.append(time.perf_counter())
+        solver.tile_static(ds, dplan, stream)
+        if buf_free[bi] is not None:
+            stream.wait_event(buf_free[bi])
+        T = plan.n_steps
+        for t0 in range(0, T, chunk_steps):
+            t1 = min(T, t0 + chunk_steps)
+            solver.run_day(ds, plan, dplan, outputs, out=out, stream=stream, steps=(t0, t1))
+            if not own:
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                cstream.wait_event(ev)
+            copy_out(hb.out[t0:t1], out[t0:t1], cstream)
+        fin = torch.cuda.Event(enable_timing=True)
+        fin.record(cstream)
+        buf_free[bi] = fin
+        done.append(fin)
+        tm.append(time.perf_counter())
+        marks.append([b - a for a, b in zip(tm, tm[1:])])
+    solver.pipeline_marks = marks
+    return done

@@ -564,7 +564,8 @@ class HipSolver:
         self._tables = {}
         self.timer = _NoTimer()
         self.plan_ms: list[float] = []      # host planning time of each solve
-        self._staging = {}            # plan staging (pinned host + device), one per stream
+        self._staging = {}            # plan staging (pinned host + device), STAGING_DEPTH per stream
+        self._staging_turn = {}
 
     # -- SVF ---------------------------------------------------------------
     def upload(self, tile: TileInputs) -> DeviceScene:
@@ -623,12 +624,17 @@ class HipSolver:
         return cubes
 
     # -- day ---------------------------------------------------------------
+    STAGING_DEPTH = 4      # plan staging buffers per stream
+
     def upload_plan(self, plan: DayPlan, stream=None):
         """Copy the day's tables to HBM from persistent pinned staging, without a host sync.
 
-        The staging buffer is rewritten only after the event of its previous
-        copy has completed; the device buffers are reused in stream order
-        (the copy runs after every kernel queued before it on the stream)."""
+        Each stream rotates over :attr:`STAGING_DEPTH` staging buffers (pinned host + device), so
+        the host can enqueue that many days ahead. A staging buffer is rewritten only after the
+        event of its previous copy has completed, and its device half only after every kernel
+        queued before that copy (stream order), so a day never reads a plan rewritten under it
+        unless it is more than STAGING_DEPTH days behind on its stream, which the event wait rules
+        out."""
         parts = [np.ascontiguousarray(plan.steps).view(np.uint8),
                  np.ascontiguousarray(plan.sun_tab).view(np.uint8).reshape(-1),
                  np.ascontiguousarray(plan.model).view(np.uint8).reshape(-1)]
@@ -636,7 +642,9 @@ class HipSolver:
         offs = np.concatenate([[0], np.cumsum([(z + 255) // 256 * 256 for z in sizes])]).astype(int)
         total = int(offs[-1])
         s = torch.cuda.current_stream(self.device) if stream is None else stream
-        key = s.cuda_stream                # one staging buffer per stream: concurrent solves on two streams
+        turn = self._staging_turn.get(s.cuda_stream, 0)
+        self._staging_turn[s.cuda_stream] = turn + 1
+        key = (s.cuda_stream, turn % self.STAGING_DEPTH)
         st = self._staging.get(key)
         if st is None or st["host"].numel() < total:
             if st is not None:
