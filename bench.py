@@ -30,11 +30,16 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA
-# VALU issue: one wave64 VALU instruction per CU per clock (4 SIMDs, each taking 4 clocks per wave64
-# instruction on 16 lanes; the same figure as the non-packed FP32 FMA rate, 78.6 TFLOP/s)
+# VALU issue on MI355X (gfx950), measured with single-class FMA/add/compare/select/min-max/bit-field
+# loops: FP32 arithmetic, compares, selects, min/max and bit-field ops issue one wave64 instruction
+# per CU per clock (562-624 G/s measured of 614.4), 32-bit integer add and logic ops two (1138 G/s
+# of 1228.8), FP32 transcendentals one every two clocks. The roofline peak is the hardware maximum,
+# two per CU per clock; `frac_full_rate_issue` is the fraction of the one-per-clock rate at which
+# all but the integer add/logic ops issue.
 N_CU = 256
 CLOCK_GHZ = 2.4
-VALU_PEAK_GINSTR = N_CU * CLOCK_GHZ
+VALU_FULL_RATE_GINSTR = N_CU * CLOCK_GHZ
+VALU_PEAK_GINSTR = 2 * VALU_FULL_RATE_GINSTR
 
 
 def parse():
@@ -731,7 +736,8 @@ def main():
         ach = valu[key] / (avg_ms / 1e3) / 1e9
         tb = traffic.get(key)
         rooflines[name] = {"bound": "valu", "achieved": round(ach, 1), "peak": VALU_PEAK_GINSTR,
-                           "unit": "G wave64 VALU instr/s", "frac": round(ach / VALU_PEAK_GINSTR, 4), "traffic": tb,
+                           "unit": "G wave64 VALU instr/s", "frac": round(ach / VALU_PEAK_GINSTR, 4),
+                           "frac_full_rate_issue": round(ach / VALU_FULL_RATE_GINSTR, 4), "traffic": tb,
                            "hbm_frac": round(tb / (avg_ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4) if tb else None,
                            "valu_insts_per_launch": valu[key], "avg_launch_ms": round(avg_ms, 4),
                            "launches_per_solve": launches // 3,
