@@ -831,9 +831,28 @@ __device__ __forceinline__ float pow4(float x) {
     return (float)(d2 * d2);
 }
 
-// Walk-source tile in LDS: a 16 x 16 pixel tile plus the walk halo.
+// Walk-source tile in LDS: a 16 x 16 pixel tile plus the walk halo. Once every wave of the block has
+// walked, the step's diffuse-sky patch terms (4 float2 per patch, one per (sh, vegsh) state) take
+// its place.
 #define FLUX_TX 16
 #define FLUX_TY 16
+
+static size_t flux_lds_bytes(int walk_halo) {
+    if (walk_halo < 0) return 0;
+    const size_t tile = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t table = (size_t)SW_NPATCH * 4 * sizeof(float2);   // overlays the tile after the walk
+    return tile > table ? tile : table;
+}
+
+// a / c from the correctly rounded reciprocal y = 1 / c with one Markstein correction: the
+// correctly rounded quotient for every finite a with 2^-96 <= |a| < 2^120 (checked for every such
+// float32 a and c in {0.9, 2, 3, 9, 11, 12, 18, 22, 23, 44, 45}); a = +-0 gives +0. Three
+// instructions instead of the ten of an IEEE division. The operands here are sums of per-cell
+// float32 radiation and albedo terms: zero, or far inside that range.
+__device__ __forceinline__ float div_rc(float a, float c, float y) {
+    const float q = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-q, c, a), y, q);
+}
 
 // one 256-pixel block of the sun step; `tile` is the block's index within its tile.
 // PLANES also writes the reference's eight output planes of the march and wall tail
@@ -1090,36 +1109,50 @@ struct PassA {
     float Lside_sky, Lside_veg, Lside_sun, Lside_sh;
 };
 
+// The diffuse-sky terms of a day step per patch and (sh, vegsh) state st = sh | vegsh << 1:
+// {diffsh * lv, diffsh * lumchi * angl * ster}, diffsh = sh - (1 - vegsh) * 0.97 (solweig.py:2069-2073,
+// Kside_veg_v2022a), each product rounded in the reference's order. One LDS read per patch
+// replaces the state select and four products.
+__device__ __forceinline__ void diffuse_table(const sw_step_t *__restrict__ sp, float2 *__restrict__ tab) {
+    const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
+    for (int i = threadIdx.x; i < SW_NPATCH * 4; i += blockDim.x) {
+        const int p = i >> 2, st = i & 3;
+        const float diffsh = (st & 2) ? ((st & 1) ? 1.f : 0.f) : ((st & 1) ? d10 : d00);
+        const sw_patch_k_t &q = sp->pk[p];
+        tab[i] = make_float2(diffsh * q.lv, diffsh * q.lumchi * q.angl * q.ster);
+    }
+}
+
 template <bool DAY, bool ROLLED>
 __device__ __forceinline__ void patch_pass_a(const sw_step_t *__restrict__ sp, const uint32_t *msh,
                                              const uint32_t *mvs, const uint32_t *mv1, const uint32_t *mv2,
-                                             float hsvf, PassA &o) {
+                                             float hsvf, const float2 *__restrict__ s_diff, PassA &o) {
     float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
     float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f;
     float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f;
-    const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
+#pragma unroll
     for (int w = 0; w < SW_MASK_WORDS; ++w) {
         const uint32_t ash = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
         const uint32_t avs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
         const uint32_t av1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
         const uint32_t av2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
-        // per-word patch booleans: tvs = vegsh == 0 or vbsh == 0; tsh = (1 - sh) * vbsh == 1;
-        // tsky = sh and vegsh
-        uint32_t btvs = ~avs | ~(av1 | av2), btsh = ~ash & av1, btsky = ash & avs;
-        uint32_t bsh = ash, bvs = avs;
+        // per-word patch booleans, bit k = patch w * 32 + k: tvs = vegsh == 0 or vbsh == 0;
+        // tsh = (1 - sh) * vbsh == 1; tsky = sh and vegsh
+        const uint32_t btvs = ~avs | ~(av1 | av2), btsh = ~ash & av1, btsky = ash & avs;
         const int pend = min(w * 32 + 32, SW_NPATCH);
         auto patch = [&](int p) {
-            const float tvs = (float)(btvs & 1u), tsky = (float)(btsky & 1u);
-            const bool tsh = btsh & 1u;
-            const bool sh = bsh & 1u, vs = bvs & 1u;
-            btvs >>= 1; btsh >>= 1; btsky >>= 1; bsh >>= 1; bvs >>= 1;
+            const uint32_t k = (uint32_t)(p - w * 32);
+            const float tvs = (float)__builtin_amdgcn_ubfe(btvs, k, 1u);
+            const float tsky = (float)__builtin_amdgcn_ubfe(btsky, k, 1u);
+            const bool tsh = __builtin_amdgcn_ubfe(btsh, k, 1u) != 0u;
             const sw_patch_k_t q = sp->pk[p];
             const sw_patch_l_t u = sp->pl[p];
             // sum + c * b with b in {0, 1}: fma(b, c, sum) rounds once, like the reference
             if (DAY) {
-                const float diffsh = vs ? (sh ? 1.f : 0.f) : (sh ? d10 : d00);
-                aniLum = aniLum + diffsh * q.lv;
-                KsideD = KsideD + diffsh * q.lumchi * q.angl * q.ster;
+                const uint32_t st = __builtin_amdgcn_ubfe(ash, k, 1u) | (__builtin_amdgcn_ubfe(avs, k, 1u) << 1);
+                const float2 dv = s_diff[4 * p + (int)st];
+                aniLum = aniLum + dv.x;
+                KsideD = KsideD + dv.y;
                 Kref_veg = __builtin_fmaf(tvs, q.k_sh, Kref_veg);
             }
             Ldown_sky = __builtin_fmaf(tsky, q.lsky_down, Ldown_sky);
@@ -1292,12 +1325,14 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
         }
     }
     __syncthreads();
-    if (!valid) return;
+    // lanes off the grid stay to the end (the block meets a barrier after the walk); they read
+    // pixel 0 and store nothing
     if ((!WALK && sp->is_day) || (day && !staged)) {
-        // the shared tile is smaller than the walk reach: refuse loudly
+        // the shared tile is smaller than the walk reach: refuse loudly (uniform over the block)
         int k = 0;
-        for (int b = 1; b <= 128; b <<= 1)
-            if (out_mask & b) out[(long)(k++) * n + t] = __builtin_nanf("");
+        if (valid)
+            for (int b = 1; b <= 128; b <<= 1)
+                if (out_mask & b) out[(long)(k++) * n + t] = __builtin_nanf("");
         return;
     }
     const float b_t = bu[t];
@@ -1321,6 +1356,8 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
         int nfirst = 0;
         while (nfirst < nsec && (float)(nfirst + 1) <= first) ++nfirst;
         const float fp1 = first + 1.f, sp1 = second + 1.f;
+        const float r_fp1 = 1.f / fp1, r_sp1 = 1.f / sp1, r_first = 1.f / first, r_second = 1.f / second;
+        const float r_09 = 1.f / 0.9f, r_18 = 1.f / 18.f, r_9 = 1.f / 9.f;
         const uint32_t wf = walkface[t];
         const float nb = b_t * -1.f + 1.f;
         const float fb = b_t;                    // buildings is 0/1
@@ -1346,17 +1383,20 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
             const float wi2 = (walbwallnosh > 0.f) ? 1.f : 0.f;
             const float lw2 = keep1 ? 0.f : wlwall;
             const float aw2 = keep1 ? 0.f : walbwall;
-            const float gl1 = ((f_wlwall + F.wlupsh) / fp1) * wsi1 + (F.wlupsh) / first * (wsi1 * -1.f + 1.f);
-            const float gl2 = ((lw2 + A.wlupsh) / sp1) * wsi2 + (A.wlupsh) / second * (wsi2 * -1.f + 1.f);
-            const float ga1 = ((f_walbwall + F.walbsh) / fp1) * wsi1 + (F.walbsh) / first * (wsi1 * -1.f + 1.f);
-            const float ga2 = ((aw2 + A.walbsh) / sp1) * wsi2 + (A.walbsh) / second * (wsi2 * -1.f + 1.f);
-            const float gn1 = ((f_walbwallnosh + F.walbnosh) / fp1) * wi1 + (F.walbnosh) / first * (wi1 * -1.f + 1.f);
-            const float gn2 = ((walbwallnosh + A.walbnosh) / second) * wi2 + (A.walbnosh) / second * (wi2 * -1.f + 1.f);
-            float gLj = (gl1 * 0.5f + gl2 * 0.4f) / 0.9f;
+            // the reference's (x / c1) * w + (y / c2) * (1 - w) with w in {0, 1}: one of the two
+            // products is a zero added to the other, so only the selected quotient is formed
+            const float gl1 = wsi1 != 0.f ? div_rc(f_wlwall + F.wlupsh, fp1, r_fp1) : div_rc(F.wlupsh, first, r_first);
+            const float gl2 = wsi2 != 0.f ? div_rc(lw2 + A.wlupsh, sp1, r_sp1) : div_rc(A.wlupsh, second, r_second);
+            const float ga1 = wsi1 != 0.f ? div_rc(f_walbwall + F.walbsh, fp1, r_fp1) : div_rc(F.walbsh, first, r_first);
+            const float ga2 = wsi2 != 0.f ? div_rc(aw2 + A.walbsh, sp1, r_sp1) : div_rc(A.walbsh, second, r_second);
+            const float gn1 = wi1 != 0.f ? div_rc(f_walbwallnosh + F.walbnosh, fp1, r_fp1)
+                                         : div_rc(F.walbnosh, first, r_first);
+            const float gn2 = div_rc(wi2 != 0.f ? walbwallnosh + A.walbnosh : A.walbnosh, second, r_second);
+            float gLj = div_rc(gl1 * 0.5f + gl2 * 0.4f, 0.9f, r_09);
             gLj = gLj + lup_t * nb;
-            float gAj = (ga1 * 0.5f + ga2 * 0.4f) / 0.9f;
+            float gAj = div_rc(ga1 * 0.5f + ga2 * 0.4f, 0.9f, r_09);
             gAj = gAj + alb_t * nb * shadow;
-            float gNj = (gn1 * 0.5f + gn2 * 0.4f) / 0.9f;
+            float gNj = div_rc(gn1 * 0.5f + gn2 * 0.4f, 0.9f, r_09);
             gNj = gNj * b_t + alb_t * nb;
             gL += gLj; gA += gAj; gN += gNj;
             const int dm = md->walk_dir[j];
@@ -1366,16 +1406,16 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
             if (dm & 8) { gLN += gLj; gAN += gAj; gNN += gNj; }
         }
         const float lup_air = se_t * sp->tak4;
-        const float gLup = gL / 18.f + lup_air;
-        galb = gA / 18.f; gnosh = gN / 18.f;
-        const float gLupE = gLE / 9.f + lup_air;
-        galbE = gAE / 9.f; gnoshE = gNE / 9.f;
-        const float gLupS = gLS / 9.f + lup_air;
-        galbS = gAS / 9.f; gnoshS = gNS / 9.f;
-        const float gLupW = gLW / 9.f + lup_air;
-        galbW = gAW / 9.f; gnoshW = gNW / 9.f;
-        const float gLupN = gLN / 9.f + lup_air;
-        galbN = gAN / 9.f; gnoshN = gNN / 9.f;
+        const float gLup = div_rc(gL, 18.f, r_18) + lup_air;
+        galb = div_rc(gA, 18.f, r_18); gnosh = div_rc(gN, 18.f, r_18);
+        const float gLupE = div_rc(gLE, 9.f, r_9) + lup_air;
+        galbE = div_rc(gAE, 9.f, r_9); gnoshE = div_rc(gNE, 9.f, r_9);
+        const float gLupS = div_rc(gLS, 9.f, r_9) + lup_air;
+        galbS = div_rc(gAS, 9.f, r_9); gnoshS = div_rc(gNS, 9.f, r_9);
+        const float gLupW = div_rc(gLW, 9.f, r_9) + lup_air;
+        galbW = div_rc(gAW, 9.f, r_9); gnoshW = div_rc(gNW, 9.f, r_9);
+        const float gLupN = div_rc(gLN, 9.f, r_9) + lup_air;
+        galbN = div_rc(gAN, 9.f, r_9); gnoshN = div_rc(gNN, 9.f, r_9);
         // ---- thermal delay (TsWaveDelay_2015a x 6, shared weights) ----
         const float tg0[6] = {gLup, gLupE, gLupS, gLupW, gLupN, sp->tg_class[k_t] * shadow + sp->ta};
         float L[6];
@@ -1388,7 +1428,7 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
             } else {
                 L[q] = tg0[q] * sp->tw_omw + T1 * sp->tw_w;
             }
-            state[(long)q * n + t] = T1;
+            if (valid) state[(long)q * n + t] = T1;
         }
         Lup = L[0]; LupE = L[1]; LupS = L[2]; LupW = L[3]; LupN = L[4];
         // ---- cylinder wedge F_sh (cylindric_wedge) ----
@@ -1422,12 +1462,18 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     // the value the full product takes (adding +0 elsewhere changes no sum
     // that starts at +0 and only grows); each accumulator keeps the
     // reference's patch order ----
+    if (staged) {
+        // every wave of the block is done with the walk tile: the step's diffuse-sky table replaces it
+        __syncthreads();
+        diffuse_table(sp, (float2 *)s_src);
+        __syncthreads();
+    }
     uint32_t msh[SW_MASK_WORDS], mvs[SW_MASK_WORDS], mv1[SW_MASK_WORDS], mv2[SW_MASK_WORDS];
     load_masks(masks, n, t, msh, mvs, mv1, mv2);
     const float hsvf = tan_asvf[t];
     PassA pa;
-    if (day) patch_pass_a<true, WALK>(sp, msh, mvs, mv1, mv2, hsvf, pa);
-    else patch_pass_a<false, WALK>(sp, msh, mvs, mv1, mv2, hsvf, pa);
+    if (day) patch_pass_a<true, WALK>(sp, msh, mvs, mv1, mv2, hsvf, (const float2 *)s_src, pa);
+    else patch_pass_a<false, WALK>(sp, msh, mvs, mv1, mv2, hsvf, nullptr, pa);
     const float aniLum = pa.aniLum, KsideD = pa.KsideD, Kref_sun = pa.Kref_sun, Kref_sh = pa.Kref_sh,
                 Kref_veg = pa.Kref_veg;
     const float Ldown_sky = pa.Ldown_sky, Ldown_veg = pa.Ldown_veg, Ldown_sun = pa.Ldown_sun, Ldown_sh = pa.Ldown_sh;
@@ -1450,20 +1496,23 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     // ---- pass B: reflected longwave ----
     const float pi_f = 3.14159265358979323846f;
     const float refl = ((Ldown_sky + Lup) * (1.f - md->ewall) * 0.5f) / pi_f;
+#pragma unroll
     for (int w = 0; w < SW_MASK_WORDS; ++w) {
         // reflected term uses patches where any of sh, vegsh, vbsh is 0
-        uint32_t tr = ~((w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4]) &
+        const uint32_t tr = ~((w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4]) &
                         (w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4]) &
                         ((w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4]) |
                          (w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4])));
         const int pend = min(w * 32 + 32, SW_NPATCH);
 #pragma unroll 4
         for (int p = w * 32; p < pend; ++p) {
-            const float trf = (tr & 1u) ? 1.f : 0.f;
-            tr >>= 1;
+            const float trf = (float)__builtin_amdgcn_ubfe(tr, (uint32_t)(p - w * 32), 1u);
             const sw_patch_k_t q = sp->pk[p];
-            Lside_ref = Lside_ref + refl * q.ster * q.cos_alt * trf;
-            Ldown_ref = Ldown_ref + refl * q.ster * q.sin_alt * trf;
+            // sum + ((refl * ster) * c) * trf with trf in {0, 1}: the product rounded in the
+            // reference's order, then added by an fma whose flag product is exact
+            const float rs = refl * q.ster;
+            Lside_ref = __builtin_fmaf(trf, rs * q.cos_alt, Lside_ref);
+            Ldown_ref = __builtin_fmaf(trf, rs * q.sin_alt, Ldown_ref);
         }
     }
     const float Lside = Lside_sky + Lside_veg + Lside_sh + Lside_sun + Lside_ref;
@@ -1474,6 +1523,7 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     const float Sstr = md->absK * (Kside * md->Fcyl + (Kdown + Kup) * md->Fup + (Knorth + Keast + Ksouth + Kwest) * md->Fside)
                      + md->absL * ((Ldown + Lup) * md->Fup + Lside * md->Fcyl + (Lnorth + Least + Lsouth + Lwest) * md->Fside);
     const float tmrt = sqrtf(sqrtf(Sstr / (md->absL * md->sbc))) - 273.2f;
+    if (!valid) return;
     int k = 0;
     if (out_mask & 1) out[(long)(k++) * n + t] = tmrt;
     if (out_mask & 2) {
@@ -1803,7 +1853,7 @@ int sw_solve_day_tiles(const sw_tile_t *tile, const sw_step_t *steps, const sw_m
         if (rc) return rc;
     }
     if (nflux == 0) return SW_OK;
-    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t lds = flux_lds_bytes(walk_halo);
     if (lds > 65536) {
         rc = hip_check(hipFuncSetAttribute((const void *)k_flux_tiles<true>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -1990,8 +2040,7 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
     if (walk_halo < -1 || walk_halo > 48) return set_err(SW_E_ARG, "sw_flux_step: walk halo out of range");
     const int tiles = ((cols + FLUX_TX - 1) / FLUX_TX) * ((rows + FLUX_TY - 1) / FLUX_TY);
     const int grid = ((tiles + 7) / 8) * 8;
-    const size_t lds = walk_halo < 0 ? 0
-                                     : (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t lds = flux_lds_bytes(walk_halo);
     if (lds > 65536) {
         rc = hip_check(hipFuncSetAttribute((const void *)k_flux_step<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds), "hipFuncSetAttribute");
@@ -2073,7 +2122,7 @@ int sw_solve_batch(const sw_batch_item_t *items_dev, const sw_batch_item_t *item
     }
     const int sun_grid = (int)(((sun_total + 7) / 8) * 8);
     const int flux_grid = (int)(((flux_total + 7) / 8) * 8);
-    const size_t lds = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t lds = flux_lds_bytes(walk_halo);
     if (lds > 65536) {
         rc = hip_check(hipFuncSetAttribute((const void *)k_flux_batch<true>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),

@@ -703,28 +703,27 @@ class HipSolver:
             t0, t1 = (int(v) for v in steps)
             if not 0 <= t0 < t1 <= T or per_launch:
                 raise ValueError(f"step range {steps} of a {T}-step day")
-            if reset_state and t0 == 0:
-                with torch.cuda.stream(stream) if stream is not None else _nullcontext():
-                    ds.state.zero_()
-            # the day flags of the range: sw_solve_day reads is_day[i - t0]
+            # the day flags of the range: sw_solve_day reads is_day[i - t0]; the state is cleared by the
+            # library's fill kernel on the day's own stream (a torch fill would run on torch's current
+            # stream and could land inside another stream's day)
             is_day = (ctypes.c_int32 * (t1 - t0))(*[int(x) for x in plan.is_day[t0:t1]])
             base = P(out) + t0 * k * ds.n * 4
             L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
-                   P(dplan["sun_tab"]), is_day, t0, t1, base, mask, plan.walk_halo, 0, sp)
+                   P(dplan["sun_tab"]), is_day, t0, t1, base, mask, plan.walk_halo, int(reset_state and t0 == 0), sp)
             return out
-        if reset_state:
-            ds.state.zero_()
         is_day = (ctypes.c_int32 * T)(*[int(x) for x in plan.is_day])
         if not per_launch:
-            # one library call enqueues the whole day
+            # one library call enqueues the whole day (state cleared on the day's stream)
             if isinstance(self.timer, KernelTimer):
                 evs = self.timer.day_events(plan.is_day)
                 L.call("sw_solve_day_timed", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
-                       P(dplan["sun_tab"]), is_day, evs, 0, T, P(out), mask, plan.walk_halo, 0, sp)
+                       P(dplan["sun_tab"]), is_day, evs, 0, T, P(out), mask, plan.walk_halo, int(bool(reset_state)), sp)
             else:
                 L.call("sw_solve_day", ctypes.byref(ds.tile_struct()), P(dplan["steps"]), P(dplan["model"]),
-                       P(dplan["sun_tab"]), is_day, 0, T, P(out), mask, plan.walk_halo, 0, sp)
+                       P(dplan["sun_tab"]), is_day, 0, T, P(out), mask, plan.walk_halo, int(bool(reset_state)), sp)
             return out
+        if reset_state:
+            L.call("sw_fill_f32", P(ds.state), 0.0, 6 * ds.n, sp)
         # per-launch path (kernel timing spans around each launch)
         stride = k * ds.n * 4
         base = P(out)
