@@ -70,7 +70,14 @@ def parse():
                          "shared copy stream, or on a copy stream per lane")
     ap.add_argument("--march", default="",
                     help="tile workload: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
-    ap.add_argument("--slots", type=int, default=3, help="execution slots of --workload serve")
+    ap.add_argument("--slots", type=int, default=3, help="most execution slots of --workload serve")
+    ap.add_argument("--serve-hbm-gb", type=float, default=0.0,
+                    help="HBM budget of the serving slot pool in GiB (0: free HBM less 10 %% of the device)")
+    ap.add_argument("--extras", default="serve,surrogate,batch",
+                    help="tile workload: also run these workloads and add their summaries to the line "
+                         "(serve: 1000-request mix at --rate; surrogate: shipped weights on a 2000² city; "
+                         "batch: the batched tiles and edit scenarios, 3 repetitions); 'none' for the tile alone")
+    ap.add_argument("--extra-requests", type=int, default=1000, help="requests of the serve extra")
     ap.add_argument("--chunk-steps", type=int, default=4, help="day steps a serving slot enqueues at a time")
     ap.add_argument("--edits", type=int, default=200, help="edits timed by --workload edits")
     ap.add_argument("--train-seconds", type=float, default=240.0, help="surrogate training budget")
@@ -139,12 +146,11 @@ def cpu_baseline(site, sun, met):
     fields, shm, vm, bm = D.svf(sc)
     D.day(sc, fields, shm, vm, bm, site.walls, site.aspect, site.landcover, met, sun, site.latitude)
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cols = site.dsm.shape[1]
-    return dict(value=round(rows * cols * met.shape[0] / dt / 1e6, 5), unit="Mpixel-steps/s", cores=cores,
-                kind="port", sample=f"C oracle (plane-sweep restatement of the reference CPU path) full solve "
-                                    f"of a {rows}x{cols}x{met.shape[0]}h tile from the same generator, "
-                                    f"{dt:.1f} s wall")
+    return dict(value=round(rows * cols * met.shape[0] / dt / 1e6, 5), unit="Mpixel-steps/s", cores=D.threads(),
+                kind="port", cpu_model=D.cpu_model(), cpus_visible=len(os.sched_getaffinity(0)),
+                sample=f"C oracle (plane-sweep restatement of the reference CPU path, OpenMP over rows) full "
+                       f"solve of a {rows}x{cols}x{met.shape[0]}h tile from the same generator, {dt:.1f} s wall")
 
 
 def c3_inputs(seed, dev):
@@ -166,8 +172,9 @@ def c3_inputs(seed, dev):
     return tiles, edits, base
 
 
-def run_batch(args):
-    """Batched exact solves: every group runs its SVF sweeps, then one sw_solve_batch day loop."""
+def run_batch(args, reps=None):
+    """Batched exact solves: every group runs its SVF sweeps, then one sw_solve_batch day loop.
+    Returns the JSON line (``reps`` timed repetitions, default ``--steps``)."""
     import numpy as np
     import torch
     from solweig_project_amd import _lib as L
@@ -210,7 +217,7 @@ def run_batch(args):
         for ds in scenes:
             ds.check_flags()
         times = []
-        for _ in range(args.steps):
+        for _ in range(reps or args.steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             b = solve_group(scenes)
@@ -259,7 +266,7 @@ def run_batch(args):
         "metric": "Mpixel-steps/s (batched exact solves, 4 uneven tiles + 64 edit scenarios of 500², 24 h)",
         "value": round(tot_px * T / (tot_ms / 1e3) / 1e6, 3),
         "unit": "Mpixel-steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(tot_ms, 3),
+        "n_gpus": 1, "steps": reps or args.steps, "warmup": 1, "ms_per_step": round(tot_ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded city tiles + edit scenarios; inputs resident in HBM, outputs left in HBM)",
         "config": {"workload": "batch: tiles 300², 700², 1200², 2000² and 64 edit scenarios (1-20 tree or "
@@ -268,7 +275,7 @@ def run_batch(args):
                                "items", "time_steps": T},
         "groups": res,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def run_edits(args):
@@ -489,13 +496,14 @@ def run_surrogate(args):
             "windows": train_info["windows"], "first_loss": train_info["losses"][0][1],
             "last_loss": train_info["losses"][-1][1]},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def run_serve(args, pipeline):
+def run_serve(args, pipeline, requests=None):
     """Serving mix: Poisson arrivals (60 % single-tree edits, 30 % met-only, 10 % full re-solves) against a
-    device-resident 2000 x 2000 x 24 h site; latency = completion - arrival, queueing included."""
-    import numpy as np
+    device-resident 2000 x 2000 x 24 h site; latency = arrival to results in pinned host memory, queueing
+    included; nearest-rank percentiles. Slots are admitted by HBM footprint against ``--serve-hbm-gb``
+    (default: the free HBM less 10 % of the device). Returns the JSON line."""
     import torch
     from solweig_project_amd import incremental as I
     from solweig_project_amd import serving as SV
@@ -515,16 +523,20 @@ def run_serve(args, pipeline):
     st = I.solve_site(solver, tile, mets[0], sun, base.latitude)
     from solweig_project_amd.pipeline import Job
     pipeline.warm(st.tile, Job(mets[0], sun, base.latitude))
+    budget = int(args.serve_hbm_gb * 2 ** 30) if args.serve_hbm_gb > 0 else None
     server = SV.Server(solver, st, mets, sun, base.latitude, pipeline, slots=args.slots,
-                       chunk_steps=args.chunk_steps)
-    # warm every path once
-    warm = SV.poisson_workload(6, rate=100.0, mix={"edit": 0.34, "met": 0.33, "full": 0.33}, seed=seed + 7,
-                               site_tile=st.tile, met_pool=mets)
+                       chunk_steps=args.chunk_steps, hbm_budget=budget)
+    # warm every path once (and create the slots the mix will use)
+    warm = SV.poisson_workload(3 * args.slots, rate=1000.0, mix={"edit": 0.34, "met": 0.33, "full": 0.33},
+                               seed=seed + 7, site_tile=st.tile, met_pool=mets)
     server.run(warm, checksums=False)
-    reqs = SV.poisson_workload(args.requests, rate=args.rate, mix={"edit": 0.6, "met": 0.3, "full": 0.1},
+    server.deferred = 0
+    nreq = requests or args.requests
+    reqs = SV.poisson_workload(nreq, rate=args.rate, mix={"edit": 0.6, "met": 0.3, "full": 0.1},
                                seed=seed + 8, site_tile=st.tile, met_pool=mets)
-    log(f"serving {len(reqs)} requests at {args.rate}/s over {reqs[-1].arrival:.1f} s with "
-        f"{len(server.slots)} slots")
+    log(f"serving {len(reqs)} requests at {args.rate}/s over {reqs[-1].arrival:.1f} s; HBM budget "
+        f"{server.budget / 2 ** 30:.1f} GiB, slot {server.slot_bytes['site'] / 2 ** 30:.2f} GiB, "
+        f"{len(server.slots)} slots warm")
     t0 = time.perf_counter()
     done = server.run(reqs, checksums=False)
     wall = time.perf_counter() - t0
@@ -532,27 +544,39 @@ def run_serve(args, pipeline):
     T = st.plan.n_steps
     px_steps = sum(n * n * T for r in done if r.kind in ("met", "full"))
     line = {
-        "metric": "p99 request latency (ms), serving mix on a 2000² x 24 h site",
+        "metric": "p99 request latency (ms), serving mix on a 2000² x 24 h site, results on the host",
         "value": allp["p99_ms"], "unit": "ms",
         "n_gpus": 1, "steps": len(done), "warmup": len(warm), "ms_per_step": allp["mean_ms"],
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded 2000² city; Poisson arrivals, seeded request mix)",
         "config": {"workload": f"Poisson arrivals at {args.rate}/s: 60% single-tree edits, 30% met-only day "
-                               f"re-runs, 10% full re-solves; continuous batching over {len(server.slots)} slots "
-                               f"(own stream and scene copy each, days fed {args.chunk_steps} steps at a time), edits first and in order "
-                               "on the site",
-                   "rows": n, "cols": n, "requests": len(done)},
+                               f"re-runs, 10% full re-solves; continuous batching over up to {args.slots} slots "
+                               f"admitted by HBM footprint (own stream and scene copy each, days fed "
+                               f"{args.chunk_steps} steps at a time, each chunk copied to pinned host memory on the "
+                               "DMA engines), edits first and in order on the site, each edit's recomputed "
+                               "output window copied to the host; latency = arrival to results on the host, "
+                               "nearest-rank percentiles",
+                   "rows": n, "cols": n, "requests": len(done), "rate_per_s": args.rate},
         "all": allp,
         "edit": SV.percentiles(done, ("edit",)),
         "met": SV.percentiles(done, ("met",)),
         "full": SV.percentiles(done, ("full",)),
+        "admission": {"hbm_budget_bytes": server.budget, "slot_bytes": server.slot_bytes["site"],
+                      "edit_reserve_bytes": server.edit_reserve, "slots": len(server.slots),
+                      "peak_busy_slots": server.peak_busy, "deferred_requests": server.deferred,
+                      "peak_pool_bytes": server.peak_pool_bytes},
         "wall_s": round(wall, 2),
         "completed_per_s": round(len(done) / wall, 2),
         "met_full_mpixel_steps_per_s": round(px_steps / wall / 1e6, 1),
         "edit_modes": {m: sum(1 for r in done if r.kind == "edit" and r.payload.get("mode") == m)
                        for m in ("tiles", "full", "none")},
+        "edit_result_mb_mean": round(sum(r.payload.get("result_bytes", 0) for r in done if r.kind == "edit")
+                                     / max(1, sum(1 for r in done if r.kind == "edit")) / 1e6, 3),
     }
-    print(json.dumps(line), flush=True)
+    del server, st
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return line
 
 
 def main():
@@ -562,23 +586,25 @@ def main():
               "design-only (tiles with read-only halos, no collectives)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
-        run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
         return
     if args.workload == "edits":
         run_edits(args)
         return
     if args.workload == "surrogate":
-        run_surrogate(args)
-        return
-    if args.workload == "serve":
-        from solweig_project_amd.pipeline import PlanPipeline
-        pipeline = PlanPipeline(workers=2)      # CPU-only planner processes, started before the GPU is used
-        try:
-            run_serve(args, pipeline)
-        finally:
-            pipeline.close()
+        print(json.dumps(run_surrogate(args)), flush=True)
         return
     from solweig_project_amd.pipeline import Job, PlanPipeline, solve_many
+    extras = [x for x in args.extras.split(",") if x and x != "none"]
+    serve_pipeline = None
+    if args.workload == "serve" or "serve" in extras:
+        serve_pipeline = PlanPipeline(workers=2)   # CPU-only planner processes, started before the GPU is used
+    if args.workload == "serve":
+        try:
+            print(json.dumps(run_serve(args, serve_pipeline)), flush=True)
+        finally:
+            serve_pipeline.close()
+        return
     pipeline = PlanPipeline(workers=3)      # CPU-only planner processes, started before the GPU is used
     import numpy as np
     import torch
@@ -845,7 +871,51 @@ def main():
         "host_plan_ms": round(plan_ms, 3),
         "walls_aspect": walls_info,
     }
+    line.update(run_extras(args, extras, serve_pipeline))
     print(json.dumps(line), flush=True)
+
+
+def run_extras(args, extras, serve_pipeline):
+    """Summaries of the other configurations, measured after the tile workload in the same run: the
+    serving mix (edit p99 to the host), the gated SVF surrogate with the shipped weights, and the batched
+    tiles and edit scenarios. Each full record is logged to stderr."""
+    import torch
+    out = {}
+    for name in extras:
+        t0 = time.perf_counter()
+        if name == "serve":
+            try:
+                r = run_serve(args, serve_pipeline, requests=args.extra_requests)
+            finally:
+                serve_pipeline.close()
+            out["serve"] = {"edit_p99_ms": r["edit"]["p99_ms"], "edit_p50_ms": r["edit"]["p50_ms"],
+                            "edits": r["edit"]["n"], "all_p99_ms": r["all"]["p99_ms"],
+                            "met_p99_ms": (r["met"] or {}).get("p99_ms"), "full_p99_ms": (r["full"] or {}).get("p99_ms"),
+                            "requests": r["config"]["requests"], "rate_per_s": r["config"]["rate_per_s"],
+                            "completed_per_s": r["completed_per_s"], "admission": r["admission"],
+                            "latency": "arrival to results in pinned host memory; nearest-rank percentiles",
+                            "site": "2000x2000x24h"}
+        elif name == "surrogate":
+            r = run_surrogate(args)
+            bar = {"svf_max_abs": 0.02, "tmrt_max_abs_K": 0.5}
+            ge, te = r["gated_error"]["max"], r["derived_tmrt_error"]["max_K"]
+            out["surrogate"] = {"speedup": r["value"], "exact_svf_ms": r["exact_svf_ms"], "gated_ms": r["gated_ms"],
+                                "gated_svf_error_max": ge, "derived_tmrt_error_max_K": te, "bar": bar,
+                                "meets_bar": bool(ge <= bar["svf_max_abs"] and te <= bar["tmrt_max_abs_K"]),
+                                "accepted_windows": f"{r['accepted']}/{r['windows']}", "gate": r.get("gate"),
+                                "params": r["config"]["params"], "site": "2000x2000"}
+        elif name == "batch":
+            r = run_batch(args, reps=3)
+            out["batch"] = {"value": r["value"], "unit": r["unit"], "ms": r["ms_per_step"],
+                            "groups": {g: {k: v[k] for k in ("items", "mpx", "ms", "mpixel_steps_per_s")}
+                                       for g, v in r["groups"].items()}}
+        else:
+            raise SystemExit(f"unknown extra {name!r}")
+        log(f"{name}: {json.dumps(r)}")
+        out[name]["seconds"] = round(time.perf_counter() - t0, 1)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

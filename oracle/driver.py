@@ -62,6 +62,24 @@ def _declare(L):
         ctypes.POINTER(OrOverride), vp]
     L.or_static.argtypes = [vp, vp, cl, vp, vp, vp]
     L.or_walk_sources.argtypes = [vp, vp, vp, vp, vp, vp, vp, cl, cf, cf, cf, cf, vp, vp, vp]
+    L.or_threads.restype = ci
+
+
+def threads() -> int:
+    """OpenMP threads the oracle's parallel regions run on."""
+    return int(lib().or_threads())
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (``/proc/cpuinfo``), or the platform's processor string."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 class OrStep(ctypes.Structure):

@@ -28,6 +28,7 @@
  * Parallelism: OpenMP over rows inside each plane operation.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -668,3 +669,6 @@ void or_walk_sources(const float *sh_sun, const float *vegsh_sun, const float *w
         sunwall[t] = (q == 1.f) ? 1.f : 0.f;
     }
 }
+
+/* Threads an OpenMP region of this library runs on (what the timed CPU baseline used). */
+int or_threads(void) { return omp_get_max_threads(); }

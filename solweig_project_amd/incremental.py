@@ -343,7 +343,7 @@ def apply_edit_window(solver: HipSolver, st: SiteState, box, dsm=None, cdsm=None
         _sync(stream)
         st.amax = amax
         st.offsets = march_offsets(solver, tile, st.plan, amax)
-        info.update(mode="full", ms=(time.perf_counter() - t0) * 1e3)
+        info.update(mode="full", out_box=(0, rows, 0, cols), ms=(time.perf_counter() - t0) * 1e3)
         st.log.append(info)
         return info
     svf_t, sun_t, flux_t = edit_tiles(st, hbox, ebox)
@@ -366,7 +366,13 @@ def apply_edit_window(solver: HipSolver, st: SiteState, box, dsm=None, cdsm=None
            int(ids[1].size), P(dev_ids[2]), int(ids[2].size), sp)
     _sync(stream)
     g = TileGrid(rows, cols)
+    # the pixel box of every tile whose outputs were recomputed (an edit's result window)
+    ty, tx = np.nonzero(flux_t)
+    out_box = None
+    if ty.size:
+        out_box = (int(ty.min()) * TILE, min(rows, (int(ty.max()) + 1) * TILE),
+                   int(tx.min()) * TILE, min(cols, (int(tx.max()) + 1) * TILE))
     info.update(mode="tiles", svf_tiles=int(ids[0].size), sun_tiles=int(ids[1].size), flux_tiles=int(ids[2].size),
-                grid_tiles=g.ty * g.tx, ms=(time.perf_counter() - t0) * 1e3)
+                grid_tiles=g.ty * g.tx, out_box=out_box, ms=(time.perf_counter() - t0) * 1e3)
     st.log.append(info)
     return info

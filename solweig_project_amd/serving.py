@@ -19,6 +19,19 @@ Edits go first: queued edits are applied before slot requests are admitted
 or advanced, and slots enqueue their day a few steps at a time (one chunk in
 flight per slot), so an edit shares the GPU with at most one chunk per slot.
 
+Admission is by HBM footprint against a byte budget (by default the free HBM
+at start-up less a reserve): a met-only or full request needs an execution
+slot, whose footprint (scene copy, march workspace, state, day outputs) is
+known from the site's shape; slots are created when a request needs one and
+the slot pool still fits the budget beside the reserve kept for edits (an
+edit's result window, at most one day of the site's outputs), and reused
+after. A request that finds no free slot and no room for another waits.
+
+A request completes when its results are in pinned host memory: an edit
+copies the outputs of every tile it recomputed (the bounding box, all steps);
+a met-only or full request copies its whole day, chunk by chunk, on the DMA
+engines (``sw_copy_to_host``).
+
 Every request's result equals the same request served alone on the site
 version it was admitted against; ``checksum`` fingerprints results for that
 comparison.
@@ -32,6 +45,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from . import _lib as L
 from . import incremental as I
 from .pipeline import Job, PlanPipeline
 from .solver import DeviceScene, HipSolver, _owned_plan
@@ -45,37 +59,44 @@ class Request:
     start: float = 0.0
     finish: float = 0.0
     result: object = None
+    deferred: bool = False    # waited for a slot to free up or fit the HBM budget
 
     @property
     def latency_ms(self) -> float:
         return (self.finish - self.arrival) * 1e3
 
 
-def poisson_workload(n: int, rate: float, mix: dict, seed: int, site_tile, met_pool: list) -> list:
+def poisson_workload(n: int, rate: float, mix: dict, seed: int, site_tile, met_pool: list,
+                     sites: dict | None = None) -> list:
     """``n`` requests with exponential inter-arrival times at ``rate`` per second; ``mix`` maps kind
     to probability. Edits add a tree crown of radius 2-6 px on open ground or clear the canopy within that
-    radius around a canopy cell."""
+    radius around a canopy cell. ``sites`` (name -> tile): each request names one of them, uniformly
+    (``site_tile`` is then unused)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     kinds = list(mix)
     p = np.array([mix[k] for k in kinds], np.float64)
     p /= p.sum()
-    rows, cols = site_tile.shape
-    ground = np.flatnonzero((site_tile.dsm == site_tile.dem).ravel())
-    canopy = np.flatnonzero((site_tile.cdsm > 0).ravel())
+    names = list(sites) if sites else [None]
+    where = {}
+    for name in names:
+        tl = site_tile if name is None else sites[name]
+        where[name] = (tl.shape[1], np.flatnonzero((tl.dsm == tl.dem).ravel()), np.flatnonzero((tl.cdsm > 0).ravel()))
     t = 0.0
     out = []
     for _ in range(n):
         t += float(rng.exponential(1.0 / rate))
         kind = kinds[int(rng.choice(len(kinds), p=p))]
-        payload = {}
+        name = names[int(rng.integers(0, len(names)))] if sites else None
+        cols, ground, canopy = where[name]
+        payload = {} if name is None else {"site": name}
         if kind == "edit":
             remove = bool(rng.random() < 0.5) and canopy.size > 0
-            cells = canopy if remove else ground          # removals clear an existing crown
-            ci, cj = divmod(int(cells[rng.integers(0, cells.size)]), cols)
+            pool = canopy if remove else ground           # removals clear an existing crown
+            ci, cj = divmod(int(pool[rng.integers(0, pool.size)]), cols)
             r = int(rng.integers(2, 7))
-            payload = dict(center=(ci, cj), radius=r, height=float(rng.uniform(3.0, 15.0)), remove=remove)
+            payload.update(center=(ci, cj), radius=r, height=float(rng.uniform(3.0, 15.0)), remove=remove)
         elif kind == "met":
-            payload = dict(met=int(rng.integers(0, len(met_pool))))
+            payload.update(met=int(rng.integers(0, len(met_pool))))
         out.append(Request(kind, t, payload))
     return out
 
@@ -107,15 +128,33 @@ def checksum(t: torch.Tensor) -> tuple:
     return int(nan.sum().item()), float(y.sum().item()), float((y * torch.remainder(w, 9973.0)).sum().item())
 
 
-class Slot:
-    """An execution slot: a stream and a scene copy with its own state, walk sources and outputs."""
+def slot_footprint(st: I.SiteState) -> int:
+    """HBM bytes of one execution slot for the site ``st`` (what :class:`Slot` allocates): the scene's
+    input rasters, composed planes, packed masks, SVF fields, static planes, walk sources, state,
+    padded march heights and step counts, and a day of outputs."""
+    rows, cols = st.tile.shape
+    n = rows * cols
+    inputs = sum(v.numel() * v.element_size() for v in st.scene.inputs.values() if v is not None)
+    # hpack, vegdsm2, buildings, cls, aspect, masks, fields, svfbuveg, tan_asvf, tan_sa, walkface, src, state
+    per_px = 16 + 4 + 4 + 1 + 4 + 4 * L.MASK_PLANES * L.MASK_WORDS + 4 * L.NFIELDS + 4 * 3 + 4 + 8 + 4 * 6
+    hpad = (rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD) * 12
+    return inputs + per_px * n + hpad + 4 * L.NPATCH + 16 + st.out.numel() * st.out.element_size()
 
-    def __init__(self, solver: HipSolver, st: I.SiteState):
+
+class Slot:
+    """An execution slot: a stream and a scene copy with its own state, walk sources and outputs, and a
+    pinned host buffer its results are delivered to."""
+
+    def __init__(self, solver: HipSolver, st: I.SiteState, site: str = "site"):
+        self.site = site
         self.stream = torch.cuda.Stream(solver.device)
         with torch.cuda.stream(self.stream):
-            self.scene = DeviceScene(st.tile, solver.device, {k: (None if v is None else v.clone())
+            # input rasters are filled by each request's snapshot of the site
+            self.scene = DeviceScene(st.tile, solver.device, {k: (None if v is None else torch.empty_like(v))
                                                               for k, v in st.scene.inputs.items()})
             self.out = torch.empty_like(st.out)
+        self.host_out = torch.empty(st.out.shape, dtype=st.out.dtype, pin_memory=True)
+        self.bytes = self.footprint()
         self.busy = None                  # the request in flight
         self.event = torch.cuda.Event()
         self.pending_plan = None
@@ -127,27 +166,84 @@ class Slot:
         s = self.scene
         return sum(t.numel() * t.element_size() for t in (s.hpack, s.vegdsm2, s.buildings, s.cls, s.aspect,
                                                           s.masks, s.fields, s.svfbuveg, s.tan_asvf, s.tan_sa,
-                                                          s.walkface, s.src, s.state, s.hpad, self.out)) + sum(
+                                                          s.walkface, s.src, s.state, s.hpad, s.counts, s.stats,
+                                                          self.out)) + sum(
             v.numel() * v.element_size() for v in s.inputs.values() if v is not None)
 
 
 class Server:
-    """One site, a FIFO of arrived requests, ``slots`` execution slots (admitted while HBM allows)."""
+    """Resident sites, a FIFO of arrived requests, execution slots admitted by HBM footprint.
 
-    def __init__(self, solver: HipSolver, st: I.SiteState, met_pool: list, sun, latitude: float,
-                 pipeline: PlanPipeline, slots: int = 4, hbm_reserve: float = 0.1, chunk_steps: int = 4):
-        self.solver, self.st, self.met_pool, self.sun, self.lat = solver, st, met_pool, sun, latitude
+    ``st``: one :class:`~solweig_project_amd.incremental.SiteState` or a dict name -> site (sites of
+    one city, of any sizes, sharing ``sun`` and ``latitude``); a request names its site in
+    ``payload["site"]`` (default: the first). ``hbm_budget``: bytes the server may hold beyond the
+    resident sites (default: the free HBM at start-up less ``hbm_reserve`` of the device's total).
+    A slot belongs to one site and holds ``slot_footprint(site)`` bytes. A met-only or full request
+    takes an idle slot of its site, else a new one if the pool fits the budget beside the edit
+    reserve (one day of the largest site's outputs, an edit's largest result window), releasing
+    idle slots of other sites first when that makes room; otherwise it waits (arrival order: no
+    request overtakes one that waits)."""
+
+    def __init__(self, solver: HipSolver, st, met_pool: list, sun, latitude: float,
+                 pipeline: PlanPipeline, slots: int = 4, hbm_reserve: float = 0.1, chunk_steps: int = 4,
+                 hbm_budget: int | None = None):
+        self.sites = dict(st) if isinstance(st, dict) else {"site": st}
+        if not self.sites:
+            raise ValueError("a server needs at least one resident site")
+        self.default_site = next(iter(self.sites))
+        self.st = self.sites[self.default_site]
+        self.solver, self.met_pool, self.sun, self.lat = solver, met_pool, sun, latitude
         self.pipeline = pipeline
         self.chunk_steps = max(1, int(chunk_steps))
+        self.max_slots = int(slots)
         self.slots = []
         free, total = torch.cuda.mem_get_info(solver.device)
-        for _ in range(slots):
-            s = Slot(solver, st)
-            if torch.cuda.mem_get_info(solver.device)[0] < hbm_reserve * total:
-                del s                   # admission by HBM footprint: stop adding slots
-                break
-            self.slots.append(s)
+        self.budget = int(hbm_budget) if hbm_budget is not None else int(free - hbm_reserve * total)
+        self.slot_bytes = {k: slot_footprint(v) for k, v in self.sites.items()}
+        self.edit_reserve = max(v.out.numel() * v.out.element_size() for v in self.sites.values())
+        big = max(self.slot_bytes.values())
+        if big + self.edit_reserve > self.budget:
+            raise ValueError(f"the HBM budget of {self.budget} B cannot hold one execution slot of the largest "
+                             f"site ({big} B) beside the edit reserve ({self.edit_reserve} B)")
+        self.pool_bytes = 0              # HBM held by the slot pool
+        self.peak_pool_bytes = 0
+        self.deferred = 0                # requests that waited for a slot (none free, no room for another)
+        self.released = 0                # idle slots released to make room for another site's slot
+        self.peak_busy = 0               # most slots busy at once
+        self.edit_host = torch.empty(self.edit_reserve // 4, dtype=torch.float32, pin_memory=True)
         self.master = torch.cuda.Stream(solver.device)
+
+    def site_of(self, req: Request) -> str:
+        name = req.payload.get("site", self.default_site)
+        if name not in self.sites:
+            raise KeyError(f"request names site {name!r}; resident sites: {list(self.sites)}")
+        return name
+
+    def _free_slot(self, site: str):
+        """An idle slot of ``site``, or a new one if the pool fits the HBM budget (after releasing idle
+        slots of other sites if that makes room); None: the request waits."""
+        idle = next((s for s in self.slots if s.busy is None and s.site == site), None)
+        if idle is not None:
+            return idle
+        need = self.slot_bytes[site]
+        room = lambda: self.budget - self.edit_reserve - self.pool_bytes   # noqa: E731
+        others = [s for s in self.slots if s.busy is None and s.site != site]
+        if need > room() + sum(s.bytes for s in others) or \
+                len(self.slots) - len(others) >= self.max_slots:
+            return None
+        if need > room() or len(self.slots) >= self.max_slots:
+            while others and (need > room() or len(self.slots) >= self.max_slots):
+                gone = others.pop()                 # idle: its stream's work and copies have completed
+                self.slots.remove(gone)
+                self.pool_bytes -= gone.bytes
+                self.released += 1
+                del gone
+            torch.cuda.empty_cache()                # hand the released blocks back for the new slot's stream
+        s = Slot(self.solver, self.sites[site], site)
+        self.pool_bytes += s.bytes
+        self.peak_pool_bytes = max(self.peak_pool_bytes, self.pool_bytes)
+        self.slots.append(s)
+        return s
 
     # -- request execution -----------------------------------------------------
     def _snapshot(self, slot: Slot):
@@ -156,7 +252,7 @@ class Server:
         ev = torch.cuda.Event()
         ev.record(self.master)
         slot.stream.wait_event(ev)
-        src, dst = self.st.scene, slot.scene
+        src, dst = self.sites[slot.site].scene, slot.scene
         with torch.cuda.stream(slot.stream):
             for k, v in src.inputs.items():
                 if v is not None:
@@ -170,18 +266,18 @@ class Server:
         dst.tile = src.tile
 
     def _start(self, slot: Slot, req: Request, now: float):
-        req.start = now
-        req.payload["version"] = self.version
+        req.start = time.perf_counter() - self.t0
+        req.payload["version"] = self.version[slot.site]
+        st = self.sites[slot.site]
         slot.busy = req
+        self._snapshot(slot)
         if req.kind == "met":
-            self._snapshot(slot)
             job = Job(self.met_pool[req.payload["met"]], self.sun, self.lat)
-            slot.pending_plan = self.pipeline.submit(self.st.tile, job)
         else:                               # full re-solve of the current inputs
-            self._snapshot(slot)
             with torch.cuda.stream(slot.stream):
                 self.solver.svf(slot.scene, slot.stream)
-            slot.pending_plan = self.pipeline.submit(self.st.tile, Job(self.st_met(), self.sun, self.lat))
+            job = Job(self.st_met(), self.sun, self.lat)
+        slot.pending_plan = self.pipeline.submit(st.tile, job)
 
     def st_met(self):
         return self.met_pool[0]
@@ -209,6 +305,7 @@ class Server:
         if slot.plan is not None and slot.next_step < slot.plan.n_steps:
             self._feed(slot)
             return False
+        # the last chunk and every chunk's copy to the host have run: the results are on the host
         req.finish = time.perf_counter() - self.t0
         if self.checksums:
             req.result = checksum(slot.out)
@@ -217,33 +314,51 @@ class Server:
         return True
 
     def _feed(self, slot: Slot):
-        """Enqueue the slot's next chunk of day steps and mark its end."""
+        """Enqueue the slot's next chunk of day steps, then its copy to the slot's pinned host buffer
+        (DMA engines, stream-ordered), and mark its end."""
         t0 = slot.next_step
         t1 = min(slot.plan.n_steps, t0 + self.chunk_steps)
         with torch.cuda.stream(slot.stream):
-            self.solver.run_day(slot.scene, slot.plan, slot.dplan, self.st.outputs, out=slot.out,
+            self.solver.run_day(slot.scene, slot.plan, slot.dplan, self.sites[slot.site].outputs, out=slot.out,
                                 stream=slot.stream, steps=(t0, t1))
+            if self.to_host:
+                L.call("sw_copy_to_host", slot.host_out[t0:t1].data_ptr(), slot.out[t0:t1].data_ptr(),
+                       slot.out[t0:t1].numel() * slot.out.element_size(), slot.stream.cuda_stream)
             slot.event.record(slot.stream)
         slot.next_step = t1
 
     def _edit(self, req: Request):
         p = req.payload
-        box, cd = tree_window(self.st.tile, p["center"], p["radius"], p["height"], p["remove"])
+        site = self.site_of(req)
+        st = self.sites[site]
+        box, cd = tree_window(st.tile, p["center"], p["radius"], p["height"], p["remove"])
         with torch.cuda.stream(self.master):
-            info = I.apply_edit_window(self.solver, self.st, box, cdsm=cd, stream=self.master)
-            self.version += 1
-            req.payload["version"] = self.version
+            info = I.apply_edit_window(self.solver, st, box, cdsm=cd, stream=self.master)
+            self.version[site] += 1
+            req.payload["version"] = self.version[site]
             req.payload["mode"] = info.get("mode")
+            ob = info.get("out_box")
+            if self.to_host and ob is not None:
+                # the edit's result: the outputs of every recomputed tile (bounding box, all steps)
+                r0, r1, c0, c1 = ob
+                win = st.out[:, :, r0:r1, c0:c1]
+                self.edit_host[:win.numel()].view(win.shape).copy_(win, non_blocking=True)
+                self.master.synchronize()
+                req.payload["result_bytes"] = win.numel() * win.element_size()
             if self.checksums:
-                req.result = checksum(self.st.out)
+                req.result = checksum(st.out)
         return info
 
     # -- the loop --------------------------------------------------------------
-    def run(self, requests: list, checksums: bool = True) -> list:
-        """Serve ``requests`` (sorted by arrival) against the wall clock; returns them with timings."""
+    def run(self, requests: list, checksums: bool = True, to_host: bool = True) -> list:
+        """Serve ``requests`` (sorted by arrival) against the wall clock; returns them with timings
+        (arrival to results on the host; ``to_host`` False ends them in HBM)."""
         self.t0 = time.perf_counter()
         self.checksums = checksums
-        self.version = 0
+        self.to_host = to_host
+        self.version = {k: 0 for k in self.sites}
+        for r in requests:
+            self.site_of(r)                 # unknown sites are refused before anything runs
         torch.cuda.synchronize()
         todo = deque(sorted(requests, key=lambda r: r.arrival))
         queue = deque()
@@ -269,13 +384,17 @@ class Server:
                 req = s.busy
                 if req is not None and self._advance(s, now):
                     done.append(req)
-            # slot work is admitted as slots free up, in arrival order
+            # slot work is admitted in arrival order as slots free up or fit the HBM budget
             while queue:
-                free = next((s for s in self.slots if s.busy is None), None)
+                free = self._free_slot(self.site_of(queue[0]))
                 if free is None:
+                    if not queue[0].deferred:
+                        queue[0].deferred = True
+                        self.deferred += 1
                     break
                 self._start(free, queue.popleft(), now)
                 progressed = True
+            self.peak_busy = max(self.peak_busy, sum(1 for s in self.slots if s.busy is not None))
             if not progressed and not any(s.busy for s in self.slots) and todo and not queue:
                 time.sleep(max(0.0, min(0.002, todo[0].arrival - now)))
             elif not progressed:
@@ -284,10 +403,16 @@ class Server:
         return [r for r in requests]
 
 
+def nearest_rank(values, q: float) -> float:
+    """The nearest-rank q-th percentile: the smallest value with at least q % of the values at or below it."""
+    v = np.sort(np.asarray(values, np.float64))
+    k = max(1, int(np.ceil(q / 100.0 * v.size)))
+    return float(v[k - 1])
+
+
 def percentiles(requests, kinds=None):
     lat = np.array([r.latency_ms for r in requests if kinds is None or r.kind in kinds])
     if lat.size == 0:
         return None
-    return {"n": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 2),
-            "p99_ms": round(float(np.percentile(lat, 99)), 2), "max_ms": round(float(lat.max()), 2),
-            "mean_ms": round(float(lat.mean()), 2)}
+    return {"n": int(lat.size), "p50_ms": round(nearest_rank(lat, 50), 2), "p99_ms": round(nearest_rank(lat, 99), 2),
+            "max_ms": round(float(lat.max()), 2), "mean_ms": round(float(lat.mean()), 2)}
