@@ -33,6 +33,7 @@ import os
 import struct
 import zipfile
 import zlib
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -50,32 +51,53 @@ _TYPES = {1: ("B", 1), 2: ("s", 1), 3: ("H", 2), 4: ("I", 4), 5: ("II", 8), 6: (
 # ---------------------------------------------------------------------------
 # GeoTIFF codec
 # ---------------------------------------------------------------------------
-def write_geotiff(path_or_file, array, geotransform, epsg: int | None = None, compress: bool = False):
-    """Write a 2-D array as a single-band Float32 GeoTIFF (GDAL geotransform convention)."""
+def _metadata_xml(band_metadata) -> str:
+    """GDAL's per-band metadata document (TIFF tag 42112): one Item per key, ``sample`` = band index."""
+    from xml.sax.saxutils import escape, quoteattr
+    items = [f"  <Item name={quoteattr(str(k))} sample=\"{b}\">{escape(str(v))}</Item>"
+             for b, md in enumerate(band_metadata) for k, v in (md or {}).items()]
+    return "<GDALMetadata>\n" + "\n".join(items) + "\n</GDALMetadata>\n" if items else ""
+
+
+def write_geotiff(path_or_file, array, geotransform, epsg: int | None = None, compress: bool = False,
+                  band_metadata=None):
+    """Write a 2-D array (one band) or a 3-D array (bands, rows, cols) as a Float32 GeoTIFF (GDAL
+    geotransform convention; bands stored one after another, PlanarConfiguration 2).
+    ``band_metadata``: one dict per band, written as GDAL band metadata (e.g. ``{"Time": ...}``)."""
     a = np.ascontiguousarray(np.asarray(array, F32))
-    if a.ndim != 2:
-        raise ValueError("a single 2-D band")
-    rows, cols = a.shape
+    if a.ndim == 2:
+        a = a[None]
+    if a.ndim != 3 or a.shape[0] < 1:
+        raise ValueError("a 2-D band or a (bands, rows, cols) stack")
+    nb, rows, cols = a.shape
     gt = [float(v) for v in geotransform]
     if len(gt) != 6:
         raise ValueError("geotransform needs 6 values")
+    if band_metadata is not None and len(band_metadata) != nb:
+        raise ValueError(f"band_metadata has {len(band_metadata)} entries for {nb} bands")
     rps = max(1, min(rows, 65536 // max(1, cols * 4)))
     strips = []
-    for r0 in range(0, rows, rps):
-        raw = a[r0:r0 + rps].astype("<f4").tobytes()
-        strips.append(zlib.compress(raw, 6) if compress else raw)
+    for b in range(nb):
+        for r0 in range(0, rows, rps):
+            raw = a[b, r0:r0 + rps].astype("<f4").tobytes()
+            strips.append(zlib.compress(raw, 6) if compress else raw)
     tags = []                          # (tag, type, values)
     tags.append((256, 4, [cols]))
     tags.append((257, 4, [rows]))
-    tags.append((258, 3, [32]))
+    tags.append((258, 3, [32] * nb))
     tags.append((259, 3, [8 if compress else 1]))
     tags.append((262, 3, [1]))
     tags.append((273, 4, [0] * len(strips)))           # patched below
-    tags.append((277, 3, [1]))
+    tags.append((277, 3, [nb]))
     tags.append((278, 4, [rps]))
     tags.append((279, 4, [len(s) for s in strips]))
-    tags.append((284, 3, [1]))
-    tags.append((339, 3, [3]))
+    tags.append((284, 3, [2 if nb > 1 else 1]))
+    if nb > 1:
+        tags.append((338, 3, [0] * (nb - 1)))           # extra samples: unspecified data
+    tags.append((339, 3, [3] * nb))
+    xml = _metadata_xml(band_metadata or [])
+    if xml:
+        tags.append((42112, 2, xml))
     if gt[2] == 0.0 and gt[4] == 0.0:
         tags.append((33550, 12, [gt[1], -gt[5], 0.0]))
         tags.append((33922, 12, [0.0, 0.0, 0.0, gt[0], gt[3], 0.0]))
@@ -100,7 +122,11 @@ def write_geotiff(path_or_file, array, geotransform, epsg: int | None = None, co
     entries = []
     for tag, typ, vals in tags:
         fmt, size = _TYPES[typ]
-        payload = struct.pack("<" + fmt[0] * len(vals), *vals)
+        if typ == 2:                    # ASCII, NUL-terminated
+            payload = vals.encode("latin-1") + b"\0"
+            vals = payload
+        else:
+            payload = struct.pack("<" + fmt[0] * len(vals), *vals)
         if len(payload) <= 4:
             entries.append([tag, typ, len(vals), payload.ljust(4, b"\0"), None])
         else:
@@ -175,6 +201,61 @@ def _parse_ifd(buf: bytes):
 
 def read_geotiff(path_or_bytes):
     """(float32 array, geotransform) of a single-band GeoTIFF."""
+    r = read_geotiff_full(path_or_bytes)
+    if r.data.shape[0] != 1:
+        raise ValueError("only single-band rasters")
+    return r.data[0], r.geotransform
+
+
+@dataclass
+class GeoRaster:
+    """A GeoTIFF's bands as float32 (bands, rows, cols), its geotransform, EPSG code (from the GeoKey
+    directory, None when absent) and per-band GDAL metadata."""
+
+    data: np.ndarray
+    geotransform: tuple | None
+    epsg: int | None
+    band_metadata: list
+
+    @property
+    def shape(self):
+        return self.data.shape[1:]
+
+
+def _geokeys_epsg(keys):
+    """ProjectedCSTypeGeoKey, else GeographicTypeGeoKey, of a GeoKey directory (tag 34735)."""
+    if not keys or len(keys) < 4:
+        return None
+    found = {}
+    for k in range(keys[3]):
+        kid, loc, cnt, val = keys[4 + 4 * k:8 + 4 * k]
+        if loc == 0 and cnt == 1:
+            found[kid] = val
+    for kid in (3072, 2048):
+        v = found.get(kid)
+        if v and v != 32767:            # 32767: user-defined
+            return int(v)
+    return None
+
+
+def _band_metadata(xml: str | None, nb: int) -> list:
+    out = [dict() for _ in range(nb)]
+    if not xml:
+        return out
+    import xml.etree.ElementTree as ET
+    try:
+        root = ET.fromstring(xml)
+    except ET.ParseError:
+        return out
+    for item in root.iter("Item"):
+        b = item.get("sample")
+        if b is not None and b.isdigit() and int(b) < nb and item.get("name") is not None:
+            out[int(b)][item.get("name")] = item.text or ""
+    return out
+
+
+def read_geotiff_full(path_or_bytes) -> GeoRaster:
+    """All bands of a GeoTIFF (strips or tiles, pixel- or band-interleaved) with its georeferencing."""
     if isinstance(path_or_bytes, (bytes, bytearray)):
         buf = bytes(path_or_bytes)
     else:
@@ -183,8 +264,7 @@ def read_geotiff(path_or_bytes):
     e, t = _parse_ifd(buf)
     cols, rows = t[256][0], t[257][0]
     spp = t.get(277, [1])[0]
-    if spp != 1:
-        raise ValueError("only single-band rasters")
+    planar = t.get(284, [1])[0]
     bits = t.get(258, [1])[0]
     fmt = t.get(339, [1])[0]
     comp = t.get(259, [1])[0]
@@ -201,24 +281,32 @@ def read_geotiff(path_or_bytes):
     def chunk(off, cnt):
         raw = buf[off:off + cnt]
         return zlib.decompress(raw) if comp in (8, 32946) else raw
-    out = np.zeros((rows, cols), dt)
+    # one plane per band when band-interleaved; a single plane of spp samples per pixel otherwise
+    sep = planar == 2 or spp == 1
+    per = 1 if sep else spp
+    nplanes = spp if sep else 1
+    out = np.zeros((nplanes, rows, cols, per), dt)
     if 322 in t:                       # tiles
         tw, th = t[322][0], t[323][0]
         offs, cnts = t[324], t[325]
         across = -(-cols // tw)
+        per_plane = across * -(-rows // th)
         for k, (o, c) in enumerate(zip(offs, cnts)):
-            ty, tx = divmod(k, across)
-            tile = np.frombuffer(chunk(o, c), dt, count=tw * th).reshape(th, tw)
+            p, kk = divmod(k, per_plane)
+            ty, tx = divmod(kk, across)
+            tile = np.frombuffer(chunk(o, c), dt, count=tw * th * per).reshape(th, tw, per)
             r0, c0 = ty * th, tx * tw
             h, w = min(th, rows - r0), min(tw, cols - c0)
-            out[r0:r0 + h, c0:c0 + w] = tile[:h, :w]
+            out[p, r0:r0 + h, c0:c0 + w] = tile[:h, :w]
     else:
         rps = t.get(278, [rows])[0]
-        r0 = 0
-        for o, c in zip(t[273], t[279]):
+        per_plane = -(-rows // rps)
+        for k, (o, c) in enumerate(zip(t[273], t[279])):
+            p, kk = divmod(k, per_plane)
+            r0 = kk * rps
             h = min(rps, rows - r0)
-            out[r0:r0 + h] = np.frombuffer(chunk(o, c), dt, count=h * cols).reshape(h, cols)
-            r0 += h
+            out[p, r0:r0 + h] = np.frombuffer(chunk(o, c), dt, count=h * cols * per).reshape(h, cols, per)
+    out = out[:, :, :, 0] if sep else np.moveaxis(out[0], -1, 0)
     gt = None
     if 34264 in t:
         m = t[34264]
@@ -227,7 +315,8 @@ def read_geotiff(path_or_bytes):
         sx, sy = t[33550][0], t[33550][1]
         i, j, _, x, y, _ = t[33922][:6]
         gt = (x - i * sx, sx, 0.0, y + j * sy, 0.0, -sy)
-    return out.astype(F32), gt
+    return GeoRaster(np.ascontiguousarray(out.astype(F32)), gt, _geokeys_epsg(t.get(34735)),
+                     _band_metadata(t.get(42112), out.shape[0]))
 
 
 def geotransforms_equal(a, b, tol: float = 1e-6) -> bool:
