@@ -724,6 +724,23 @@ def main():
         raise RuntimeError("a host-to-host solve delivered outputs different from the device-resident solve")
     log(f"host-to-host: {ms:.3f} ms per solve ({value:.1f} Mpixel-steps/s), median gap between solve "
         f"completions {gap_ms:.3f} ms; outputs of all {args.steps} solves checked on the host")
+    # this box's device-to-host rate on the DMA engines (the copy path of the timed region), alone
+    from solweig_project_amd import _lib as LP
+    cs = torch.cuda.Stream(dev)
+    src, dst = dev_outs[0], hbufs[0].out
+    nb = src.numel() * src.element_size()
+    pe = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    LP.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), nb, cs.cuda_stream)
+    pe[0].record(cs)
+    for _ in range(3):
+        LP.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), nb, cs.cuda_stream)
+    pe[1].record(cs)
+    cs.synchronize()
+    d2h_gbs = 3 * nb / (pe[0].elapsed_time(pe[1]) / 1e3) / 1e9
+    out_bytes = (16 + 3 * T) * rows * cols * 4
+    bound_ms = max(dev_ms, out_bytes / (d2h_gbs * 1e9) * 1e3)
+    log(f"device-to-host {d2h_gbs:.1f} GB/s alone: {out_bytes / 1e6:.0f} MB of outputs per solve take "
+        f"{out_bytes / d2h_gbs / 1e6:.2f} ms; host-to-host bound max(device, copy) = {bound_ms:.2f} ms")
     del hbufs
 
     # rooflines: kernels timed alone (one lane, HIP events on the launch stream, 3 solves), each against
@@ -866,6 +883,8 @@ def main():
         "value_device_resident": round(dev_value, 3),
         "ms_per_step_device_resident": round(dev_ms, 3),
         "median_solve_gap_ms": round(gap_ms, 3),
+        "pcie_d2h_gbs": round(d2h_gbs, 1),
+        "h2h_bound_ms": round(bound_ms, 3),
         "kernel_ms": {k: round(t / nn, 4) for k, (nn, t) in ksum_alone.items()},
         "kernel_ms_lanes": {k: round(t / nn, 4) for k, (nn, t) in ksum.items()},
         "host_plan_ms": round(plan_ms, 3),
