@@ -84,10 +84,11 @@ def parse():
     ap.add_argument("--train-sites", type=int, default=40, help="1024x1024 training cities for the surrogate")
     ap.add_argument("--surrogate-weights",
                     default=str(Path(__file__).resolve().parent / "solweig_project_amd" / "data"
-                                / "svf_surrogate_16_32_64_128.npz"),
-                    help="trained surrogate weights (.npz; default: the shipped network); '' trains a new one")
+                                / "svf_surrogate_32_64_128_256.npz"),
+                    help="trained surrogate weights (.npz; default: the shipped network of the surrogate "
+                         "configuration, 1.95 M parameters; its channels are read from the file); '' trains a new one")
     ap.add_argument("--save-weights", default="", help="write the trained surrogate weights here")
-    ap.add_argument("--channels", default="16,32,64,128", help="surrogate level channels")
+    ap.add_argument("--channels", default="32,64,128,256", help="surrogate level channels of a network trained here")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run k_pack_heights over 16 Mi pixels (12 B read + 16 B written per pixel) "
                          "as the known-bytes reference for rocprof FETCH_SIZE / WRITE_SIZE")
@@ -371,12 +372,13 @@ def run_surrogate(args):
     dev = torch.device("cuda:0")
     solver = HipSolver(dev)
     ch = tuple(int(c) for c in args.channels.split(","))
-    net = SG.SurrogateNet(ch, dev, seed=args.seed)
     train_info = None
     if args.surrogate_weights and Path(args.surrogate_weights).exists():
-        net.load_state_dict(dict(np.load(args.surrogate_weights)))
-        log(f"loaded surrogate weights {args.surrogate_weights}")
+        net = SG.load_weights(args.surrogate_weights, dev)
+        ch = net.channels
+        log(f"loaded surrogate weights {args.surrogate_weights} (channels {ch}, {net.n_params} parameters)")
     else:
+        net = SG.SurrogateNet(ch, dev, seed=args.seed)
         log(f"labelling {args.train_sites} training cities (1024x1024) with the exact SVF sweep")
         sites = SG.exact_sites(solver, args.train_sites, 1024, seed=args.seed + 500)
         # time a few steps to size the run to the budget
@@ -406,9 +408,9 @@ def run_surrogate(args):
     exact = ds.fields[:SG.N_FIELDS].clone()
     exact_all = ds.fields.clone()
 
-    def gated(model, reps=5):
+    def gated(model, reps=5, criterion="mean", samples=4):
         """Median wall time of ``reps`` gated passes after a warm one (each starts from the exact fields)."""
-        gate = SG.SvfSurrogate(model, tol=0.02, samples=4, seed=args.seed)
+        gate = SG.SvfSurrogate(model, tol=0.02, samples=samples, seed=args.seed, criterion=criterion)
         gate.svf(solver, ds)                     # warm
         walls = []
         for _ in range(reps):
@@ -435,6 +437,7 @@ def run_surrogate(args):
         torch.cuda.synchronize()
         return out[:, 0]
     tmrt_exact = day_tmrt()
+    tmrt_exact_keep = tmrt_exact.clone()
     rep, wall, err = gated(net)
     tmrt_gated = day_tmrt()
     d3 = (tmrt_gated - tmrt_exact).abs()
@@ -460,6 +463,27 @@ def run_surrogate(args):
     untrained = SG.SurrogateNet(ch, dev, seed=args.seed + 7)
     rep_u, wall_u, err_u = gated(untrained)
     exact_ms = float(np.median(ex_ms))
+    # the bar: max |dSVF| <= 0.02 over every pixel and field, max |dTmrt| <= 0.5 K over every
+    # pixel-step of the day derived from the gated fields; gates of increasing strictness
+    gates = []
+    for crit, samp in (("mean", 4), ("max", 4), ("max", 16), ("max", 64)):
+        g_rep, g_wall, g_err = gated(net, criterion=crit, samples=samp)
+        g_tmrt = day_tmrt()
+        g_dt = (g_tmrt - tmrt_exact_keep).abs()
+        gmax, tmax = float(g_err.max()), float(g_dt.max())
+        gates.append({"criterion": crit, "samples_per_core": samp, "accepted": g_rep.accepted,
+                      "windows": g_rep.windows, "gated_ms": round(g_wall, 3), "speedup": round(exact_ms / g_wall, 3),
+                      "svf_error_max": round(gmax, 4), "svf_error_mean": round(float(g_err.mean()), 5),
+                      "tmrt_error_max_K": round(tmax, 4), "tmrt_frac_over_0.5K": round(float((g_dt > 0.5).float().mean()), 6),
+                      "meets_bar": bool(gmax <= 0.02 and tmax <= 0.5)})
+        log(f"gate {crit}/{samp}: {g_rep.accepted}/{g_rep.windows} accepted, {g_wall:.2f} ms "
+            f"({exact_ms / g_wall:.2f}x), max dSVF {gmax:.4f}, max dTmrt {tmax:.3f} K")
+        del g_tmrt, g_dt, g_err
+        solver.svf(ds)
+        torch.cuda.synchronize()
+    ok = [g for g in gates if g["meets_bar"]]
+    at_bar = max(ok, key=lambda g: g["speedup"]) if ok else None
+    del tmrt_exact_keep
     # MFMA roofline of the network: 2 x MACs per 384^2 window as inference runs it (model channels; the
     # last two layers over the core) x windows / network time
     fpw = 2 * SG.macs_per_window(ch, core=SG.CORE)
@@ -489,6 +513,8 @@ def run_surrogate(args):
         "gated_error": {"max": round(float(err.max()), 4), "mean": round(float(err.mean()), 5), "p99": round(q, 4)},
         "network_error_ungated": {"max": round(float(raw_err.max()), 4), "mean": round(float(raw_err.mean()), 5)},
         "derived_tmrt_error": derived,
+        "gates": gates,
+        "at_bar": at_bar,
         "untrained_same_gate": {"accepted": rep_u.accepted, "gated_ms": round(wall_u, 3),
                                 "speedup": round(exact_ms / wall_u, 3), "max_error": round(float(err_u.max()), 4)},
         "training": None if train_info is None else {
@@ -915,14 +941,26 @@ def run_extras(args, extras, serve_pipeline):
                             "latency": "arrival to results in pinned host memory; nearest-rank percentiles",
                             "site": "2000x2000x24h"}
         elif name == "surrogate":
+            import copy
+            fast = copy.copy(args)
+            fast.surrogate_weights = str(ROOT / "solweig_project_amd" / "data" / "svf_surrogate_16_32_64_128.npz")
+            rf = run_surrogate(fast)
+            log(f"surrogate (fast network): {json.dumps(rf)}")
             r = run_surrogate(args)
             bar = {"svf_max_abs": 0.02, "tmrt_max_abs_K": 0.5}
             ge, te = r["gated_error"]["max"], r["derived_tmrt_error"]["max_K"]
             out["surrogate"] = {"speedup": r["value"], "exact_svf_ms": r["exact_svf_ms"], "gated_ms": r["gated_ms"],
                                 "gated_svf_error_max": ge, "derived_tmrt_error_max_K": te, "bar": bar,
                                 "meets_bar": bool(ge <= bar["svf_max_abs"] and te <= bar["tmrt_max_abs_K"]),
-                                "accepted_windows": f"{r['accepted']}/{r['windows']}", "gate": r.get("gate"),
-                                "params": r["config"]["params"], "site": "2000x2000"}
+                                "accepted_windows": f"{r['accepted']}/{r['windows']}", "gate": "mean, 4 tiles per core",
+                                "at_bar": r["at_bar"],
+                                "gates": [{k: g[k] for k in ("criterion", "samples_per_core", "speedup", "svf_error_max",
+                                                             "tmrt_error_max_K", "meets_bar")} for g in r["gates"]],
+                                "params": r["config"]["params"], "site": "2000x2000",
+                                "fast_network": {"params": rf["config"]["params"], "speedup": rf["value"],
+                                                 "gated_svf_error_max": rf["gated_error"]["max"],
+                                                 "derived_tmrt_error_max_K": rf["derived_tmrt_error"]["max_K"],
+                                                 "at_bar": rf["at_bar"]}}
         elif name == "batch":
             r = run_batch(args, reps=3)
             out["batch"] = {"value": r["value"], "unit": r["unit"], "ms": r["ms_per_step"],

@@ -491,6 +491,11 @@ int sw_gate_error(const float *out, const float *fields, const int32_t *org, con
                   const int32_t *fm, float *werr, int nwin, int hw, int cs, int nl, int rows, int cols, void *stream);
 int sw_gate_write(const float *out, const int32_t *org, const int32_t *fm, const uint8_t *sampled, float *fields,
                   int nwin, int hw, int cs, int core, int c0, int nl, int rows, int cols, void *stream);
+/* sw_gate_max_error: as sw_gate_error, but werr[w] is the largest absolute difference of any of the nl
+ * fields at any sampled pixel of window w (no counterpart in the reference). */
+int sw_gate_max_error(const float *out, const float *fields, const int32_t *org, const int32_t *pix,
+                      const int32_t *seg, const int32_t *fm, float *werr, int nwin, int hw, int cs, int nl, int rows,
+                      int cols, void *stream);
 
 #ifdef __cplusplus
 }

@@ -201,6 +201,7 @@ SIGNATURES = {
     "sw_cnn_gather_labels": (_I, [_VP] * 4 + [_I] * 6 + [_VP]),
     "sw_cnn_scatter_cores": (_I, [_VP] * 4 + [_I] * 8 + [_VP]),
     "sw_gate_error": (_I, [_VP] * 7 + [_I] * 6 + [_VP]),
+    "sw_gate_max_error": (_I, [_VP] * 7 + [_I] * 6 + [_VP]),
     "sw_gate_write": (_I, [_VP] * 5 + [_I] * 8 + [_VP]),
     "sw_find_walls": (_I, [_VP, _VP, _I, _I, ctypes.c_double, _VP]),
     "sw_wall_filter_size": (_I, [ctypes.c_double]),

@@ -735,6 +735,10 @@ __global__ void __launch_bounds__(256) k_scatter_cores(const float *__restrict__
 // gate: mean absolute difference between the network output and the exact fields over the sampled
 // pixels of each window (pixels pix[seg[w] .. seg[w + 1]) of window w, per pixel the mean over the nl
 // fields), one block per window with a fixed-order reduction (deterministic)
+// per window over its sampled pixels: MAX = false, the mean over pixels of the mean absolute
+// difference over the nl fields; MAX = true, the largest absolute difference of any field at any
+// sampled pixel (fixed reduction order either way)
+template <bool MAX>
 __global__ void __launch_bounds__(256) k_gate_error(const float *__restrict__ out, const float *__restrict__ fields,
                                                      const int2 *__restrict__ org, const int32_t *__restrict__ pix,
                                                      const int32_t *__restrict__ seg, const int32_t *__restrict__ fm,
@@ -750,16 +754,20 @@ __global__ void __launch_bounds__(256) k_gate_error(const float *__restrict__ ou
         const int r = p / cols, c = p - r * cols;
         const long o = (((long)w * hw + (r - org[w].x)) * hw + (c - org[w].y)) * cs;
         float e = 0.f;
-        for (int k = 0; k < nl; ++k) e += fabsf(out[o + k] - fields[(long)fm[k] * n + p]);
-        acc += e / (float)nl;
+        for (int k = 0; k < nl; ++k) {
+            const float d = fabsf(out[o + k] - fields[(long)fm[k] * n + p]);
+            e = MAX ? fmaxf(e, d) : e + d;
+        }
+        acc = MAX ? fmaxf(acc, e) : acc + e / (float)nl;
     }
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if ((int)threadIdx.x < st)
+            red[threadIdx.x] = MAX ? fmaxf(red[threadIdx.x], red[threadIdx.x + st]) : red[threadIdx.x] + red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) werr[w] = s1 > s0 ? red[0] / (float)(s1 - s0) : 0.f;
+    if (threadIdx.x == 0) werr[w] = s1 > s0 ? (MAX ? red[0] : red[0] / (float)(s1 - s0)) : 0.f;
 }
 
 // gate: network output cores into the field planes except at sampled pixels (which keep the exact
@@ -1047,9 +1055,22 @@ extern "C" int sw_gate_error(const float *out, const float *fields, const int32_
     if (nwin < 0 || hw <= 0 || cs <= 0 || nl <= 0 || nl > cs || rows <= 0 || cols <= 0)
         return set_err(SW_E_ARG, "sw_gate_error: bad sizes");
     if (nwin == 0) return SW_OK;
-    hipLaunchKernelGGL(k_gate_error, dim3(nwin), dim3(256), 0, S(stream), out, fields, (const int2 *)org, pix, seg, fm,
-                       werr, hw, cs, nl, rows, cols);
+    hipLaunchKernelGGL(k_gate_error<false>, dim3(nwin), dim3(256), 0, S(stream), out, fields, (const int2 *)org, pix,
+                       seg, fm, werr, hw, cs, nl, rows, cols);
     return hip_check(hipGetLastError(), "k_gate_error");
+}
+
+extern "C" int sw_gate_max_error(const float *out, const float *fields, const int32_t *org, const int32_t *pix,
+                                 const int32_t *seg, const int32_t *fm, float *werr, int nwin, int hw, int cs, int nl,
+                                 int rows, int cols, void *stream) {
+    if (!out || !fields || !org || !pix || !seg || !fm || !werr)
+        return set_err(SW_E_ARG, "sw_gate_max_error: NULL pointer");
+    if (nwin < 0 || hw <= 0 || cs <= 0 || nl <= 0 || nl > cs || rows <= 0 || cols <= 0)
+        return set_err(SW_E_ARG, "sw_gate_max_error: bad sizes");
+    if (nwin == 0) return SW_OK;
+    hipLaunchKernelGGL(k_gate_error<true>, dim3(nwin), dim3(256), 0, S(stream), out, fields, (const int2 *)org, pix,
+                       seg, fm, werr, hw, cs, nl, rows, cols);
+    return hip_check(hipGetLastError(), "k_gate_error<max>");
 }
 
 extern "C" int sw_gate_write(const float *out, const int32_t *org, const int32_t *fm, const uint8_t *sampled, float *fields,

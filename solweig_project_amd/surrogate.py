@@ -475,14 +475,21 @@ class SvfSurrogate:
     The network predicts fields 0..9 of every 256 x 256 core; SVFtotal
     (field 15) follows from them as in the fold (svf - (1 - svfveg) * 0.97).
     The gate runs the exact kernel on ``samples`` random 16 x 16 tiles of
-    every core, accepts a core when the mean absolute difference over its
-    sampled tiles and the 10 fields is at most ``tol``, and re-solves the
-    rejected cores exactly. Sampled tiles and rejected cores carry exact
-    values; the aveg fields (10..14) are exact only there.
+    every core and accepts a core when its error over the sampled tiles and
+    the 10 fields is at most ``tol``: with ``criterion="mean"`` the mean
+    absolute difference, with ``criterion="max"`` the largest absolute
+    difference of any field at any sampled pixel. Rejected cores are
+    re-solved exactly. Sampled tiles and rejected cores carry exact values;
+    the aveg fields (10..14) are exact only there. Unsampled pixels of an
+    accepted core are bounded by neither criterion.
     """
 
-    def __init__(self, net: SurrogateNet, tol: float = 0.02, samples: int = 4, seed: int = 0):
+    def __init__(self, net: SurrogateNet, tol: float = 0.02, samples: int = 4, seed: int = 0,
+                 criterion: str = "mean"):
+        if criterion not in ("mean", "max"):
+            raise ValueError(f"gate criterion {criterion!r}: 'mean' or 'max'")
         self.net, self.tol, self.samples, self.seed = net, tol, samples, seed
+        self.criterion = criterion
         self._geo = {}
         self.ps = None
         self._side = None
@@ -579,7 +586,8 @@ class SvfSurrogate:
         main.wait_event(sampled_done)
         fm = _field_map(dev)
         werr = torch.empty(len(org), dtype=torch.float32, device=dev)
-        L.call("sw_gate_error", L.ptr(ps.out), L.ptr(ds.fields), L.ptr(g["d_org"]), L.ptr(g["pix32"]),
+        L.call("sw_gate_max_error" if self.criterion == "max" else "sw_gate_error", L.ptr(ps.out), L.ptr(ds.fields),
+               L.ptr(g["d_org"]), L.ptr(g["pix32"]),
                L.ptr(g["seg"]), L.ptr(fm), L.ptr(werr), len(org), ps.hw, COUT_HEAD, N_FIELDS, rows, cols, sp)
         L.call("sw_gate_write", L.ptr(ps.out), L.ptr(g["d_org"]), L.ptr(fm), L.ptr(g["sampled"]), L.ptr(ds.fields),
                len(org), ps.hw, COUT_HEAD, ps.core, ps.c0, N_FIELDS, rows, cols, sp)
@@ -617,13 +625,28 @@ def _to_host(t, stream):
     return t.cpu().numpy()
 
 
-DEFAULT_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "svf_surrogate_16_32_64_128.npz"
+_DATA = __import__("pathlib").Path(__file__).resolve().parent / "data"
+#: the fast network (0.49 M parameters, channels 16, 32, 64, 128)
+DEFAULT_WEIGHTS = _DATA / "svf_surrogate_16_32_64_128.npz"
+#: the network of the surrogate configuration (1.95 M parameters, channels 32, 64, 128, 256, trained on
+#: 20 000 windows of 40 cities)
+CONFIG_WEIGHTS = _DATA / "svf_surrogate_32_64_128_256.npz"
 
 
-def load_default(device="cuda:0") -> SurrogateNet:
-    """The trained network shipped with the package (channels 16, 32, 64, 128)."""
-    with np.load(DEFAULT_WEIGHTS, allow_pickle=False) as z:
+def load_weights(path, device="cuda:0") -> SurrogateNet:
+    """A trained network from a weights file written by ``state_dict`` (channels and parameters)."""
+    with np.load(path, allow_pickle=False) as z:
         d = {k: z[k] for k in z.files}
     net = SurrogateNet(tuple(int(c) for c in d["channels"]), device)
     net.load_state_dict(d)
     return net
+
+
+def load_default(device="cuda:0") -> SurrogateNet:
+    """The fast trained network shipped with the package (channels 16, 32, 64, 128)."""
+    return load_weights(DEFAULT_WEIGHTS, device)
+
+
+def load_config(device="cuda:0") -> SurrogateNet:
+    """The trained network of the surrogate configuration (channels 32, 64, 128, 256)."""
+    return load_weights(CONFIG_WEIGHTS, device)

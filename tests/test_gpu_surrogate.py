@@ -196,8 +196,10 @@ def test_window_copies(S):
     assert np.array_equal(dst.cpu().numpy(), de)
 
 
-def test_gated_surrogate_on_a_city(S):
-    """The shipped network on an unseen 768 x 640 city: every rejected core is exact, accepted cores
+@pytest.mark.parametrize("which", ["default", "config"])
+def test_gated_surrogate_on_a_city(S, which):
+    """Each shipped network (the fast 0.49 M-parameter one and the 1.95 M-parameter network of the
+    surrogate configuration) on an unseen 768 x 640 city: every rejected core is exact, accepted cores
     hold the gate's mean error, the derived day's Tmrt stays within 0.5 K per core (mean), and an
     untrained network is rejected everywhere (the gate falls back)."""
     from solweig_project_amd import synth
@@ -209,7 +211,8 @@ def test_gated_surrogate_on_a_city(S):
     solver.svf(ds)
     torch.cuda.synchronize()
     exact = ds.fields.clone()
-    net = S.load_default()
+    net = S.load_default() if which == "default" else S.load_config()
+    assert (which, net.n_params) in (("default", 489424), ("config", 1951120))
     gate = S.SvfSurrogate(net, tol=0.02, samples=4)
     rep = gate.svf(solver, ds)
     f = ds.fields.view(16, 768, 640)
@@ -249,3 +252,36 @@ def test_gated_surrogate_on_a_city(S):
     rep_u = S.SvfSurrogate(untrained, tol=0.02, samples=4).svf(solver, ds)
     assert rep_u.accepted == 0
     assert torch.equal(ds.fields[:10], exact[:10])
+
+
+def test_max_gate_error_equals_torch(S):
+    """sw_gate_max_error: per window, the largest |network - exact| of any of the 10 fields at any
+    sampled pixel, equal to the same maximum taken in torch (max is exact, so bit for bit); a core is
+    accepted exactly when that maximum is within the tolerance."""
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import HipSolver, TileInputs
+    solver = HipSolver("cuda:0")
+    site = synth.make_site(512, 384, seed=779, with_aspect=False)
+    ds = solver.upload(TileInputs.from_site(site))
+    solver.svf(ds)
+    torch.cuda.synchronize()
+    gate = S.SvfSurrogate(S.load_default(), tol=0.02, samples=8, criterion="max")
+    rep = gate.svf(solver, ds)
+    g = gate._geometry(512, 384)
+    out = gate.ps.out.float()
+    hw, cs = gate.ps.hw, out.shape[-1]
+    pix = g["pix"].long()
+    seg = g["seg"].cpu().numpy()
+    org = torch.from_numpy(np.ascontiguousarray(g["org"], np.int64)).to(pix.device)
+    fm = torch.tensor(S.FIELD_MAP, device=pix.device).long()
+    n = 512 * 384
+    exact = ds.fields.view(16, n)
+    for w in range(rep.windows):
+        p = pix[seg[w]:seg[w + 1]]
+        r, c = p // 384, p % 384
+        o = out.view(-1, hw, hw, cs)[w, r - org[w, 0], c - org[w, 1], :S.N_FIELDS]
+        want = (o - exact[fm][:, p].T).abs().max().item()
+        assert rep.window_errors[w] == np.float32(want), (w, rep.window_errors[w], want)
+    assert rep.accepted == int((rep.window_errors <= 0.02).sum())
+    with pytest.raises(ValueError, match="criterion"):
+        S.SvfSurrogate(S.load_default(), criterion="median")
