@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane"), default="own",
                     help="tile workload: where the host-to-host copies run: on each lane's stream, on one "
                          "shared copy stream, or on a copy stream per lane")
+    ap.add_argument("--h2h-chunk", type=int, default=6,
+                    help="tile workload: day steps per chunk of the host-to-host region (each chunk's planes are "
+                         "copied out right after it)")
     ap.add_argument("--march", default="",
                     help="tile workload: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
     ap.add_argument("--slots", type=int, default=3, help="most execution slots of --workload serve")
@@ -727,11 +730,12 @@ def main():
     n_warm_h2h = max(n_warm, len(h2h_lanes) * solver.STAGING_DEPTH)
     spare = hbufs[args.steps:]
     solve_many_host(solver, h2h_lanes, [job] * n_warm_h2h, pipeline, host_in,
-                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs)
+                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs, chunk_steps=args.h2h_chunk)
     torch.cuda.synchronize()
     log(f"timing {args.steps} host-to-host solves over {len(h2h_lanes)} lane(s)")
     h0 = time.perf_counter()
-    done = solve_many_host(solver, h2h_lanes, [job] * args.steps, pipeline, host_in, hbufs[:args.steps], dev_outs)
+    done = solve_many_host(solver, h2h_lanes, [job] * args.steps, pipeline, host_in, hbufs[:args.steps], dev_outs,
+                           chunk_steps=args.h2h_chunk)
     torch.cuda.synchronize()
     h1 = time.perf_counter()
     hm = np.array(solver.pipeline_marks) * 1e3
@@ -898,7 +902,7 @@ def main():
         "config": {"workload": f"full SOLWEIG solve {rows}x{cols}x{T}h (SVF sweep + sun march + GVF + "
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
                    "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": len(h2h_lanes),
-                   "lanes_device_resident": len(lanes),
+                   "lanes_device_resident": len(lanes), "h2h_chunk_steps": args.h2h_chunk,
                    "timed_region": "host-resident inputs (6 rasters, pinned) -> host-resident outputs (16 SVF "
                                    "fields + 24 x {tmrt, utci, shadow}, pinned, one buffer set per solve): "
                                    "upload, SVF sweep, day loop, download; day plans in planner processes",

@@ -1247,12 +1247,56 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
         acc.fs += f;                                                                         \
     }
     int s = 0;
-    for (; s < nfirst; ++s) SW_WALK_STEP(s)
-    snap = acc;
+    if (INTERIOR) {
+        // Every source is in the tile, so the reads of a step depend on nothing computed in the
+        // walk: four steps' reads are issued together (two dependent LDS reads per step, whose
+        // latency otherwise stalls the wave), then the four steps are accumulated in order. After
+        // the wave's flag products are all 0, a further step adds exactly bub to kb and +0 to every
+        // other sum (0 * finite source), which is what the closed-form finish below adds for it,
+        // so testing for the exit once per group changes no result.
+#define SW_WALK_ACC(E, D)                                                                    \
+    {                                                                                        \
+        f = f * D.z;                                                                         \
+        acc.wlupsh = __builtin_fmaf(f, __uint_as_float(E.x), acc.wlupsh);                    \
+        acc.walbsh = __builtin_fmaf(f, D.x, acc.walbsh);                                     \
+        acc.walbnosh = __builtin_fmaf(f, D.y, acc.walbnosh);                                 \
+        bub = fmaxf(bub, D.w * f);                                                           \
+        acc.kb += bub;                                                                       \
+        acc.fs += f;                                                                         \
+    }
+#define SW_WALK_GROUP(S)                                                                     \
+    {                                                                                        \
+        const uint2 e0 = s_src[lbase + wdx[S] * lw + wdy[S]];                                \
+        const uint2 e1 = s_src[lbase + wdx[S + 1] * lw + wdy[S + 1]];                        \
+        const uint2 e2 = s_src[lbase + wdx[S + 2] * lw + wdy[S + 2]];                        \
+        const uint2 e3 = s_src[lbase + wdx[S + 3] * lw + wdy[S + 3]];                        \
+        const float4 d0 = *(const float4 *)((const char *)s_dec + (e0.y & 0x7F0u));          \
+        const float4 d1 = *(const float4 *)((const char *)s_dec + (e1.y & 0x7F0u));          \
+        const float4 d2 = *(const float4 *)((const char *)s_dec + (e2.y & 0x7F0u));          \
+        const float4 d3 = *(const float4 *)((const char *)s_dec + (e3.y & 0x7F0u));          \
+        SW_WALK_ACC(e0, d0) SW_WALK_ACC(e1, d1) SW_WALK_ACC(e2, d2) SW_WALK_ACC(e3, d3)      \
+    }
+        for (; s + 4 <= nfirst; s += 4) SW_WALK_GROUP(s)
+        for (; s < nfirst; ++s) SW_WALK_STEP(s)
+        snap = acc;
+        for (; s + 4 <= nsec; s += 4) {
+            if (!wave_any(f != 0.f)) break;
+            SW_WALK_GROUP(s)
+        }
+        for (; s < nsec; ++s) {
+            if (!wave_any(f != 0.f)) break;
+            SW_WALK_STEP(s)
+        }
+#undef SW_WALK_GROUP
+#undef SW_WALK_ACC
+    } else {
+        for (; s < nfirst; ++s) SW_WALK_STEP(s)
+        snap = acc;
 #pragma unroll 2
-    for (; s < nsec; ++s) {
-        if (!wave_any(f != 0.f)) break;
-        SW_WALK_STEP(s)
+        for (; s < nsec; ++s) {
+            if (!wave_any(f != 0.f)) break;
+            SW_WALK_STEP(s)
+        }
     }
 #undef SW_WALK_STEP
     // every lane has f == 0 here (or the walk is complete): bub stays and f
