@@ -199,10 +199,15 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
  * patch p (consecutive steps move at most one cell); the library applies
  * the reference's amax stop test with amax from stats, writing the executed
  * step count of each patch to counts[153]. hpad is caller-owned workspace of
- * (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 3 floats (below 2 GiB);
- * stats[3] is
+ * sw_svf_workspace_floats(rows, cols) floats: the padded heights,
+ * (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 3 floats (below 2 GiB), followed
+ * by the maxima of max(building, canopy) over SW_SVF_COARSE x SW_SVF_COARSE cells of
+ * the padded grid, which let the march skip chunks of steps that cannot change a
+ * mask (no counterpart in the reference; outputs unchanged); stats[3] is
  * rewritten (1 when some height is negative, which selects the
  * bounds-checked march). */
+#define SW_SVF_COARSE 8
+int64_t sw_svf_workspace_floats(int rows, int cols);
 int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
                  uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream);
 /* March launch configuration (no counterpart in the reference; chosen per tile shape by the
@@ -316,8 +321,8 @@ int sw_svf_sweep_cfg(const void *hpack, uint32_t *stats, const int32_t *offs, co
  * loops and harnesses that forbid hipMalloc). sw_svf_sweep_cfg without the
  * counts and hpad workspace arguments: both live in a static device array of
  * SW_SVF_SCRATCH_BYTES that each GPU the library runs on holds until the
- * process ends. Limit: (rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 12
- * + 1024 <= SW_SVF_SCRATCH_BYTES (tiles up to about 2300 x 2300); larger tiles
+ * process ends. Limit: sw_svf_workspace_floats(rows, cols) * 4 + 1024
+ * <= SW_SVF_SCRATCH_BYTES (tiles up to about 2300 x 2300); larger tiles
  * are refused with SW_E_ARG before anything is enqueued. Calls on one GPU
  * share its array: calls that overlap on one GPU (several streams or host
  * threads) race; serialise them on one stream. Allocates nothing (no

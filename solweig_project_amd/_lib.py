@@ -22,6 +22,14 @@ WALK_AZ = 18
 WALK_MAX = 64
 NCLASS = 8
 SVF_PAD = 16
+SVF_COARSE = 8          # SW_SVF_COARSE
+
+
+def svf_workspace_floats(rows: int, cols: int) -> int:
+    """Floats of the SVF march workspace ``hpad`` (``sw_svf_workspace_floats``): the padded heights (3 per
+    cell) followed by the maxima over SVF_COARSE x SVF_COARSE cells of the padded grid."""
+    H, W = rows + 2 * SVF_PAD, cols + 2 * SVF_PAD
+    return 3 * H * W + (-(-H // SVF_COARSE)) * (-(-W // SVF_COARSE))
 
 _c_i32 = ctypes.c_int32
 _c_f = ctypes.c_float
@@ -155,6 +163,7 @@ SIGNATURES = {
     "sw_copy_async": (_I, [_VP, _VP, _I64, _VP]),
     "sw_copy2d_async": (_I, [_VP, _I64, _VP, _I64, _I64, _I64, _VP]),
     "sw_fill_f32": (_I, [_VP, _F, _I64, _VP]),
+    "sw_svf_workspace_floats": (_I64, [_I, _I]),
     "sw_stream_create": (_I, [_VP]),
     "sw_stream_destroy": (_I, [_VP]),
     "sw_copy_to_host": (_I, [_VP, _VP, _I64, _VP]),

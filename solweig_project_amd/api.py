@@ -151,7 +151,7 @@ def svf_calculator(patch_option, amaxvalue=None, a=None, vegdem=None, vegdem2=No
     hpack = torch.empty((n, 4), dtype=torch.float32, device=dev)
     stats = torch.empty(4, dtype=torch.int32, device=dev)
     counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
-    hpad = torch.empty(((rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD), 3), dtype=torch.float32, device=dev)
+    hpad = torch.empty(L.svf_workspace_floats(rows, cols), dtype=torch.float32, device=dev)
     masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
     fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
     sp = _stream_ptr(None)

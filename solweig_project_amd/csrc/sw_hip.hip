@@ -224,6 +224,27 @@ k_pad_heights(const float4 *__restrict__ hp, float *__restrict__ hpad, uint32_t 
     if (threadIdx.x == 0 && s_neg) atomicOr(&stats[3], 1u);
 }
 
+// Maxima of max(building, canopy) over SW_SVF_COARSE x SW_SVF_COARSE cells of the padded grid,
+// stored behind the padded heights (the march's chunk-skip test)
+__global__ void __launch_bounds__(256)
+k_pad_coarse(const float *__restrict__ hpad, float *__restrict__ coarse, int H, int W, int Hc, int Wc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)Hc * Wc) return;
+    const int R = (int)(i / Wc), C = (int)(i - (long)R * Wc);
+    float m = 0.f;
+    for (int dr = 0; dr < SW_SVF_COARSE; ++dr) {
+        const int r = R * SW_SVF_COARSE + dr;
+        if (r >= H) break;
+        for (int dc = 0; dc < SW_SVF_COARSE; ++dc) {
+            const int c = C * SW_SVF_COARSE + dc;
+            if (c >= W) break;
+            const long q = 3 * ((long)r * W + c);
+            m = fmaxf(m, fmaxf(hpad[q], hpad[q + 1]));
+        }
+    }
+    coarse[i] = m;
+}
+
 struct H3 {
     float x, y, z;
 };
@@ -242,6 +263,11 @@ __device__ __forceinline__ H3 ld3(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off) 
 }
 
 __device__ __forceinline__ bool outside(const H3 &h) { return (__float_as_uint(h.z) >> 31) != 0u; }
+
+// one coarse maximum through a buffer resource (an offset past the array reads 0)
+__device__ __forceinline__ float ldc(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, byte_off, 0, 0));
+}
 
 // ---------------------------------------------------------------------------
 // SVF patch march. Semantics of shadow(): out-of-bounds sources read an
@@ -299,10 +325,18 @@ __device__ __forceinline__ uint32_t svf_out(const SvfLane &st) {
 // the last step of a chunk finds the exit the per-step test would have
 // found in that chunk, and the steps run past it change no output.
 #define SVF_CHUNK 8                 // default steps per gather chunk (autotuned: 4, 8 or 12)
+//
+// Chunk skip: when every cell a chunk's steps read has max(building, canopy) - dz <= a (tested on the
+// coarse maxima of the cells covering the chunk's bounding box against the chunk's smallest dz; float
+// subtraction is monotone, so each step's h - dz <= that bound), no step of the chunk sets sh or a
+// vegetation condition, and the chunk's whole effect is vs = vs && !sh, vbb = vbb || vs, which is
+// applied without reading the cells. The exit test then uses the last step's cell coordinates
+// (within the pad, an out-of-grid cell is exactly one whose z reads -0).
 template <int CHUNK>
 __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, uint32_t tpb, float at,
                                                    float veg2t, bool valid, const int4 *__restrict__ T,
-                                                   int count, float hmax) {
+                                                   int count, float hmax, __amdgpu_buffer_rsrc_t crs,
+                                                   int pr, int pc, int rows, int cols, int Hc, int Wc) {
     static_assert(CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
     SvfLane st;
     st.sh = false; st.vs = false; st.vbb = false;
@@ -319,14 +353,44 @@ __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, ui
         int4 e[CHUNK];
 #pragma unroll
         for (int k = 0; k < CHUNK; ++k) e[k] = T[s + k];
+        // the chunk's offset bounding box and smallest dz, the same for every lane (the tables move
+        // monotonically in x and y with non-decreasing dz, so the first and last steps bound them)
+        const int xl = min(e[0].x, e[CHUNK - 1].x), xh = max(e[0].x, e[CHUNK - 1].x);
+        const int yl = min(e[0].y, e[CHUNK - 1].y), yh = max(e[0].y, e[CHUNK - 1].y);
+        const float dzl = __int_as_float(e[0].z), dl = __int_as_float(e[CHUNK - 1].z);
+        bool skip = false;
         if (active) {
-            H3 h[CHUNK];
+            // coarse cells past the grid read 0 (range-checked loads), which the steps they cover
+            // would read as well
+            const int R0 = (pr + xl) >> 3, R1 = (pr + xh) >> 3, C0 = (pc + yl) >> 3, C1 = (pc + yh) >> 3;
+            if (R1 - R0 <= 1 && C1 - C0 <= 1) {
+                const uint32_t o00 = (uint32_t)(R0 * Wc + C0) * 4u, o01 = (uint32_t)(R0 * Wc + C1) * 4u;
+                const uint32_t o10 = (uint32_t)(R1 * Wc + C0) * 4u, o11 = (uint32_t)(R1 * Wc + C1) * 4u;
+                const bool inb = R0 >= 0 && C0 >= 0 && R1 < Hc && C1 < Wc;
+                const float m = fmaxf(fmaxf(ldc(crs, o00), ldc(crs, o01)), fmaxf(ldc(crs, o10), ldc(crs, o11)));
+                skip = inb && (m - dzl <= at);
+            }
+        }
+        // uniform branches: a wave whose active lanes all skip issues no gather and no step update
+        const bool need = active && !skip;
+        if (wave_any(need)) {
+            if (need) {
+                H3 h[CHUNK];
 #pragma unroll
-            for (int k = 0; k < CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
+                for (int k = 0; k < CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
 #pragma unroll
-            for (int k = 0; k < CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
-            const float dl = __int_as_float(e[CHUNK - 1].z);
-            active = !((st.sh & !st.vs) | outside(h[CHUNK - 1]) | (hmax - dl <= at));
+                for (int k = 0; k < CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
+                active = !((st.sh & !st.vs) | outside(h[CHUNK - 1]) | (hmax - dl <= at));
+            }
+        }
+        if (wave_any(active && skip)) {
+            if (active && skip) {
+                st.vs = st.vs & !st.sh;
+                st.vbb = st.vbb | st.vs;
+                const int lr = pr + e[CHUNK - 1].x - SW_SVF_PAD, lc = pc + e[CHUNK - 1].y - SW_SVF_PAD;
+                const bool out = ((unsigned)lr >= (unsigned)rows) | ((unsigned)lc >= (unsigned)cols);
+                active = !((st.sh & !st.vs) | out | (hmax - dl <= at));
+            }
         }
     }
     for (; s < count; ++s) {
@@ -419,6 +483,10 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)hpad, (short)0, (int)nbytes,
                                                                           0x00020000);
     const uint32_t tpb = valid ? (uint32_t)(((long)(r + SW_SVF_PAD) * W + (c + SW_SVF_PAD)) * 12) : 0u;
+    const int Hc = (rows + 2 * SW_SVF_PAD + SW_SVF_COARSE - 1) / SW_SVF_COARSE;
+    const int Wc = (W + SW_SVF_COARSE - 1) / SW_SVF_COARSE;
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(hpad + 3 * (long)(rows + 2 * SW_SVF_PAD) * W), (short)0, (int)((long)Hc * Wc * 4), 0x00020000);
     const long t = valid ? (long)r * cols + c : 0;
     float at = 0.f, veg2t = 0.f;
     if (valid) {
@@ -439,7 +507,8 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
         const bool chk = checked || (counts[p] > 0 && __int_as_float(tab[offs[p]].z) < 0.f);
         const uint32_t m = chk
             ? march_svf_checked(rs, tpb, r, c, rows, cols, at, veg2t, valid, tab + offs[p], counts[p], hmax)
-            : march_svf_fast<CHUNK>(rs, tpb, at, veg2t, valid, tab + offs[p], counts[p], hmax);
+            : march_svf_fast<CHUNK>(rs, tpb, at, veg2t, valid, tab + offs[p], counts[p], hmax, crs,
+                                    r + SW_SVF_PAD, c + SW_SVF_PAD, rows, cols, Hc, Wc);
         const uint32_t bit = 1u << (p - p0);
         if (m & 1u) bsh |= bit;
         if (m & 2u) bvs |= bit;
@@ -1749,6 +1818,23 @@ int sw_compose(const float *dsm, const float *dem, const float *cdsm, const floa
     return hip_check(hipGetLastError(), "k_compose");
 }
 
+int64_t sw_svf_workspace_floats(int rows, int cols) {
+    if (rows <= 0 || cols <= 0) return 0;
+    const int64_t H = rows + 2 * SW_SVF_PAD, W = cols + 2 * SW_SVF_PAD;
+    const int64_t Hc = (H + SW_SVF_COARSE - 1) / SW_SVF_COARSE, Wc = (W + SW_SVF_COARSE - 1) / SW_SVF_COARSE;
+    return 3 * H * W + Hc * Wc;
+}
+
+// the coarse maxima behind the padded heights
+static int pad_coarse(void *hpad, int rows, int cols, void *stream) {
+    const int H = rows + 2 * SW_SVF_PAD, W = cols + 2 * SW_SVF_PAD;
+    const int Hc = (H + SW_SVF_COARSE - 1) / SW_SVF_COARSE, Wc = (W + SW_SVF_COARSE - 1) / SW_SVF_COARSE;
+    float *coarse = (float *)hpad + 3 * (long)H * W;
+    hipLaunchKernelGGL(k_pad_coarse, dim3(grid_for((long)Hc * Wc, 256)), dim3(256), 0, S(stream),
+                       (const float *)hpad, coarse, H, W, Hc, Wc);
+    return hip_check(hipGetLastError(), "k_pad_coarse");
+}
+
 int sw_svf_march(const void *hpack, const int32_t *offs, const int32_t *tab, int32_t *counts, void *hpad,
                  uint32_t *masks, uint32_t *stats, int rows, int cols, void *stream) {
     return sw_svf_march_cfg(hpack, offs, tab, counts, hpad, masks, stats, rows, cols, nullptr, stream);
@@ -1773,6 +1859,8 @@ int sw_svf_march_cfg(const void *hpack, const int32_t *offs, const int32_t *tab,
     hipLaunchKernelGGL(k_pad_heights, dim3(grid_for(padded, 256)), dim3(256), 0, S(stream), (const float4 *)hpack,
                        (float *)hpad, stats, rows, cols);
     rc = hip_check(hipGetLastError(), "k_pad_heights");
+    if (rc) return rc;
+    rc = pad_coarse(hpad, rows, cols, stream);
     if (rc) return rc;
     const long tiles = (long)((rows + SVF_TILE - 1) / SVF_TILE) * ((cols + SVF_TILE - 1) / SVF_TILE);
     const int chunk = cfg ? cfg->chunk : SVF_CHUNK, shape = cfg ? cfg->shape : 0, split = cfg ? cfg->split : 1;
@@ -1822,8 +1910,7 @@ int sw_svf_sweep_noalloc(const void *hpack, uint32_t *stats, const int32_t *offs
     if (!hpack || !stats || !offs || !tab || !vegdsm2 || !w_iso || !w_aniso || !ring || !nann || !dirm || !masks ||
         !fields)
         return set_err(SW_E_ARG, "sw_svf_sweep_noalloc: NULL pointer");
-    const long long need =
-        (long long)(rows + 2 * SW_SVF_PAD) * (cols + 2 * SW_SVF_PAD) * 12 + SW_SCRATCH_COUNTS_BYTES;
+    const long long need = (long long)sw_svf_workspace_floats(rows, cols) * 4 + SW_SCRATCH_COUNTS_BYTES;
     if (need > (long long)SW_SVF_SCRATCH_BYTES) {
         char buf[200];
         snprintf(buf, sizeof(buf),
@@ -1834,8 +1921,9 @@ int sw_svf_sweep_noalloc(const void *hpack, uint32_t *stats, const int32_t *offs
     }
     void *base = nullptr;
     if ((rc = scratch_ptr(&base))) return rc;
-    // both parts are rewritten in full before they are read: k_svf_counts writes all 153 counts and
-    // k_pad_heights every padded cell, so the previous call's contents never leak into this one
+    // both parts are rewritten in full before they are read: k_svf_counts writes all 153 counts,
+    // k_pad_heights every padded cell and k_pad_coarse every coarse maximum, so the previous call's
+    // contents never leak into this one
     int32_t *counts = (int32_t *)base;
     void *hpad = (unsigned char *)base + SW_SCRATCH_COUNTS_BYTES;
     return sw_svf_sweep_cfg(hpack, stats, offs, tab, vegdsm2, w_iso, w_aniso, ring, nann, dirm, counts, hpad, masks,
@@ -1862,6 +1950,8 @@ int sw_svf_sweep_tiles(const void *hpack, uint32_t *stats, const int32_t *offs, 
     hipLaunchKernelGGL(k_pad_heights, dim3(grid_for(padded, 256)), dim3(256), 0, S(stream), (const float4 *)hpack,
                        (float *)hpad, stats, rows, cols);
     rc = hip_check(hipGetLastError(), "k_pad_heights");
+    if (rc) return rc;
+    rc = pad_coarse(hpad, rows, cols, stream);
     if (rc) return rc;
     const unsigned g = (unsigned)(((ntiles + 7) / 8) * 8);
     // short lists (under 8 blocks per CU at one word per block) split each word over four blocks

@@ -137,7 +137,7 @@ def slot_footprint(st: I.SiteState) -> int:
     inputs = sum(v.numel() * v.element_size() for v in st.scene.inputs.values() if v is not None)
     # hpack, vegdsm2, buildings, cls, aspect, masks, fields, svfbuveg, tan_asvf, tan_sa, walkface, src, state
     per_px = 16 + 4 + 4 + 1 + 4 + 4 * L.MASK_PLANES * L.MASK_WORDS + 4 * L.NFIELDS + 4 * 3 + 4 + 8 + 4 * 6
-    hpad = (rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD) * 12
+    hpad = L.svf_workspace_floats(rows, cols) * 4
     return inputs + per_px * n + hpad + 4 * L.NPATCH + 16 + st.out.numel() * st.out.element_size()
 
 

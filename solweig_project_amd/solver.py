@@ -366,7 +366,7 @@ class DeviceScene:
         self.aspect = torch.empty(n, dtype=torch.float32, device=dev)
         self.stats = torch.empty(4, dtype=torch.int32, device=dev)
         self.counts = torch.empty(L.NPATCH, dtype=torch.int32, device=dev)
-        self.hpad = torch.empty(((rows + 2 * L.SVF_PAD) * (cols + 2 * L.SVF_PAD), 3), dtype=torch.float32,
+        self.hpad = torch.empty(L.svf_workspace_floats(rows, cols), dtype=torch.float32,
                                 device=dev)
         self.masks = torch.empty((L.MASK_PLANES * L.MASK_WORDS, n), dtype=torch.int32, device=dev)
         self.fields = torch.empty((L.NFIELDS, n), dtype=torch.float32, device=dev)
@@ -582,6 +582,12 @@ class HipSolver:
                 if b > a and (max(abs(int(dx[a])), abs(int(dy[a]))) > 1 or (b - a > 1 and max(
                         np.abs(np.diff(dx[a:b])).max(), np.abs(np.diff(dy[a:b])).max()) > 1)):
                     raise UnsupportedInput(f"patch {p}: march step longer than one cell")
+                # the fast march bounds a chunk of steps by its first and last step: monotone x and y,
+                # non-decreasing dz (a table with negative dz, the zenith patch, takes the checked march)
+                if b - a > 1 and dz[a] >= 0 and not ((np.all(np.diff(dx[a:b]) >= 0) or np.all(np.diff(dx[a:b]) <= 0)) and (
+                        np.all(np.diff(dy[a:b]) >= 0) or np.all(np.diff(dy[a:b]) <= 0)) and np.all(
+                        np.diff(dz[a:b]) >= 0)):
+                    raise UnsupportedInput(f"patch {p}: march steps not monotone")
             pitch = cols + 2 * L.SVF_PAD
             tab = np.stack([dx, dy, dz.view(np.int32), (dx * pitch + dy) * 12], axis=1).astype(np.int32)
             self._tables[key] = (torch.from_numpy(np.ascontiguousarray(offs)).to(self.device),
