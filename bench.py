@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--h2h-chunk", type=int, default=6,
                     help="tile workload: day steps per chunk of the host-to-host region (each chunk's planes are "
                          "copied out right after it)")
+    ap.add_argument("--h2h-fields", choices=("last", "first"), default="last",
+                    help="tile workload: copy each solve's SVF fields after its day (default) or before it")
     ap.add_argument("--march", default="",
                     help="tile workload: pin the SVF march launch to CHUNK,SHAPE,SPLIT (e.g. 8,0,2) instead of autotuning")
     ap.add_argument("--slots", type=int, default=3, help="most execution slots of --workload serve")
@@ -730,12 +732,13 @@ def main():
     n_warm_h2h = max(n_warm, len(h2h_lanes) * solver.STAGING_DEPTH)
     spare = hbufs[args.steps:]
     solve_many_host(solver, h2h_lanes, [job] * n_warm_h2h, pipeline, host_in,
-                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs, chunk_steps=args.h2h_chunk)
+                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs, chunk_steps=args.h2h_chunk,
+                    fields_last=args.h2h_fields == "last")
     torch.cuda.synchronize()
     log(f"timing {args.steps} host-to-host solves over {len(h2h_lanes)} lane(s)")
     h0 = time.perf_counter()
     done = solve_many_host(solver, h2h_lanes, [job] * args.steps, pipeline, host_in, hbufs[:args.steps], dev_outs,
-                           chunk_steps=args.h2h_chunk)
+                           chunk_steps=args.h2h_chunk, fields_last=args.h2h_fields == "last")
     torch.cuda.synchronize()
     h1 = time.perf_counter()
     hm = np.array(solver.pipeline_marks) * 1e3
@@ -903,6 +906,7 @@ def main():
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
                    "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": len(h2h_lanes),
                    "lanes_device_resident": len(lanes), "h2h_chunk_steps": args.h2h_chunk,
+                   "h2h_fields_copy": args.h2h_fields,
                    "timed_region": "host-resident inputs (6 rasters, pinned) -> host-resident outputs (16 SVF "
                                    "fields + 24 x {tmrt, utci, shadow}, pinned, one buffer set per solve): "
                                    "upload, SVF sweep, day loop, download; day plans in planner processes",

@@ -140,7 +140,7 @@ def pinned_inputs(tile):
 
 
 def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, host_outs, dev_outs,
-                    outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 6):
+                    outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 6, fields_last: bool = True):
     """Host-to-host solves: for job k, upload the pinned input rasters to lane k mod len(lanes),
     run the SVF sweep and the day, and deliver the 16 SVF fields and every step's output planes
     into ``host_outs[k]`` (pinned :class:`HostBuffers`).
@@ -156,7 +156,10 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
     stream-ordered with events, never a host wait: a lane's next sweep waits for the copy of its
     previous fields, and a day waits for the copies out of the device buffer it is about to reuse
     (``len(dev_outs)`` jobs back). Plans come from the planner processes, ``pipeline.workers`` jobs
-    ahead. Returns each job's completion event (its last copy), in job order."""
+    ahead. ``fields_last`` (own-stream copies only): the SVF fields go out after the day rather than
+    right after the sweep, so the day starts without waiting for them on the copy engine (the
+    lane's next sweep follows the copy in stream order). Returns each job's
+    completion event (its last copy), in job order."""
     import torch
 
     from . import _lib as L
@@ -187,9 +190,11 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
             solver.svf(ds, stream)
             ev_fields = torch.cuda.Event()
             ev_fields.record(stream)
+        late = fields_last and own
         if not own:
             cstream.wait_event(ev_fields)
-        copy_out(hb.fields, ds.fields, cstream)
+        if not late:
+            copy_out(hb.fields, ds.fields, cstream)
         if not own:
             fields_out[li] = torch.cuda.Event()
             fields_out[li].record(cstream)
@@ -212,6 +217,8 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
                 ev.record(stream)
                 cstream.wait_event(ev)
             copy_out(hb.out[t0:t1], out[t0:t1], cstream)
+        if late:
+            copy_out(hb.fields, ds.fields, cstream)
         fin = torch.cuda.Event(enable_timing=True)
         fin.record(cstream)
         buf_free[bi] = fin

@@ -50,3 +50,39 @@ def test_two_lanes_equal_solves_alone():
         torch.cuda.synchronize()
         assert np.array_equal(lanes[k][2].cpu().numpy(), alone.cpu().numpy(), equal_nan=True), k
         assert torch.equal(lanes[k][0].fields, ds.fields)
+
+
+@pytest.mark.parametrize("fields_last", [True, False])
+def test_host_to_host_solves_deliver_what_the_device_computes(fields_last):
+    """The host-to-host pipeline (pinned inputs uploaded per job, day in chunks, each chunk copied to
+    the job's pinned host buffers on the copy engines, SVF fields copied before or after the day)
+    delivers every job's fields and outputs as the job solved alone computes them, over three lanes
+    and a ring of two device output buffers (so a buffer is reused while earlier copies run)."""
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    from solweig_project_amd import sunpos, synth
+    from solweig_project_amd.pipeline import HostBuffers, Job, pinned_inputs, solve_many, solve_many_host
+    from solweig_project_amd.solver import DeviceScene, HipSolver, TileInputs, upload_inputs
+    print(f"[gpu-test] host-to-host lanes, fields_last={fields_last}", flush=True)
+    site = synth.make_site(160, 144, seed=71)
+    tile = TileInputs.from_site(site)
+    sun = sunpos.day_sun_table(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    jobs = [Job(synth.make_met(sun.altitude, seed=72 + k), sun, site.latitude) for k in range(5)]
+    solver = HipSolver("cuda:0")
+    dev = solver.device
+    T = jobs[0].met.shape[0]
+    lanes = [(DeviceScene(tile, dev, upload_inputs(tile, dev)), torch.cuda.Stream(dev), None) for _ in range(3)]
+    host_in = pinned_inputs(tile)
+    hb = [HostBuffers(torch.full((16, 160, 144), -1.0).pin_memory(), torch.full((T, 3, 160, 144), -1.0).pin_memory())
+          for _ in jobs]
+    dev_outs = [torch.empty((T, 3, 160, 144), device=dev) for _ in range(2)]
+    done = solve_many_host(solver, lanes, jobs, _InlinePlanner(), host_in, hb, dev_outs, chunk_steps=5,
+                           fields_last=fields_last)
+    assert len(done) == len(jobs)
+    torch.cuda.synchronize()
+    for k, job in enumerate(jobs):
+        ds = DeviceScene(tile, dev, upload_inputs(tile, dev))
+        alone = solve_many(solver, ds, [job], _InlinePlanner(), out=torch.empty_like(dev_outs[0]))
+        torch.cuda.synchronize()
+        assert np.array_equal(hb[k].out.numpy(), alone.cpu().numpy(), equal_nan=True), k
+        assert np.array_equal(hb[k].fields.numpy().reshape(16, -1), ds.fields.cpu().numpy()), k
