@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane"), default="own",
                     help="tile workload: where the host-to-host copies run: on each lane's stream, on one "
                          "shared copy stream, or on a copy stream per lane")
-    ap.add_argument("--h2h-chunk", type=int, default=6,
+    ap.add_argument("--h2h-chunk", type=int, default=8,
                     help="tile workload: day steps per chunk of the host-to-host region (each chunk's planes are "
                          "copied out right after it)")
     ap.add_argument("--h2h-fields", choices=("last", "first"), default="last",

@@ -140,7 +140,7 @@ def pinned_inputs(tile):
 
 
 def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, host_outs, dev_outs,
-                    outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 6, fields_last: bool = True):
+                    outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 8, fields_last: bool = True):
     """Host-to-host solves: for job k, upload the pinned input rasters to lane k mod len(lanes),
     run the SVF sweep and the day, and deliver the 16 SVF fields and every step's output planes
     into ``host_outs[k]`` (pinned :class:`HostBuffers`).
