@@ -908,7 +908,7 @@ __device__ __forceinline__ float pow4(float x) {
 
 static size_t flux_lds_bytes(int walk_halo) {
     if (walk_halo < 0) return 0;
-    const size_t tile = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo) * sizeof(uint2);
+    const size_t tile = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo);   // one flag byte per cell
     const size_t table = (size_t)SW_NPATCH * 4 * sizeof(float2);   // overlays the tile after the walk
     return tile > table ? tile : table;
 }
@@ -1287,7 +1287,7 @@ struct WalkAcc {
 };
 
 template <bool INTERIOR>
-__device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, const float4 *__restrict__ s_dec,
+__device__ __forceinline__ void walk_azimuth(const uint8_t *__restrict__ s_src, const float4 *__restrict__ s_dec,
                                              const sw_model_t *__restrict__ md, int j, int nsec, int nfirst,
                                              int lbase, int lw, int r, int c, int rows, int cols, float fb,
                                              WalkAcc &acc, WalkAcc &snap) {
@@ -1302,10 +1302,9 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
         if (!INTERIOR)                                                                       \
             live = live && ((unsigned)(r + ddx) < (unsigned)rows) && ((unsigned)(c + ddy) < (unsigned)cols); \
         if (INTERIOR || live) {                                                              \
-            const uint2 e = s_src[lbase + ddx * lw + ddy];                                   \
-            const float4 d = *(const float4 *)((const char *)s_dec + (e.y & 0x7F0u));        \
-            ta = d.x; tn = d.y; tb = d.z; tw = d.w;                                          \
-            tl = __uint_as_float(e.x);                                                       \
+            const float4 d = s_dec[s_src[lbase + ddx * lw + ddy]];                          \
+            tl = d.x; ta = d.y; tn = fabsf(d.z); tw = d.w;                                   \
+            tb = (__float_as_int(d.z) >= 0) ? 1.f : 0.f;                                     \
         }                                                                                    \
         f = f * tb;                                                                          \
         acc.wlupsh = __builtin_fmaf(f, tl, acc.wlupsh);                                      \
@@ -1325,24 +1324,21 @@ __device__ __forceinline__ void walk_azimuth(const uint2 *__restrict__ s_src, co
         // so testing for the exit once per group changes no result.
 #define SW_WALK_ACC(E, D)                                                                    \
     {                                                                                        \
-        f = f * D.z;                                                                         \
-        acc.wlupsh = __builtin_fmaf(f, __uint_as_float(E.x), acc.wlupsh);                    \
-        acc.walbsh = __builtin_fmaf(f, D.x, acc.walbsh);                                     \
-        acc.walbnosh = __builtin_fmaf(f, D.y, acc.walbnosh);                                 \
+        f = (__float_as_int(D.z) >= 0) ? f : 0.f;                                            \
+        acc.wlupsh = __builtin_fmaf(f, D.x, acc.wlupsh);                                     \
+        acc.walbsh = __builtin_fmaf(f, D.y, acc.walbsh);                                     \
+        acc.walbnosh = __builtin_fmaf(f, fabsf(D.z), acc.walbnosh);                          \
         bub = fmaxf(bub, D.w * f);                                                           \
         acc.kb += bub;                                                                       \
         acc.fs += f;                                                                         \
     }
 #define SW_WALK_GROUP(S)                                                                     \
     {                                                                                        \
-        const uint2 e0 = s_src[lbase + wdx[S] * lw + wdy[S]];                                \
-        const uint2 e1 = s_src[lbase + wdx[S + 1] * lw + wdy[S + 1]];                        \
-        const uint2 e2 = s_src[lbase + wdx[S + 2] * lw + wdy[S + 2]];                        \
-        const uint2 e3 = s_src[lbase + wdx[S + 3] * lw + wdy[S + 3]];                        \
-        const float4 d0 = *(const float4 *)((const char *)s_dec + (e0.y & 0x7F0u));          \
-        const float4 d1 = *(const float4 *)((const char *)s_dec + (e1.y & 0x7F0u));          \
-        const float4 d2 = *(const float4 *)((const char *)s_dec + (e2.y & 0x7F0u));          \
-        const float4 d3 = *(const float4 *)((const char *)s_dec + (e3.y & 0x7F0u));          \
+        const uint32_t e0 = s_src[lbase + wdx[S] * lw + wdy[S]];                             \
+        const uint32_t e1 = s_src[lbase + wdx[S + 1] * lw + wdy[S + 1]];                     \
+        const uint32_t e2 = s_src[lbase + wdx[S + 2] * lw + wdy[S + 2]];                     \
+        const uint32_t e3 = s_src[lbase + wdx[S + 3] * lw + wdy[S + 3]];                     \
+        const float4 d0 = s_dec[e0], d1 = s_dec[e1], d2 = s_dec[e2], d3 = s_dec[e3];          \
         SW_WALK_ACC(e0, d0) SW_WALK_ACC(e1, d1) SW_WALK_ACC(e2, d2) SW_WALK_ACC(e3, d3)      \
     }
         for (; s + 4 <= nfirst; s += 4) SW_WALK_GROUP(s)
@@ -1414,15 +1410,22 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
         for (int i = threadIdx.x; i < lw * lh; i += blockDim.x) {
             const int ly = i / lw, lx = i - ly * lw;
             const int gr = r0 - halo + ly, gc = c0 - halo + lx;
-            uint2 v = make_uint2(0u, 0u);
-            if ((unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols) v = src[(long)gr * cols + gc];
-            s_src[i] = v;
+            uint32_t v = 0u;
+            if ((unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols)
+                v = ((const uint32_t *)src)[2 * ((long)gr * cols + gc) + 1];
+            ((uint8_t *)s_src)[i] = (uint8_t)(v >> 4);
         }
         if (threadIdx.x < 128) {
+            // decode of a walk source's flag byte {sh, vegsh, sunwall, ground, class}: its Lup source
+            // (the sun step's expression, so the same bits), albedo x shadow, albedo signed by the
+            // ground flag (the sign bit carries it, also for albedo 0), sunwall
             const int shb = threadIdx.x & 1, vsb = (threadIdx.x >> 1) & 1, k = (threadIdx.x >> 4) & 7;
             const float al = md->alb_class[k];
             const float shd = (float)shb - (1.f - (float)vsb) * omp;
-            s_dec[threadIdx.x] = make_float4(al * shd, al, (float)((threadIdx.x >> 3) & 1),
+            const float se = md->sbc * md->emis_class[k];
+            const float xx = sp->tg_class[k] * shd + sp->ta + 273.15f;
+            const float tl = se * pow4(xx) - se * sp->tak4;
+            s_dec[threadIdx.x] = make_float4(tl, al * shd, ((threadIdx.x >> 3) & 1) ? al : -al,
                                              (float)((threadIdx.x >> 2) & 1));
         }
         if (threadIdx.x == 128) {
@@ -1480,9 +1483,11 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
         for (int j = 0; j < SW_WALK_AZ; ++j) {
             WalkAcc A, F;
             if (interior)
-                walk_azimuth<true>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
+                walk_azimuth<true>((const uint8_t *)s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols,
+                                   fb, A, F);
             else
-                walk_azimuth<false>(s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows, cols, fb, A, F);
+                walk_azimuth<false>((const uint8_t *)s_src, s_dec, md, j, nsec, nfirst, lbase, lw, r, c, rows,
+                                    cols, fb, A, F);
             const int kb = (int)A.kb, kw = nsec - (int)A.fs, fkb = (int)F.kb, fkw = nfirst - (int)F.fs;
             const float wwall = A.kb, wlwall = s_tlw[kb], walbwall = s_tal[kb];
             const float walbwallnosh = s_tal[kw];
