@@ -1656,8 +1656,9 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     if (out_mask & 128) out[(long)(k++) * n + t] = wbgt_point(tmrt, shadow, sp->ta_mat, sp->wbgt_w, sp->wbgt_a);
 }
 
+// 8 waves per SIMD (63 VGPRs, a few spilled; the walk waits on LDS and gains from the extra wave)
 template <bool WALK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
             const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
             const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
@@ -1670,7 +1671,7 @@ k_flux_step(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
 }
 
 template <bool WALK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_flux_tiles(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
              const float *__restrict__ svfbuveg, const float *__restrict__ tan_asvf,
              const float *__restrict__ tan_sa, const uint32_t *__restrict__ walkface,
@@ -1687,7 +1688,7 @@ k_flux_tiles(const uint32_t *__restrict__ masks, const uint2 *__restrict__ src,
 }
 
 template <bool WALK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_flux_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const sw_model_t *__restrict__ model,
              int out_mask, int out_step, int step, int halo) {
     const int g = xcd_tile(blockIdx.x, gridDim.x);
