@@ -1028,8 +1028,11 @@ k_sun_step(const float4 *__restrict__ hp, const float *__restrict__ walls,
            const sw_model_t *__restrict__ model, const int4 *__restrict__ suntab,
            const uint32_t *__restrict__ stats, uint2 *__restrict__ src, int step, int rows, int cols,
            float *__restrict__ planes) {
-    sun_step_body<false, PLANES>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols,
-                                 xcd_tile(blockIdx.x, gridDim.x), planes);
+    // 16 x 16 pixel tiles, 8 x 8 pixels per wave (exits are spatially correlated)
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    if (g >= ((cols + FLUX_TX - 1) / FLUX_TX) * ((rows + FLUX_TY - 1) / FLUX_TY)) return;
+    sun_step_body<true, PLANES>(hp, walls, aspect, cls, bu, steps, model, suntab, stats, src, step, rows, cols, g,
+                                planes);
 }
 
 // the sun step over listed 16 x 16 tiles
@@ -1097,10 +1100,10 @@ k_sun_batch(const BatchItemDev *__restrict__ items, int nitems, int total, const
     if (g >= total) return;
     const BatchItemDev *it = items + batch_owner<0>(items, nitems, g);
     const BatchTileDev &T = it->tile;
-    sun_step_body<false>((const float4 *)T.hpack, (const float *)T.walls, (const float *)T.aspect_rad,
-                  (const uint8_t *)T.cls, (const float *)T.buildings, (const sw_step_t *)it->steps, model,
-                  (const int4 *)it->sun_tab, (const uint32_t *)T.stats, (uint2 *)T.src, step, T.rows, T.cols,
-                  g - it->sun_block0);
+    sun_step_body<true>((const float4 *)T.hpack, (const float *)T.walls, (const float *)T.aspect_rad,
+                        (const uint8_t *)T.cls, (const float *)T.buildings, (const sw_step_t *)it->steps, model,
+                        (const int4 *)it->sun_tab, (const uint32_t *)T.stats, (uint2 *)T.src, step, T.rows, T.cols,
+                        g - it->sun_block0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1768,7 +1771,6 @@ __global__ void k_march_wh23_single(const float *__restrict__ a, const float *__
 // C ABI
 // ---------------------------------------------------------------------------
 static inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
-static inline int grid_xcd(long n) { return ((grid_for(n, 256) + 7) / 8) * 8; }
 
 extern "C" {
 
@@ -2140,13 +2142,16 @@ static int sun_step(const void *hpack, const float *walls, const float *aspect_r
         return set_err(SW_E_ARG, buf);
     }
     const long n = (long)rows * cols;
+    (void)n;
+    const long tiles = (long)((rows + FLUX_TY - 1) / FLUX_TY) * ((cols + FLUX_TX - 1) / FLUX_TX);
+    const unsigned tg = (unsigned)(((tiles + 7) / 8) * 8);
     if (planes) {
-        hipLaunchKernelGGL(k_sun_step<true>, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack,
+        hipLaunchKernelGGL(k_sun_step<true>, dim3(tg), dim3(256), 0, S(stream), (const float4 *)hpack,
                            walls, aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src,
                            step, rows, cols, planes);
         return hip_check(hipGetLastError(), "k_sun_step<planes>");
     }
-    hipLaunchKernelGGL(k_sun_step<false>, dim3(grid_xcd(n)), dim3(256), 0, S(stream), (const float4 *)hpack, walls,
+    hipLaunchKernelGGL(k_sun_step<false>, dim3(tg), dim3(256), 0, S(stream), (const float4 *)hpack, walls,
                        aspect_rad, cls, buildings, steps, model, (const int4 *)sun_tab, stats, (uint2 *)src, step,
                        rows, cols, (float *)nullptr);
     return hip_check(hipGetLastError(), "k_sun_step");
@@ -2198,7 +2203,10 @@ int sw_flux_step(const uint32_t *masks, const uint32_t *src, const float *svfbuv
     return hip_check(hipGetLastError(), "k_flux_step");
 }
 
-static int item_blocks_sun(const sw_tile_t &t) { return grid_for((long)t.rows * t.cols, 256); }
+// the sun march runs on the same 16 x 16 tiles as the flux step (8 x 8 pixels per wave)
+static int item_blocks_sun(const sw_tile_t &t) {
+    return ((t.cols + FLUX_TX - 1) / FLUX_TX) * ((t.rows + FLUX_TY - 1) / FLUX_TY);
+}
 static int item_blocks_flux(const sw_tile_t &t) {
     return ((t.cols + FLUX_TX - 1) / FLUX_TX) * ((t.rows + FLUX_TY - 1) / FLUX_TY);
 }
