@@ -255,6 +255,36 @@ def run_batch(args, reps=None):
         graph_ms = ev[0].elapsed_time(ev[1])
         L.call("sw_graph_destroy", gph)
         torch.cuda.synchronize()
+        # the same day in one persistent launch (sw_solve_batch_persistent): eager and captured, and its
+        # outputs against the per-phase launches' (bit for bit)
+        ref_outs = [o.clone() for o in b.outs]
+        b.run(persistent=True)
+        torch.cuda.synchronize()
+        b.check()
+        ev[0].record()
+        b.run(persistent=True)
+        ev[1].record()
+        torch.cuda.synchronize()
+        b.check()
+        pers_ms = ev[0].elapsed_time(ev[1])
+        pers_equal = all(torch.equal(o.view(torch.int32), r.view(torch.int32)) for o, r in zip(b.outs, ref_outs))
+        L.call("sw_graph_begin", s.cuda_stream)
+        with torch.cuda.stream(s):
+            b.run(stream=s, persistent=True)
+        L.call("sw_graph_end", s.cuda_stream, __import__("ctypes").byref(gph))
+        L.call("sw_graph_launch", gph, s.cuda_stream)
+        s.synchronize()
+        with torch.cuda.stream(s):
+            ev[0].record(s)
+            L.call("sw_graph_launch", gph, s.cuda_stream)
+            ev[1].record(s)
+        s.synchronize()
+        b.check()
+        pers_graph_ms = ev[0].elapsed_time(ev[1])
+        L.call("sw_graph_destroy", gph)
+        pers_equal = pers_equal and all(torch.equal(o.view(torch.int32), r.view(torch.int32))
+                                        for o, r in zip(b.outs, ref_outs))
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         sequential(scenes, b)
         torch.cuda.synchronize()
@@ -262,8 +292,11 @@ def run_batch(args, reps=None):
         res[g] = {"items": len(scenes), "mpx": round(px[g] / 1e6, 3), "ms": round(ms, 2),
                   "mpixel_steps_per_s": round(px[g] * T / (ms / 1e3) / 1e6, 1),
                   "day_batched_ms": round(day_ms, 2), "day_graph_replay_ms": round(graph_ms, 2),
+                  "day_persistent_ms": round(pers_ms, 2), "day_persistent_graph_replay_ms": round(pers_graph_ms, 2),
+                  "persistent_equals_per_phase_launches": bool(pers_equal),
                   "launches_per_day_batched": int(sum(b.is_day)) * 2 + (T - int(sum(b.is_day))),
                   "sequential_one_launch_per_tile_ms": round(seq_ms, 2)}
+        log(f"{g}: persistent day {pers_ms:.1f} ms eager, {pers_graph_ms:.1f} ms graph, equal: {pers_equal}")
         log(f"{g}: {ms:.1f} ms per batched solve; day {day_ms:.1f} ms eager, {graph_ms:.1f} ms graph; "
             f"one launch per tile {seq_ms:.1f} ms")
     tot_px = sum(px.values())

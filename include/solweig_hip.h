@@ -381,6 +381,24 @@ int sw_solve_batch(const sw_batch_item_t *items_dev, const sw_batch_item_t *item
                    const sw_model_t *model, const int32_t *is_day, int t0, int t1, int out_mask, int walk_halo,
                    int reset_state, void *stream);
 
+/* The same batched day loop in one persistent launch (per 64 steps): every
+ * block stays resident and runs the sun-march and flux phases of every step,
+ * drawing the batch's tiles from per-XCD counters, with grid-wide barriers
+ * between the phases (the reference has no counterpart; it replaces the
+ * per-step launches of sw_solve_batch, whose outputs it equals bit for bit).
+ * The grid is as many blocks as the device holds at once (cooperative launch;
+ * a plain launch of the same grid under stream capture), at most one per
+ * tile. `sync` is a device workspace of SW_DAY_SYNC_WORDS uint32 (barrier and
+ * tile counters, cleared on the stream before each launch); after the stream
+ * completes, sync[SW_DAY_SYNC_ERR] != 0 means a barrier wait exceeded
+ * timeout_s and the launch stopped early (its outputs are incomplete). Two
+ * such launches must not run at once on one GPU. */
+#define SW_DAY_SYNC_WORDS 256
+#define SW_DAY_SYNC_ERR 9
+int sw_solve_batch_persistent(const sw_batch_item_t *items_dev, const sw_batch_item_t *items_host, int nitems,
+                              const sw_model_t *model, const int32_t *is_day, int t0, int t1, int out_mask,
+                              int walk_halo, int reset_state, uint32_t *sync, double timeout_s, void *stream);
+
 /* UTCI of elementwise inputs (utci_calculator); -999 inputs give -999. */
 int sw_utci(const float *ta, const float *rh, const float *tmrt, const float *va, float *out,
             int64_t n, void *stream);

@@ -23,6 +23,8 @@ WALK_MAX = 64
 NCLASS = 8
 SVF_PAD = 16
 SVF_COARSE = 8          # SW_SVF_COARSE
+DAY_SYNC_WORDS = 256    # SW_DAY_SYNC_WORDS: barrier workspace of sw_solve_batch_persistent
+DAY_SYNC_ERR = 9        # SW_DAY_SYNC_ERR: its error-flag word
 
 
 def svf_workspace_floats(rows: int, cols: int) -> int:
@@ -202,6 +204,7 @@ SIGNATURES = {
     "sw_solve_day_tiles": (_I, [_VP] * 5 + [_I, _I, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP]),
     "sw_batch_layout": (_I, [_VP, _I, _VP, _VP]),
     "sw_solve_batch": (_I, [_VP, _VP, _I, _VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "sw_solve_batch_persistent": (_I, [_VP, _VP, _I, _VP, _VP, _I, _I, _I, _I, _I, _VP, ctypes.c_double, _VP]),
     "sw_cnn_fwd": (_I, [_VP, _I, _VP]),
     "sw_cnn_bwd": (_I, [_VP, _I, _VP]),
     "sw_cnn_bwd_scratch_bytes": (_I, [_VP, _VP]),

@@ -875,9 +875,27 @@ class DayBatch:
             self.solver.svf(ds, stream)
             self.solver.tile_static(ds, dict(model=self.model), stream)
 
-    def run(self, stream=None, reset_state=True):
-        """Enqueue the batched day; outputs land in ``self.outs``."""
-        L.call("sw_solve_batch", L.ptr(self.items_dev), self.items_host, len(self.scenes), L.ptr(self.model),
-               self._is_day_c, 0, self.n_steps, self.mask, self.walk_halo, 1 if reset_state else 0,
-               _stream_ptr(stream))
+    def run(self, stream=None, reset_state=True, persistent=False, timeout_s=10.0):
+        """Enqueue the batched day; outputs land in ``self.outs``.
+
+        ``persistent``: the whole day in one persistent launch
+        (``sw_solve_batch_persistent``) instead of one launch per phase and
+        step; the outputs are the same bit for bit. Its barrier workspace is
+        ``self.sync``; :meth:`check` reads its error flag after the stream."""
+        if persistent:
+            if getattr(self, "sync", None) is None:
+                self.sync = torch.zeros(L.DAY_SYNC_WORDS, dtype=torch.int32, device=self.solver.device)
+            L.call("sw_solve_batch_persistent", L.ptr(self.items_dev), self.items_host, len(self.scenes),
+                   L.ptr(self.model), self._is_day_c, 0, self.n_steps, self.mask, self.walk_halo,
+                   1 if reset_state else 0, L.ptr(self.sync), float(timeout_s), _stream_ptr(stream))
+        else:
+            L.call("sw_solve_batch", L.ptr(self.items_dev), self.items_host, len(self.scenes), L.ptr(self.model),
+                   self._is_day_c, 0, self.n_steps, self.mask, self.walk_halo, 1 if reset_state else 0,
+                   _stream_ptr(stream))
         return self.outs
+
+    def check(self):
+        """Raise if the last persistent launch stopped at a barrier time-out (call after its stream)."""
+        sync = getattr(self, "sync", None)
+        if sync is not None and int(sync[L.DAY_SYNC_ERR].item()) != 0:
+            raise RuntimeError("sw_solve_batch_persistent: a grid barrier timed out; the day is incomplete")

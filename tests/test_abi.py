@@ -142,3 +142,28 @@ def test_build_flags_keep_one_rounding_per_operation():
         assert bad not in flags, bad
     make = (Path(__file__).resolve().parents[1] / "oracle" / "Makefile").read_text()
     assert "-ffp-contract=off" in make and "-ffast-math" not in make and "-Ofast" not in make
+
+
+def test_persistent_day_refusals_return_before_any_launch():
+    """sw_solve_batch_persistent refuses a missing barrier workspace, a time-out outside (0, 60] s,
+    an empty output mask and items not laid out by sw_batch_layout before touching the device."""
+    L.load()
+    fake = ctypes.c_void_p(256)
+    items = (L.BatchItemStruct * 1)()
+    items[0].tile.rows, items[0].tile.cols = 16, 16
+    for name in ("hpack", "walls", "aspect_rad", "buildings", "cls", "stats", "masks", "svfbuveg", "tan_asvf",
+                 "tan_svfalfa", "walkface", "src", "state"):
+        setattr(items[0].tile, name, 256)
+    items[0].steps, items[0].sun_tab, items[0].out = 256, 256, 256
+    day = (ctypes.c_int32 * 2)(0, 1)
+    args = lambda sync, mask, timeout: (fake, items, 1, fake, day, 0, 2, mask, 22, 0, sync, timeout, None)
+    with pytest.raises(L.HipError, match="NULL pointer"):
+        L.call("sw_solve_batch_persistent", *args(None, 1, 5.0))
+    for bad in (0.0, -1.0, 61.0, float("nan")):
+        with pytest.raises(L.HipError, match="timeout"):
+            L.call("sw_solve_batch_persistent", *args(fake, 1, bad))
+    with pytest.raises(L.HipError, match="selects nothing"):
+        L.call("sw_solve_batch_persistent", *args(fake, 0, 5.0))
+    items[0].flux_block0 = 3
+    with pytest.raises(L.HipError, match="sw_solve_batch_persistent: items not laid out"):
+        L.call("sw_solve_batch_persistent", *args(fake, 1, 5.0))

@@ -1,5 +1,6 @@
 """Batched day loop (sw_solve_batch): tiles of uneven size and edit scenarios
-in one launch per kernel and step, against single-tile solves of each item.
+in one launch per kernel and step, against single-tile solves of each item;
+the persistent day (sw_solve_batch_persistent) against sw_solve_batch.
 
 Every item's outputs must equal, bit for bit, the single-tile solve of the
 same scene (which the parity tests check against the reference).
@@ -113,6 +114,107 @@ def test_batch_day_graph_replays_match_eager(env):
         if g.value:
             L.call("sw_graph_destroy", g)
         L.call("sw_stream_destroy", s)
+
+
+def _persistent_phases(batch):
+    """Barriers the persistent day passes: two per day step, one per night step."""
+    return 2 * sum(batch.is_day) + (batch.n_steps - sum(batch.is_day))
+
+
+def test_persistent_day_equals_per_phase_launches(env):
+    """sw_solve_batch_persistent (one launch for the day) against sw_solve_batch (one launch per
+    phase and step) on uneven tiles and edit scenarios: every output bit for bit, no barrier
+    time-out, every phase passed."""
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import DayBatch, TileInputs
+    from solweig_project_amd import _lib as L
+    base = synth.make_site(100, 90, seed=45)
+    tiles = [TileInputs.from_site(synth.make_site(r, c, seed=80 + k))
+             for k, (r, c) in enumerate(((96, 80), (64, 130), (150, 50), (33, 47), (17, 3)))]
+    tiles += [TileInputs.from_site(synth.site_with(base, d, c, lc))
+              for d, c, lc, _ in synth.edit_scenarios(base, 3, seed=46)]
+    _say(f"persistent day on {len(tiles)} items")
+    solver = env["solver"]
+    batch = DayBatch(solver, [solver.upload(t) for t in tiles], env["met"], env["sun"], env["lat"], outputs=OUTS)
+    batch.prepare()
+    batch.run()
+    torch.cuda.synchronize()
+    refs = [o.clone() for o in batch.outs]
+    for rep in range(2):
+        for o in batch.outs:
+            o.fill_(float(rep + 7))
+        batch.run(persistent=True)
+        torch.cuda.synchronize()
+        batch.check()
+        assert int(batch.sync[10].item()) == _persistent_phases(batch)
+        for k, (got, ref) in enumerate(zip(batch.outs, refs)):
+            assert _eq(got, ref), (rep, k)
+    assert int(batch.sync[L.DAY_SYNC_ERR].item()) == 0
+
+
+def test_persistent_day_graph_replays_match_eager(env):
+    """The persistent day captured into a graph (a plain launch of the same co-resident grid, after
+    its counter-clearing kernel) replays three times with the per-phase launches' outputs."""
+    from solweig_project_amd import _lib as L
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import DayBatch, TileInputs
+    _say("captured persistent day, 3 replays")
+    solver = env["solver"]
+    tiles = [TileInputs.from_site(synth.make_site(r, c, seed=90 + r)) for r, c in ((72, 64), (40, 100))]
+    batch = DayBatch(solver, [solver.upload(t) for t in tiles], env["met"], env["sun"], env["lat"], outputs=OUTS)
+    batch.prepare()
+    batch.run()
+    torch.cuda.synchronize()
+    refs = [o.clone() for o in batch.outs]
+    s = torch.cuda.Stream("cuda:0")
+    g = ctypes.c_void_p()
+    batch.sync = torch.zeros(L.DAY_SYNC_WORDS, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    try:
+        L.call("sw_graph_begin", s.cuda_stream)
+        batch.run(stream=s, persistent=True)
+        L.call("sw_graph_end", s.cuda_stream, ctypes.byref(g))
+        for rep in range(3):
+            for o in batch.outs:
+                o.fill_(float(rep))
+            torch.cuda.synchronize()
+            L.call("sw_graph_launch", g, s.cuda_stream)
+            s.synchronize()
+            batch.check()
+            for got, ref in zip(batch.outs, refs):
+                assert _eq(got, ref), rep
+    finally:
+        if g.value:
+            L.call("sw_graph_destroy", g)
+
+
+def test_persistent_day_stops_at_a_barrier_time_out(env):
+    """A barrier wait bounded by (almost) zero time: the launch ends, every block leaves at a
+    barrier, and the error flag reports an incomplete day unless every wait happened to be
+    immediate (then the outputs are complete and equal)."""
+    from solweig_project_amd import synth
+    from solweig_project_amd.solver import DayBatch, TileInputs
+    from solweig_project_amd import _lib as L
+    _say("persistent day with a 1 ns barrier limit")
+    solver = env["solver"]
+    tiles = [TileInputs.from_site(synth.make_site(64, 64, seed=97))]
+    batch = DayBatch(solver, [solver.upload(t) for t in tiles], env["met"], env["sun"], env["lat"], outputs=OUTS)
+    batch.prepare()
+    batch.run()
+    torch.cuda.synchronize()
+    ref = batch.outs[0].clone()
+    batch.run(persistent=True, timeout_s=1e-9)
+    torch.cuda.synchronize()
+    if int(batch.sync[L.DAY_SYNC_ERR].item()) != 0:
+        with pytest.raises(RuntimeError, match="timed out"):
+            batch.check()
+        assert int(batch.sync[10].item()) < _persistent_phases(batch)
+    else:
+        assert _eq(batch.outs[0], ref)
+    batch.run(persistent=True)          # the next launch clears the flag
+    torch.cuda.synchronize()
+    batch.check()
+    assert _eq(batch.outs[0], ref)
 
 
 def test_batch_rejects_items_not_laid_out(env):
