@@ -221,6 +221,19 @@ def _t(x):
     return torch.tensor(x, device=_CPU)
 
 
+_CONST = {}
+
+
+def _c(x):
+    """``_t(x)`` for a constant: one tensor per value, created on first use (the chain only reads
+    these, so sharing them changes no result and saves a tensor construction per use)."""
+    key = (type(x), x)
+    t = _CONST.get(key)
+    if t is None:
+        t = _CONST[key] = torch.tensor(x, device=_CPU)
+    return t
+
+
 def daylen_snup(jday: float, lat: float):
     DOY = _t(jday)
     XLAT = _t(lat)
@@ -270,16 +283,16 @@ def clearness_index(zen, jday, Ta, RH, radG, latitude: float, P):
         G = G[2]
     else:
         G = G[3]
-    a2 = _t(17.27)
-    b2 = _t(237.7)
+    a2 = _c(17.27)
+    b2 = _c(237.7)
     Td = (b2 * (((a2 * Ta) / (b2 + Ta)) + torch.log(RH))) / (a2 - (((a2 * Ta) / (b2 + Ta)) + torch.log(RH)))
     Td = (Td * 1.8) + 32
-    u = torch.exp(0.1133 - torch.log(_t(G + 1.0)) + 0.0393 * Td)
+    u = torch.exp(0.1133 - torch.log(_c(G + 1.0)) + 0.0393 * Td)
     Tw = 1 - 0.077 * ((u * m) ** 0.3)
     Tar = 0.935 ** m
     I0 = Itoa * torch.cos(zen) * Trpg * Tw * D * Tar
-    I0 = torch.where(torch.abs(zen) > torch.pi / 2, _t(0.0), I0)
-    I0 = torch.where(torch.isnan(I0), _t(0.0), I0)
+    I0 = torch.where(torch.abs(zen) > torch.pi / 2, _c(0.0), I0)
+    I0 = torch.where(torch.isnan(I0), _c(0.0), I0)
     corr = 0.1473 * torch.log(90 - (zen / torch.pi * 180)) + 0.3454
     CI = radG / I0 + (1 - corr)
     I0et = Itoa * torch.cos(zen) * D
@@ -309,7 +322,7 @@ def diffuse_fraction(radG, altitude, Kt, Ta, RH):
         else:
             radD = radG * (0.426 * Kt - 0.256 * torch.sin(alfa) + 0.00349 * Ta + 0.0734 * RH)
     radI = (radG - radD) / torch.sin(alfa)
-    radI = torch.where(radI < 0, _t(0.0), radI)
+    radI = torch.where(radI < 0, _c(0.0), radI)
     radI = torch.where((altitude < 1) & (radI > radG), radG, radI) if isinstance(altitude, torch.Tensor) \
         else (radG if (altitude < 1 and bool(radI > radG)) else radI)
     radD = torch.where(radD > radG, radG, radD)
@@ -355,7 +368,7 @@ def perez_luminance(zen_deg: float, azimuth_deg: float, radD, radI, jday: float)
     Returns a (153,) float32 tensor normalised to sum 1.
     """
     acoeff, bcoeff, ccoeff, dcoeff, ecoeff, sky = _perez_consts()
-    deg2rad = torch.tensor(np.pi / 180, device=_CPU)
+    deg2rad = _c(np.pi / 180)
     altitude = 90 - zen_deg
     zen = torch.tensor(zen_deg) * deg2rad
     azimuth = torch.tensor(azimuth_deg) * deg2rad
@@ -375,7 +388,7 @@ def perez_luminance(zen_deg: float, azimuth_deg: float, radD, radI, jday: float)
         AirMass = 1 / torch.sin(altitude) + 0.50572 * torch.pow(180 * altitude / torch.pi + 6.07995, -1.6364)
     PerezBrightness = (AirMass * Idh) / I0
     if Idh <= 10:
-        PerezBrightness = _t(0.0)
+        PerezBrightness = _c(0.0)
     bounds = (1.065, 1.230, 1.500, 1.950, 2.800, 4.500, 6.200)
     k = 7
     for j, bnd in enumerate(bounds):
@@ -686,6 +699,11 @@ class DayLoop:
         self.dectime = np.asarray(sun.dectime, dtype=np.float64)
         self.jday_row = np.asarray(sun.jday, dtype=np.float64)
         # the wet-bulb iteration spans every forcing row at once (utci_process.py:817-823)
+        # per-day constants of the chain: read-only tensors shared by the steps
+        self._snup = {}
+        self._lc_t = (torch.tensor(self.lc.tgk), torch.tensor(self.lc.tstart), torch.tensor(self.lc.tmaxlst))
+        self._lcw_t = tuple(torch.tensor([[v]], dtype=torch.float64).float()
+                            for v in (self.lc.tgk_wall, self.lc.tstart_wall, self.lc.tmaxlst_wall))
         self.wet_bulb = wet_bulb_from_rh(self.P.numpy() * 1000.0, (self.Ta + self.uhii).numpy() + 273.15,
                                          self.RH.numpy())
 
@@ -716,8 +734,10 @@ class DayLoop:
         RH = self.RH[i]
         radG = self.radG[i]
         P = self.P[i]
-        SBC = torch.tensor(5.67051e-8)
-        SNUP = daylen_snup(jday, self.lat)
+        SBC = _c(5.67051e-8)
+        SNUP = self._snup.get(jday)
+        if SNUP is None:
+            SNUP = self._snup[jday] = daylen_snup(jday, self.lat)
         ea = 6.107 * 10 ** ((7.5 * Ta) / (237.3 + Ta)) * (RH / 100.0)
         msteg = 46.5 * (ea / (Ta + 273.15))
         esky = (1 - (1 + msteg) * torch.exp(-((1.2 + 3.0 * msteg) ** 0.5))) + 0
@@ -735,7 +755,7 @@ class DayLoop:
         sp.wbgt_a = globe_coefficient(sp.va)
         tak4_64 = (Ta + 273.15) ** 4
         sp.tak4 = float(np.float32(tak4_64.item()))
-        deg2rad_t = torch.tensor(np.pi / 180)
+        deg2rad_t = _c(np.pi / 180)
         geom = self.geom_day if sp.is_day else self.geom_night
         sp.geom = geom
         d = sp.diag
@@ -764,22 +784,18 @@ class DayLoop:
             # ground / wall temperature wave
             frac = (dectime - torch.floor(dectime)) - SNUP / 24
             snup24 = SNUP / 24
-            tgk = torch.tensor(self.lc.tgk)
-            tst = torch.tensor(self.lc.tstart)
-            tmx = torch.tensor(self.lc.tmaxlst)
+            tgk, tst, tmx = self._lc_t
             Tgamp = tgk * altmax + tst
             Tg = _tgwave(Tgamp, tmx, frac, snup24)
             if self.lc.present:
-                tgkw = torch.tensor([[self.lc.tgk_wall]], dtype=torch.float64).float()
-                tstw = torch.tensor([[self.lc.tstart_wall]], dtype=torch.float64).float()
-                tmxw = torch.tensor([[self.lc.tmaxlst_wall]], dtype=torch.float64).float()
+                tgkw, tstw, tmxw = self._lcw_t
                 Tgampwall = tgkw * altmax + tstw
                 Tgwall = _tgwave(Tgampwall, tmxw, frac, snup24)
             else:
                 Tgampwall = self.lc.tgk_wall * altmax + self.lc.tstart_wall
                 Tgwall = Tgampwall * torch.sin((((dectime - torch.floor(dectime)) - SNUP / 24)
                                                 / (self.lc.tmaxlst_wall / 24 - SNUP / 24)) * np.pi / 2)
-            Tgwall = torch.maximum(Tgwall, torch.tensor(0))
+            Tgwall = torch.maximum(Tgwall, _c(0))
             radI0, radD0 = diffuse_fraction(I0, altitude.item(), 1.0, Ta.item(), RH.item())
             corr = 0.1473 * torch.log(90 - (zen / np.pi * 180)) + 0.3454
             CI_Tg = (radG / radI0) + (1 - corr)
@@ -790,10 +806,10 @@ class DayLoop:
             Tg = Tg * CI_TgG
             Tgwall = Tgwall * CI_TgG
             if self.lc.present:
-                Tg = torch.maximum(Tg, torch.tensor(0))
+                Tg = torch.maximum(Tg, _c(0))
             sp.tg_class = Tg.float().numpy().astype(F32).reshape(-1)
             sp.tgwall = float(torch.as_tensor(Tgwall).float().reshape(-1)[0])
-            ewall_t = torch.tensor(EWALL)
+            ewall_t = _c(EWALL)
             lwall = SBC * ewall_t * (Tgwall + Ta + 273.15) ** 4 - SBC * ewall_t * (Ta + 273.15) ** 4
             sp.lwall = float(torch.as_tensor(lwall).float().reshape(-1)[0])
             # thermal delay (TsWaveDelay_2015a, all six planes share it)
@@ -816,7 +832,7 @@ class DayLoop:
             sp.tan_zen = float(torch.tan(torch.tensor(zen.item(), dtype=torch.float32)))
             # Kside scalars (alt/azi as Python floats -> f32 tensors)
             alt_k = torch.tensor(altitude.item())
-            deg2rad_k = torch.tensor(torch.pi / 180.0)
+            deg2rad_k = _c(torch.pi / 180.0)
             sp.cosalt_k = _f(torch.cos(alt_k * deg2rad_k))
             albedo = ALBEDO_B
             sp.sunlit_k = _f((albedo * (radI * torch.cos(alt_k * deg2rad_k)) + (radD * 0.5)) / torch.pi)
@@ -846,8 +862,8 @@ class DayLoop:
             esky = CI * esky + (1 - CI) * 1.0
         st.CI = CI
         sp.diag["esky"] = float(esky)
-        ewall_t = torch.tensor(EWALL)
-        pi_t = torch.tensor(np.pi)
+        ewall_t = _c(EWALL)
+        pi_t = _c(np.pi)
         esky_band = 1 - (1 - esky) * torch.exp(0.308 * geom.band_zterm)
         # per ring: (esky_band * SBC) in float32, then * (Ta + 273.15)^4 / pi in float64;
         # per patch: float32(base) * ster * float32(sin or cos of the ring altitude)

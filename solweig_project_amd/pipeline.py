@@ -94,32 +94,48 @@ def solve_many(solver, ds, jobs, pipeline: PlanPipeline, outputs=("tmrt", "utci"
 
     ``lanes``: a list of (scene, stream, out) triples; job k runs on lane k mod len(lanes), so
     consecutive independent solves overlap on the device (each lane's buffers, plan staging and
-    outputs are its own; ``ds``, ``stream`` and ``out`` are then unused). Returns the last lane's out.
+    outputs are its own; ``ds``, ``stream`` and ``out`` are then unused). The SVF sweeps run one
+    lane ahead of the days: job k + len(lanes) - 1's sweep is enqueued before job k waits for its
+    plan, so the device has sweeps to run while the first plans are computed. Returns the last
+    lane's out.
     """
     import time
     if not jobs:
         return out
-    marks = []
+    lane_of = (lambda k: lanes[k % len(lanes)]) if lanes else (lambda k: (ds, stream, out))
+    depth = len(lanes) - 1 if lanes else 0
     ahead = max(1, int(getattr(pipeline, "workers", 1)))
-    pending = [pipeline.submit(ds.tile, j) for j in jobs[:ahead]]
+    pending = [pipeline.submit(lane_of(k)[0].tile, j) for k, j in enumerate(jobs[:ahead])]
+    sweep_ms = [0.0] * len(jobs)
+
+    def sweep(k):
+        t = time.perf_counter()
+        sc, st, _ = lane_of(k)
+        solver.svf(sc, st)
+        sweep_ms[k] = time.perf_counter() - t
+
+    for k in range(min(depth, len(jobs))):
+        sweep(k)
+    marks = []
     for k, job in enumerate(jobs):
-        if lanes:
-            ds, stream, out = lanes[k % len(lanes)]
-        t0 = time.perf_counter()
-        solver.svf(ds, stream)
-        if k + ahead < len(jobs):
-            pending.append(pipeline.submit(ds.tile, jobs[k + ahead]))
+        sc, st, o = lane_of(k)
+        if not lanes:
+            o = out                      # one lane: each job reuses the previous job's output buffer
+        if k + depth < len(jobs):
+            sweep(k + depth)
         t1 = time.perf_counter()
+        if k + ahead < len(jobs):
+            pending.append(pipeline.submit(sc.tile, jobs[k + ahead]))
         plan = pending.pop(0).get()
         t2 = time.perf_counter()
         if hasattr(solver, "plan_ms") and hasattr(plan, "plan_ms"):
             solver.plan_ms.append(plan.plan_ms)
-        dplan = solver.upload_plan(plan, stream)
+        dplan = solver.upload_plan(plan, st)
         t3 = time.perf_counter()
-        solver.tile_static(ds, dplan, stream)
-        out = solver.run_day(ds, plan, dplan, outputs, out=out, stream=stream)
+        solver.tile_static(sc, dplan, st)
+        out = solver.run_day(sc, plan, dplan, outputs, out=o, stream=st)
         t4 = time.perf_counter()
-        marks.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3))
+        marks.append((sweep_ms[k + depth] if k + depth < len(jobs) else 0.0, t2 - t1, t3 - t2, t4 - t3))
     solver.pipeline_marks = marks
     return out
 
@@ -173,34 +189,50 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
         L.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), src.numel() * src.element_size(), cstream.cuda_stream)
 
     import time
+
+    def sweep(k):
+        """Upload job k's inputs and sweep its SVF on its lane (and copy the fields out unless late)."""
+        t = time.perf_counter()
+        li = k % len(lanes)
+        ds, stream, cstream = lanes[li]
+        own = cstream is None
+        cstream = stream if own else cstream
+        with torch.cuda.stream(stream):
+            if fields_out[li] is not None:
+                stream.wait_event(fields_out[li])
+            for name, tin in host_inputs.items():
+                ds.inputs[name].copy_(tin, non_blocking=True)
+            solver.svf(ds, stream)
+            ev_fields = torch.cuda.Event()
+            ev_fields.record(stream)
+        if not own:
+            cstream.wait_event(ev_fields)
+        if not (fields_last and own):
+            copy_out(host_outs[k].fields, ds.fields, cstream)
+        if not own:
+            fields_out[li] = torch.cuda.Event()
+            fields_out[li].record(cstream)
+        sweep_s[k] = time.perf_counter() - t
+
+    # the sweeps run len(lanes) - 1 jobs ahead of the days: job k + depth's sweep is enqueued (on its
+    # lane, after that lane's previous day) before job k waits for its plan
+    depth = len(lanes) - 1
+    sweep_s = [0.0] * len(jobs)
+    for k in range(min(depth, len(jobs))):
+        sweep(k)
     marks = []
     for k, job in enumerate(jobs):
-        tm = [time.perf_counter()]
         li, bi = k % len(lanes), k % len(dev_outs)
         ds, stream, cstream = lanes[li]
         own = cstream is None
         cstream = stream if own else cstream
         out = dev_outs[bi]
         hb = host_outs[k]
-        with torch.cuda.stream(stream):
-            if fields_out[li] is not None:
-                stream.wait_event(fields_out[li])
-            for name, t in host_inputs.items():
-                ds.inputs[name].copy_(t, non_blocking=True)
-            solver.svf(ds, stream)
-            ev_fields = torch.cuda.Event()
-            ev_fields.record(stream)
-        late = fields_last and own
-        if not own:
-            cstream.wait_event(ev_fields)
-        if not late:
-            copy_out(hb.fields, ds.fields, cstream)
-        if not own:
-            fields_out[li] = torch.cuda.Event()
-            fields_out[li].record(cstream)
+        if k + depth < len(jobs):
+            sweep(k + depth)
+        tm = [time.perf_counter()]
         if k + ahead < len(jobs):
             pending.append(pipeline.submit(ds.tile, jobs[k + ahead]))
-        tm.append(time.perf_counter())
         plan = pending.pop(0).get()
         tm.append(time.perf_counter())
         dplan = solver.upload_plan(plan, stream)
@@ -217,13 +249,13 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
                 ev.record(stream)
                 cstream.wait_event(ev)
             copy_out(hb.out[t0:t1], out[t0:t1], cstream)
-        if late:
+        if fields_last and own:
             copy_out(hb.fields, ds.fields, cstream)
         fin = torch.cuda.Event(enable_timing=True)
         fin.record(cstream)
         buf_free[bi] = fin
         done.append(fin)
         tm.append(time.perf_counter())
-        marks.append([b - a for a, b in zip(tm, tm[1:])])
+        marks.append([sweep_s[k + depth] if k + depth < len(jobs) else 0.0] + [b - a for a, b in zip(tm, tm[1:])])
     solver.pipeline_marks = marks
     return done
