@@ -63,11 +63,12 @@ def parse():
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
     ap.add_argument("--lanes", type=int, default=2,
                     help="tile workload: consecutive solves alternate over this many scene copies on their own streams")
-    ap.add_argument("--h2h-lanes", type=int, default=3,
-                    help="tile workload: lanes of the host-to-host timed region (each copies on its own stream)")
-    ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane"), default="own",
-                    help="tile workload: where the host-to-host copies run: on each lane's stream, on one "
-                         "shared copy stream, or on a copy stream per lane")
+    ap.add_argument("--h2h-lanes", type=int, default=2,
+                    help="tile workload: lanes of the host-to-host timed region")
+    ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane", "thread"), default="thread",
+                    help="tile workload: where the host-to-host copies run: issued by a copy thread on a stream "
+                         "of their own once each chunk is computed (the lanes never wait for a copy), on each "
+                         "lane's stream, on one shared copy stream, or on a copy stream per lane")
     ap.add_argument("--h2h-chunk", type=int, default=8,
                     help="tile workload: day steps per chunk of the host-to-host region (each chunk's planes are "
                          "copied out right after it)")
@@ -760,18 +761,29 @@ def main():
     hbufs = [HostBuffers(torch.empty((16, rows, cols), dtype=torch.float32, pin_memory=True),
                          torch.empty((T, 3, rows, cols), dtype=torch.float32, pin_memory=True))
              for _ in range(n_h2h)]
-    dev_outs = [lanes[0][2]] + [torch.empty_like(lanes[0][2]) for _ in range(2)]   # ring of day outputs
+    threaded = args.h2h_copy == "thread"
+    n_ring = 6 if threaded else 3
+    dev_outs = [lanes[0][2]] + [torch.empty_like(lanes[0][2]) for _ in range(n_ring - 1)]   # ring of day outputs
+    if threaded:
+        from solweig_project_amd.pipeline import solve_many_host_threaded
+        fields_ring = [torch.empty_like(ds.fields) for _ in range(n_ring)]
+        tl = [(sc, st) for sc, st, _ in h2h_lanes]
+
+        def run_h2h(jobs_, outs_):
+            return solve_many_host_threaded(solver, tl, jobs_, pipeline, host_in, outs_, dev_outs, fields_ring,
+                                            chunk_steps=args.h2h_chunk)
+    else:
+        def run_h2h(jobs_, outs_):
+            return solve_many_host(solver, h2h_lanes, jobs_, pipeline, host_in, outs_, dev_outs,
+                                   chunk_steps=args.h2h_chunk, fields_last=args.h2h_fields == "last")
     # warm-up: every lane and every plan staging buffer once (first-use allocations stay out of the timing)
-    n_warm_h2h = max(n_warm, len(h2h_lanes) * solver.STAGING_DEPTH)
+    n_warm_h2h = max(n_warm, len(h2h_lanes) * solver.STAGING_DEPTH, n_ring)
     spare = hbufs[args.steps:]
-    solve_many_host(solver, h2h_lanes, [job] * n_warm_h2h, pipeline, host_in,
-                    [spare[i % n_spare] for i in range(n_warm_h2h)], dev_outs, chunk_steps=args.h2h_chunk,
-                    fields_last=args.h2h_fields == "last")
+    run_h2h([job] * n_warm_h2h, [spare[i % n_spare] for i in range(n_warm_h2h)])
     torch.cuda.synchronize()
     log(f"timing {args.steps} host-to-host solves over {len(h2h_lanes)} lane(s)")
     h0 = time.perf_counter()
-    done = solve_many_host(solver, h2h_lanes, [job] * args.steps, pipeline, host_in, hbufs[:args.steps], dev_outs,
-                           chunk_steps=args.h2h_chunk, fields_last=args.h2h_fields == "last")
+    done = run_h2h([job] * args.steps, hbufs[:args.steps])
     torch.cuda.synchronize()
     h1 = time.perf_counter()
     hm = np.array(solver.pipeline_marks) * 1e3
@@ -939,7 +951,8 @@ def main():
                                f"flux -> Tmrt -> UTCI), 30N DOY 180", "rows": rows, "cols": cols,
                    "time_steps": T, "pixel_size_m": site.pixel_size, "lanes": len(h2h_lanes),
                    "lanes_device_resident": len(lanes), "h2h_chunk_steps": args.h2h_chunk,
-                   "h2h_fields_copy": args.h2h_fields,
+                   "h2h_copies": args.h2h_copy,
+                   "h2h_fields_copy": "after the day" if args.h2h_copy == "thread" else args.h2h_fields,
                    "timed_region": "host-resident inputs (6 rasters, pinned) -> host-resident outputs (16 SVF "
                                    "fields + 24 x {tmrt, utci, shadow}, pinned, one buffer set per solve): "
                                    "upload, SVF sweep, day loop, download; day plans in planner processes",

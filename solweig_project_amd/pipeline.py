@@ -259,3 +259,107 @@ def solve_many_host(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, ho
         marks.append([sweep_s[k + depth] if k + depth < len(jobs) else 0.0] + [b - a for a, b in zip(tm, tm[1:])])
     solver.pipeline_marks = marks
     return done
+
+
+def solve_many_host_threaded(solver, lanes, jobs, pipeline: PlanPipeline, host_inputs, host_outs, dev_outs,
+                             fields_ring, outputs=("tmrt", "utci", "shadow"), chunk_steps: int = 8):
+    """Host-to-host solves with the downloads issued by a copy thread on a stream of their own.
+
+    The lanes' streams never wait for a download: each day chunk writes into device buffer
+    ``dev_outs[k mod len(dev_outs)]`` and each job's SVF fields are copied on the device into
+    ``fields_ring[k mod len(fields_ring)]``; the copy thread waits (on the host) for the event that
+    ends each chunk and then issues its download on the copy stream, which therefore holds nothing
+    but copies. A lane reuses a device buffer only after the download out of it (``len(dev_outs)``
+    jobs back) has completed. ``lanes``: (scene, stream) pairs. Returns each job's completion
+    event (its last download), in job order."""
+    import queue
+    import threading
+    import time
+
+    import torch
+
+    from . import _lib as L
+    dev = lanes[0][1].device
+    cs = torch.cuda.Stream(dev)
+    work = queue.Queue()
+
+    def copier():
+        while True:
+            item = work.get()
+            if item is None:
+                return
+            ready, copies, fin, issued = item
+            ready.synchronize()
+            for dst, src in copies:
+                L.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), src.numel() * src.element_size(),
+                       cs.cuda_stream)
+            fin.record(cs)
+            issued.set()
+
+    th = threading.Thread(target=copier, daemon=True)
+    th.start()
+    ahead = max(1, int(getattr(pipeline, "workers", 1)))
+    pending = [pipeline.submit(lanes[0][0].tile, j) for j in jobs[:ahead]]
+    R = len(dev_outs)
+    last_copy = [None] * R                  # (completion event, issued flag) of the last download per ring slot
+    done = []
+    depth = len(lanes) - 1
+    sweep_s = [0.0] * len(jobs)
+
+    def sweep(k):
+        t = time.perf_counter()
+        ds, stream = lanes[k % len(lanes)]
+        with torch.cuda.stream(stream):
+            for name, tin in host_inputs.items():
+                ds.inputs[name].copy_(tin, non_blocking=True)
+            solver.svf(ds, stream)
+        sweep_s[k] = time.perf_counter() - t
+
+    try:
+        for k in range(min(depth, len(jobs))):
+            sweep(k)
+        marks = []
+        for k, job in enumerate(jobs):
+            ds, stream = lanes[k % len(lanes)]
+            b = k % R
+            out, fr, hb = dev_outs[b], fields_ring[b], host_outs[k]
+            if k + depth < len(jobs):
+                sweep(k + depth)
+            tm = [time.perf_counter()]
+            if k + ahead < len(jobs):
+                pending.append(pipeline.submit(ds.tile, jobs[k + ahead]))
+            plan = pending.pop(0).get()
+            tm.append(time.perf_counter())
+            dplan = solver.upload_plan(plan, stream)
+            tm.append(time.perf_counter())
+            solver.tile_static(ds, dplan, stream)
+            if last_copy[b] is not None:
+                fin, issued = last_copy[b]
+                issued.wait()
+                stream.wait_event(fin)
+            T = plan.n_steps
+            fin = None
+            for t0 in range(0, T, chunk_steps):
+                t1 = min(T, t0 + chunk_steps)
+                solver.run_day(ds, plan, dplan, outputs, out=out, stream=stream, steps=(t0, t1))
+                ready = torch.cuda.Event()
+                ready.record(stream)
+                fin, issued = torch.cuda.Event(enable_timing=True), threading.Event()
+                work.put((ready, [(hb.out[t0:t1], out[t0:t1])], fin, issued))
+            with torch.cuda.stream(stream):
+                fr.copy_(ds.fields, non_blocking=True)
+            ready = torch.cuda.Event()
+            ready.record(stream)
+            fin, issued = torch.cuda.Event(enable_timing=True), threading.Event()
+            work.put((ready, [(hb.fields, fr)], fin, issued))
+            last_copy[b] = (fin, issued)
+            done.append((fin, issued))
+            tm.append(time.perf_counter())
+            marks.append([sweep_s[k + depth] if k + depth < len(jobs) else 0.0] + [b_ - a_ for a_, b_ in zip(tm, tm[1:])])
+        solver.pipeline_marks = marks
+    finally:
+        work.put(None)
+        th.join()
+    for _, issued in done:
+        issued.wait()
+    return [fin for fin, _ in done]

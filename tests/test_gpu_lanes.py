@@ -86,3 +86,40 @@ def test_host_to_host_solves_deliver_what_the_device_computes(fields_last):
         torch.cuda.synchronize()
         assert np.array_equal(hb[k].out.numpy(), alone.cpu().numpy(), equal_nan=True), k
         assert np.array_equal(hb[k].fields.numpy().reshape(16, -1), ds.fields.cpu().numpy()), k
+
+
+def test_threaded_host_to_host_solves_deliver_what_the_device_computes():
+    """The copy-thread pipeline (downloads issued by a host thread on a stream of their own after each
+    chunk's event, device rings of two day-output and two field buffers, so a ring slot is reused
+    while earlier downloads may still run) delivers every job's fields and outputs as the job solved
+    alone computes them, over two lanes."""
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    from solweig_project_amd import sunpos, synth
+    from solweig_project_amd.pipeline import (HostBuffers, Job, pinned_inputs, solve_many,
+                                              solve_many_host_threaded)
+    from solweig_project_amd.solver import DeviceScene, HipSolver, TileInputs, upload_inputs
+    print("[gpu-test] threaded host-to-host lanes", flush=True)
+    site = synth.make_site(150, 170, seed=73)
+    tile = TileInputs.from_site(site)
+    sun = sunpos.day_sun_table(site.latitude, site.longitude, site.utc_offset, 2021, 180)
+    jobs = [Job(synth.make_met(sun.altitude, seed=74 + k), sun, site.latitude) for k in range(6)]
+    solver = HipSolver("cuda:0")
+    dev = solver.device
+    T = jobs[0].met.shape[0]
+    lanes = [(DeviceScene(tile, dev, upload_inputs(tile, dev)), torch.cuda.Stream(dev)) for _ in range(2)]
+    host_in = pinned_inputs(tile)
+    hb = [HostBuffers(torch.full((16, 150, 170), -1.0).pin_memory(), torch.full((T, 3, 150, 170), -1.0).pin_memory())
+          for _ in jobs]
+    dev_outs = [torch.empty((T, 3, 150, 170), device=dev) for _ in range(2)]
+    fields_ring = [torch.empty_like(lanes[0][0].fields) for _ in range(2)]
+    done = solve_many_host_threaded(solver, lanes, jobs, _InlinePlanner(), host_in, hb, dev_outs, fields_ring,
+                                    chunk_steps=7)
+    assert len(done) == len(jobs)
+    torch.cuda.synchronize()
+    for k, job in enumerate(jobs):
+        ds = DeviceScene(tile, dev, upload_inputs(tile, dev))
+        alone = solve_many(solver, ds, [job], _InlinePlanner(), out=torch.empty_like(dev_outs[0]))
+        torch.cuda.synchronize()
+        assert np.array_equal(hb[k].out.numpy(), alone.cpu().numpy(), equal_nan=True), k
+        assert np.array_equal(hb[k].fields.numpy().reshape(16, -1), ds.fields.cpu().numpy()), k
