@@ -282,6 +282,7 @@ def solve_many_host_threaded(solver, lanes, jobs, pipeline: PlanPipeline, host_i
     dev = lanes[0][1].device
     cs = torch.cuda.Stream(dev)
     work = queue.Queue()
+    failure = []                             # the copy thread's exception, re-raised by the caller
 
     def copier():
         while True:
@@ -289,12 +290,22 @@ def solve_many_host_threaded(solver, lanes, jobs, pipeline: PlanPipeline, host_i
             if item is None:
                 return
             ready, copies, fin, issued = item
-            ready.synchronize()
-            for dst, src in copies:
-                L.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), src.numel() * src.element_size(),
-                       cs.cuda_stream)
-            fin.record(cs)
-            issued.set()
+            try:
+                if not failure:
+                    ready.synchronize()
+                    for dst, src in copies:
+                        L.call("sw_copy_to_host", dst.data_ptr(), src.data_ptr(), src.numel() * src.element_size(),
+                               cs.cuda_stream)
+                    fin.record(cs)
+            except BaseException as e:       # noqa: BLE001 - handed to the caller
+                failure.append(e)
+            finally:
+                issued.set()                 # never leave the caller waiting, also after a failure
+
+    def wait_issued(issued):
+        issued.wait()
+        if failure:
+            raise RuntimeError("a host-to-host download failed") from failure[0]
 
     th = threading.Thread(target=copier, daemon=True)
     th.start()
@@ -335,20 +346,20 @@ def solve_many_host_threaded(solver, lanes, jobs, pipeline: PlanPipeline, host_i
             solver.tile_static(ds, dplan, stream)
             if last_copy[b] is not None:
                 fin, issued = last_copy[b]
-                issued.wait()
+                wait_issued(issued)
                 stream.wait_event(fin)
             T = plan.n_steps
             fin = None
             for t0 in range(0, T, chunk_steps):
                 t1 = min(T, t0 + chunk_steps)
                 solver.run_day(ds, plan, dplan, outputs, out=out, stream=stream, steps=(t0, t1))
-                ready = torch.cuda.Event()
+                ready = torch.cuda.Event(blocking=True)   # the copy thread sleeps, not spins, on it
                 ready.record(stream)
                 fin, issued = torch.cuda.Event(enable_timing=True), threading.Event()
                 work.put((ready, [(hb.out[t0:t1], out[t0:t1])], fin, issued))
             with torch.cuda.stream(stream):
                 fr.copy_(ds.fields, non_blocking=True)
-            ready = torch.cuda.Event()
+            ready = torch.cuda.Event(blocking=True)   # the copy thread sleeps, not spins, on it
             ready.record(stream)
             fin, issued = torch.cuda.Event(enable_timing=True), threading.Event()
             work.put((ready, [(hb.fields, fr)], fin, issued))
@@ -361,5 +372,5 @@ def solve_many_host_threaded(solver, lanes, jobs, pipeline: PlanPipeline, host_i
         work.put(None)
         th.join()
     for _, issued in done:
-        issued.wait()
+        wait_issued(issued)
     return [fin for fin, _ in done]
