@@ -27,6 +27,12 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# Hardware queues per process (read when the HIP runtime starts, so before torch): the runtime puts
+# streams on its queues round robin, and with the default 4, two of the host-to-host timed region's
+# five streams (null, three lanes, the copy stream) would share a queue, so one lane would wait
+# behind the other's work or behind the downloads (11.2-12.7 against 10.3 ms per solve with 8).
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SW_BENCH_HW_QUEUES", "8")
+
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0      # dense BF16 MFMA
@@ -63,7 +69,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
     ap.add_argument("--lanes", type=int, default=2,
                     help="tile workload: consecutive solves alternate over this many scene copies on their own streams")
-    ap.add_argument("--h2h-lanes", type=int, default=2,
+    ap.add_argument("--h2h-lanes", type=int, default=3,
                     help="tile workload: lanes of the host-to-host timed region")
     ap.add_argument("--h2h-copy", choices=("own", "shared", "per-lane", "thread"), default="thread",
                     help="tile workload: where the host-to-host copies run: issued by a copy thread on a stream "
@@ -767,11 +773,20 @@ def main():
     if threaded:
         from solweig_project_amd.pipeline import solve_many_host_threaded
         fields_ring = [torch.empty_like(ds.fields) for _ in range(n_ring)]
-        tl = [(sc, st) for sc, st, _ in h2h_lanes]
+        # the lanes' streams and the copy stream created one after another: the runtime puts
+        # consecutive streams on different hardware queues (a copy stream sharing a lane's queue
+        # would serialise that lane behind the downloads)
+        h2h_streams = [torch.cuda.Stream(dev) for _ in range(len(h2h_lanes) + 1)]
+        for st in h2h_streams:               # first use in creation order
+            with torch.cuda.stream(st):
+                torch.zeros(1, device=dev)
+        torch.cuda.synchronize()
+        tl = [(sc, st) for (sc, _, _), st in zip(h2h_lanes, h2h_streams)]
+        copy_stream = h2h_streams[-1]
 
         def run_h2h(jobs_, outs_):
             return solve_many_host_threaded(solver, tl, jobs_, pipeline, host_in, outs_, dev_outs, fields_ring,
-                                            chunk_steps=args.h2h_chunk)
+                                            chunk_steps=args.h2h_chunk, copy_stream=copy_stream)
     else:
         def run_h2h(jobs_, outs_):
             return solve_many_host(solver, h2h_lanes, jobs_, pipeline, host_in, outs_, dev_outs,

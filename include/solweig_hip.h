@@ -160,6 +160,11 @@ int sw_copy2d_async(void *dst, int64_t dpitch, const void *src, int64_t spitch, 
  * (sw_host_alloc). No counterpart in the reference (its residency ABI copies with
  * cudaMemcpyAsync, sw_residency.cu:53-331). */
 int sw_copy_to_host(void *dst, const void *src, int64_t bytes, void *stream);
+/* Upload from pinned host memory by a copy kernel (the compute units read the host buffer), so an
+ * upload never queues behind downloads on the DMA engines. Both pointers 16-byte aligned; a source
+ * that is not pinned host memory is refused (SW_E_ARG). No counterpart in the reference (its
+ * uploads are torch/cudaMemcpy copies). */
+int sw_copy_to_device_cu(void *dst, const void *src, int64_t bytes, void *stream);
 int sw_fill_f32(float *dst, float value, int64_t n, void *stream);
 int sw_stream_create(void **stream);
 int sw_stream_destroy(void *stream);

@@ -169,6 +169,7 @@ SIGNATURES = {
     "sw_stream_create": (_I, [_VP]),
     "sw_stream_destroy": (_I, [_VP]),
     "sw_copy_to_host": (_I, [_VP, _VP, _I64, _VP]),
+    "sw_copy_to_device_cu": (_I, [_VP, _VP, _I64, _VP]),
     "sw_event_create": (_I, [_VP]),
     "sw_event_destroy": (_I, [_VP]),
     "sw_event_record": (_I, [_VP, _VP]),

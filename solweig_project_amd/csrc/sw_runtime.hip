@@ -135,6 +135,41 @@ int sw_copy_to_host(void *dst, const void *src, int64_t bytes, void *stream) {
                          "sw_copy_to_host: hipMemcpyAsync");
 }
 
+// Upload from pinned host memory by the compute units: each thread reads 16-byte words of the host
+// buffer over PCIe (device-addressable pinned memory) and writes them to HBM; the last bytes of a
+// size that is not a multiple of 16 are copied by the first threads one byte each.
+__global__ void k_copy_from_host(uint4 *__restrict__ dst, const uint4 *__restrict__ src, long n16,
+                                 unsigned char *__restrict__ dtail, const unsigned char *__restrict__ stail,
+                                 int ntail) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntail) dtail[t] = stail[t];
+}
+
+int sw_copy_to_device_cu(void *dst, const void *src, int64_t bytes, void *stream) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src)))
+        return swi_set_err(SW_E_ARG, "sw_copy_to_device_cu: invalid arguments");
+    if (bytes == 0) return SW_OK;
+    if ((((uintptr_t)dst) & 15u) || (((uintptr_t)src) & 15u))
+        return swi_set_err(SW_E_ARG, "sw_copy_to_device_cu: source and destination must be 16-byte aligned");
+    // the kernel dereferences src on the device: only pinned (device-addressable) host memory qualifies
+    hipPointerAttribute_t pa;
+    if (hipPointerGetAttributes(&pa, src) != hipSuccess || pa.type != hipMemoryTypeHost) {
+        (void)hipGetLastError();
+        return swi_set_err(SW_E_ARG, "sw_copy_to_device_cu: the source is not pinned host memory");
+    }
+    const long n16 = (long)(bytes / 16);
+    const int ntail = (int)(bytes - n16 * 16);
+    long blocks = (n16 + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy_from_host, dim3((unsigned)blocks), dim3(256), 0, swi_stream(stream), (uint4 *)dst,
+                       (const uint4 *)src, n16, (unsigned char *)dst + n16 * 16,
+                       (const unsigned char *)src + n16 * 16, ntail);
+    return swi_hip_check(hipGetLastError(), "k_copy_from_host");
+}
+
 int sw_fill_f32(float *dst, float value, int64_t n, void *stream) {
     if (n < 0 || (n > 0 && !dst)) return swi_set_err(SW_E_ARG, "sw_fill_f32: invalid arguments");
     if (n == 0) return SW_OK;

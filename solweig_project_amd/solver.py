@@ -664,9 +664,10 @@ class HipSolver:
         host = st["host"].numpy()
         for p, o in zip(parts, offs):
             host[o:o + p.nbytes] = p
-        with torch.cuda.stream(s):
-            st["dev"][:total].copy_(st["host"][:total], non_blocking=True)
-            st["event"].record(s)
+        # the upload runs on the compute units (a copy kernel reading the pinned staging buffer), so it
+        # never queues behind output downloads on the DMA engines
+        L.call("sw_copy_to_device_cu", st["dev"].data_ptr(), st["host"].data_ptr(), total, s.cuda_stream)
+        st["event"].record(s)
         d = st["dev"]
         return dict(steps=d[offs[0]:offs[0] + sizes[0]], sun_tab=d[offs[1]:offs[1] + sizes[1]],
                     model=d[offs[2]:offs[2] + sizes[2]])

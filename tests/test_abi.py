@@ -8,6 +8,7 @@ import re
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from solweig_project_amd import _lib as L
@@ -167,3 +168,15 @@ def test_persistent_day_refusals_return_before_any_launch():
     items[0].flux_block0 = 3
     with pytest.raises(L.HipError, match="sw_solve_batch_persistent: items not laid out"):
         L.call("sw_solve_batch_persistent", *args(fake, 1, 5.0))
+
+
+def test_device_upload_refusals():
+    """sw_copy_to_device_cu refuses unaligned pointers and a source that is not pinned host memory before
+    any launch (its kernel dereferences the source on the device)."""
+    L.load()
+    buf = np.zeros(64, np.uint8)
+    with pytest.raises(L.HipError, match="16-byte aligned"):
+        L.call("sw_copy_to_device_cu", ctypes.c_void_p(256 + 4), ctypes.c_void_p(512), 64, None)
+    src = buf.ctypes.data + (-buf.ctypes.data) % 16
+    with pytest.raises(L.HipError, match="not pinned host memory"):
+        L.call("sw_copy_to_device_cu", ctypes.c_void_p(4096), ctypes.c_void_p(src), 16, None)
