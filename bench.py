@@ -67,7 +67,7 @@ def parse():
                          "re-solves of a 2000x2000 site at --rate requests/s")
     ap.add_argument("--requests", type=int, default=1000, help="requests served by --workload serve")
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --workload serve (per s)")
-    ap.add_argument("--lanes", type=int, default=2,
+    ap.add_argument("--lanes", type=int, default=3,
                     help="tile workload: consecutive solves alternate over this many scene copies on their own streams")
     ap.add_argument("--h2h-lanes", type=int, default=3,
                     help="tile workload: lanes of the host-to-host timed region")
