@@ -297,6 +297,17 @@ __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, ui
                                                    int count, float hmax, __amdgpu_buffer_rsrc_t crs,
                                                    int pr, int pc, int rows, int cols, int Hc, int Wc) {
     static_assert(CHUNK + 1 <= SW_SVF_PAD, "march chunk reads must stay inside the pad");
+    // After the first step the state of the wave's 64 lanes lives in three lane masks (SGPRs): SH (sh),
+    // VS (vegsh), VBB (vbshvegsh sum > 0), plus A (lanes still marching). A step's compares are ballots
+    // and its update is five scalar bit operations for the whole wave:
+    //   SH |= c1; VS = (VS | (c2 & ~c3)) & ~SH; VBB |= VS.
+    // The update may run on every lane: a lane that reads no cell this chunk (it skips the chunk or has
+    // stopped) has c1 = c2 = c3 = 0 and gets VS &= ~SH, VBB |= VS, which is exactly a skipped chunk's
+    // effect, and a stopped lane is unchanged by it (after any update VS implies !SH and VBB covers VS;
+    // a lane stopped at its first step has VS = 0). Lanes that read no cell point their loads past the
+    // buffer (the range check returns zeros without a memory access); on this path heights and dz are
+    // >= 0 and so is the target's height, so a zero source compares false.
+    constexpr uint32_t kNoRead = 0xF0000000u;
     SvfLane st;
     st.sh = false; st.vs = false; st.vbb = false;
     bool active = valid && count > 0;
@@ -306,62 +317,79 @@ __device__ __forceinline__ uint32_t march_svf_fast(__amdgpu_buffer_rsrc_t rs, ui
         svf_first(st, at, veg2t, h.x, h.y, h.z, __int_as_float(e.z));
         active = !(st.sh & !st.vs);
     }
+    uint64_t SH = __builtin_amdgcn_ballot_w64(st.sh), VS = __builtin_amdgcn_ballot_w64(st.vs), VBB = 0;
+    uint64_t A = __builtin_amdgcn_ballot_w64(active);
     int s = 1;
     for (; s + CHUNK <= count; s += CHUNK) {
-        if (!wave_any(active)) break;
+        if (A == 0) break;
+        const int4 *__restrict__ Tc = T + s;
         int4 e[CHUNK];
 #pragma unroll
-        for (int k = 0; k < CHUNK; ++k) e[k] = T[s + k];
+        for (int k = 0; k < CHUNK; ++k) e[k] = Tc[k];
         // the chunk's offset bounding box and smallest dz, the same for every lane (the tables move
         // monotonically in x and y with non-decreasing dz, so the first and last steps bound them)
         const int xl = min(e[0].x, e[CHUNK - 1].x), xh = max(e[0].x, e[CHUNK - 1].x);
         const int yl = min(e[0].y, e[CHUNK - 1].y), yh = max(e[0].y, e[CHUNK - 1].y);
         const float dzl = __int_as_float(e[0].z), dl = __int_as_float(e[CHUNK - 1].z);
         bool skip = false;
-        if (active) {
+        const int R0 = (pr + xl) >> 3, R1 = (pr + xh) >> 3, C0 = (pc + yl) >> 3, C1 = (pc + yh) >> 3;
+        if (__builtin_amdgcn_inverse_ballot_w64(A) && R1 - R0 <= 1 && C1 - C0 <= 1) {
             // coarse cells past the grid read 0 (range-checked loads), which the steps they cover
-            // would read as well
-            const int R0 = (pr + xl) >> 3, R1 = (pr + xh) >> 3, C0 = (pc + yl) >> 3, C1 = (pc + yh) >> 3;
-            if (R1 - R0 <= 1 && C1 - C0 <= 1) {
-                const uint32_t o00 = (uint32_t)(R0 * Wc + C0) * 4u, o01 = (uint32_t)(R0 * Wc + C1) * 4u;
-                const uint32_t o10 = (uint32_t)(R1 * Wc + C0) * 4u, o11 = (uint32_t)(R1 * Wc + C1) * 4u;
-                const bool inb = R0 >= 0 && C0 >= 0 && R1 < Hc && C1 < Wc;
-                const float m = fmaxf(fmaxf(ldc(crs, o00), ldc(crs, o01)), fmaxf(ldc(crs, o10), ldc(crs, o11)));
-                skip = inb && (m - dzl <= at);
-            }
+            // would read as well (row offsets by 24-bit products: |R| and Wc stay far below 2^23)
+            const int q0 = __mul24(R0, Wc), q1 = __mul24(R1, Wc);
+            const uint32_t o00 = (uint32_t)(q0 + C0) * 4u, o01 = (uint32_t)(q0 + C1) * 4u;
+            const uint32_t o10 = (uint32_t)(q1 + C0) * 4u, o11 = (uint32_t)(q1 + C1) * 4u;
+            const bool inb = R0 >= 0 && C0 >= 0 && R1 < Hc && C1 < Wc;
+            const float m = fmaxf(fmaxf(ldc(crs, o00), ldc(crs, o01)), fmaxf(ldc(crs, o10), ldc(crs, o11)));
+            skip = inb && (m - dzl <= at);
         }
-        // uniform branches: a wave whose active lanes all skip issues no gather and no step update
-        const bool need = active && !skip;
-        if (wave_any(need)) {
-            if (need) {
-                H3 h[CHUNK];
+        const uint64_t N = A & ~__builtin_amdgcn_ballot_w64(skip);
+        if (N != 0) {
+            // lanes that skip or have stopped read past the buffer
+            const uint32_t base = __builtin_amdgcn_inverse_ballot_w64(N) ? tpb : kNoRead;
+            H3 h[CHUNK];
 #pragma unroll
-                for (int k = 0; k < CHUNK; ++k) h[k] = ld3(rs, tpb + (uint32_t)e[k].w);
+            for (int k = 0; k < CHUNK; ++k) h[k] = ld3(rs, base + (uint32_t)e[k].w);
 #pragma unroll
-                for (int k = 0; k < CHUNK; ++k) svf_next(st, at, h[k].x, h[k].y, h[k].z, __int_as_float(e[k].z));
-                active = !((st.sh & !st.vs) | outside(h[CHUNK - 1]) | (hmax - dl <= at));
+            for (int k = 0; k < CHUNK; ++k) {
+                const float dz = __int_as_float(e[k].z);
+                const uint64_t c1 = __builtin_amdgcn_ballot_w64(h[k].x - dz > at);
+                const uint64_t c2 = __builtin_amdgcn_ballot_w64(h[k].y - dz > at);
+                const uint64_t c3 = __builtin_amdgcn_ballot_w64(h[k].z - dz > at);
+                SH |= c1;
+                VS = (VS | (c2 & ~c3)) & ~SH;
+                VBB |= VS;
             }
+        } else {
+            VS &= ~SH;
+            VBB |= VS;
         }
-        if (wave_any(active && skip)) {
-            if (active && skip) {
-                st.vs = st.vs & !st.sh;
-                st.vbb = st.vbb | st.vs;
-                const int lr = pr + e[CHUNK - 1].x - SW_SVF_PAD, lc = pc + e[CHUNK - 1].y - SW_SVF_PAD;
-                const bool out = ((unsigned)lr >= (unsigned)rows) | ((unsigned)lc >= (unsigned)cols);
-                active = !((st.sh & !st.vs) | out | (hmax - dl <= at));
-            }
-        }
+        // exits on the chunk's last step (within the pad, an out-of-grid cell is one past the edges)
+        const int lr = pr + e[CHUNK - 1].x - SW_SVF_PAD, lc = pc + e[CHUNK - 1].y - SW_SVF_PAD;
+        const uint64_t out = __builtin_amdgcn_ballot_w64(((unsigned)lr >= (unsigned)rows) |
+                                                         ((unsigned)lc >= (unsigned)cols));
+        const uint64_t low = __builtin_amdgcn_ballot_w64(hmax - dl <= at);
+        A &= ~((SH & ~VS) | out | low);
     }
     for (; s < count; ++s) {
-        if (!wave_any(active)) break;
+        if (A == 0) break;
         const int4 e = T[s];
-        if (active) {
-            const H3 h = ld3(rs, tpb + (uint32_t)e.w);
-            const float dz = __int_as_float(e.z);
-            svf_next(st, at, h.x, h.y, h.z, dz);
-            active = !((st.sh & !st.vs) | outside(h) | (hmax - dz <= at));
-        }
+        const uint32_t base = __builtin_amdgcn_inverse_ballot_w64(A) ? tpb : kNoRead;
+        const H3 h = ld3(rs, base + (uint32_t)e.w);
+        const float dz = __int_as_float(e.z);
+        const uint64_t c1 = __builtin_amdgcn_ballot_w64(h.x - dz > at);
+        const uint64_t c2 = __builtin_amdgcn_ballot_w64(h.y - dz > at);
+        const uint64_t c3 = __builtin_amdgcn_ballot_w64(h.z - dz > at);
+        SH |= c1;
+        VS = (VS | (c2 & ~c3)) & ~SH;
+        VBB |= VS;
+        const uint64_t out = __builtin_amdgcn_ballot_w64(outside(h));
+        const uint64_t low = __builtin_amdgcn_ballot_w64(hmax - dz <= at);
+        A &= ~((SH & ~VS) | out | low);
     }
+    st.sh = __builtin_amdgcn_inverse_ballot_w64(SH);
+    st.vs = __builtin_amdgcn_inverse_ballot_w64(VS);
+    st.vbb = __builtin_amdgcn_inverse_ballot_w64(VBB);
     return svf_out(st);
 }
 
