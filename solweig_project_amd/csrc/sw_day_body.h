@@ -81,6 +81,8 @@ __device__ __forceinline__ float funkey(uint32_t k) {
     return (k & 0x80000000u) ? __uint_as_float(k & 0x7FFFFFFFu) : __uint_as_float(~k);
 }
 
+typedef unsigned int sw_u3 __attribute__((ext_vector_type(3)));
+
 // sun march core shared by the single-step ABI and the day step kernel.
 struct SunOut {
     int sh, vs, vb;
@@ -137,6 +139,57 @@ __device__ __forceinline__ SunOut march_sun_pixel(const float4 *__restrict__ hp,
     SunOut o;
     o.sh = sh; o.vs = vs; o.vb = vb; o.shvo = f - at; o.shvoveg = svv; o.stop = stop;
     return o;
+}
+
+// The same march with the wave's vegsh and marching lanes in lane masks (SGPRs), for waves whose
+// every lane has a >= 0 and vegetation >= 0 and steps with dz >= 0 (then every lane takes the exits,
+// and an out-of-grid source is the lane's last step). f, the vegetation running maximum and vb stay
+// per lane. The running maxima and compares run on every lane: a lane that reads no cell (stopped,
+// invalid or past the grid) points its load past the buffer and gets sources 0 - dz <= 0, which
+// leave f >= a >= 0 and the vegetation maximum >= 0 unchanged and compare false, exactly as the
+// reference's zero source does for a lane at its out-of-grid step; vb counts only marching lanes.
+// The vegsh update VS = (VS | V2) & ~SH leaves a stopped lane unchanged (after every step vegsh
+// implies !sh).
+template <bool TRACK>
+__device__ __forceinline__ SunOut march_sun_masks(__amdgpu_buffer_rsrc_t hrs, uint32_t tbytes, int r, int c,
+                                                  int rows, int cols, float at, float vegt, bool valid, bool wall,
+                                                  const int4 *__restrict__ tab, int count, float amax, float hmax) {
+    constexpr uint32_t kNoRead = 0xF0000000u;
+    float f = at, svv = vegt;
+    int vb = 0, stop = count;
+    uint64_t VS = 0, A = __builtin_amdgcn_ballot_w64(valid);
+    const uint64_t WALL = __builtin_amdgcn_ballot_w64(wall);
+    for (int s = 0; s < count; ++s) {
+        if (A == 0) break;
+        const int4 e = tab[s];
+        const float dzs = __int_as_float(e.z), dzp = __int_as_float(e.w);
+        const uint64_t INB = __builtin_amdgcn_ballot_w64((unsigned)(r + e.x) < (unsigned)rows) &
+                             __builtin_amdgcn_ballot_w64((unsigned)(c + e.y) < (unsigned)cols);
+        const uint32_t off = tbytes + (uint32_t)(e.x * cols + e.y) * 16u;
+        const uint32_t o = __builtin_amdgcn_inverse_ballot_w64(A & INB) ? off : kNoRead;
+        const sw_u3 v = __builtin_amdgcn_raw_buffer_load_b96(hrs, o, 0, 0);
+        const float hx = __uint_as_float(v.x), hy = __uint_as_float(v.y), hz = __uint_as_float(v.z);
+        const float tmp = hx - dzs, tv = hy - dzs, tv2 = hz - dzs, lf = hy - dzp, lg = hz - dzp;
+        f = fmaxf(f, tmp);
+        svv = fmaxf(svv, tv);
+        const uint64_t SH = __builtin_amdgcn_ballot_w64(f > at);
+        const uint64_t c1 = __builtin_amdgcn_ballot_w64(tv > at), c2 = __builtin_amdgcn_ballot_w64(tv2 > at);
+        const uint64_t c3 = __builtin_amdgcn_ballot_w64(lf > at), c4 = __builtin_amdgcn_ballot_w64(lg > at);
+        // v2 = 1 when one to three of the four conditions hold (four count as none)
+        const uint64_t V2 = (c1 | c2 | c3 | c4) & ~(c1 & c2 & c3 & c4);
+        VS = (VS | V2) & ~SH;
+        vb += __builtin_amdgcn_inverse_ballot_w64(VS & A) ? 1 : 0;
+        const uint64_t low = __builtin_amdgcn_ballot_w64(hmax - dzp <= at);
+        const uint64_t stopped = A & (~INB | (~WALL & ((SH & ~VS) | low)));
+        if (TRACK) stop = __builtin_amdgcn_inverse_ballot_w64(stopped) ? s : stop;
+        A &= ~stopped;
+        if (!(amax >= dzs)) break;
+    }
+    SunOut out;
+    out.sh = (f > at) ? 1 : 0;
+    out.vs = __builtin_amdgcn_inverse_ballot_w64(VS) ? 1 : 0;
+    out.vb = vb; out.shvo = f - at; out.shvoveg = svv; out.stop = stop;
+    return out;
 }
 
 // The vegetation shadow volume over the steps a lane skipped after its exit (steps stop + 1 ..
@@ -234,8 +287,21 @@ __device__ __forceinline__ void sun_step_body(const float4 *__restrict__ hp, con
         vegt = h.y;
         w = walls[t];
     }
-    const SunOut o = march_sun_pixel<true, PLANES>(hp, rows, cols, r, c, at, vegt, valid, w > 0.f,
-                                                   suntab + sp->sun_off, sp->sun_count, amax, hmax);
+    // waves whose lanes all take the exits (no negative height, dz >= 0) march in lane masks
+    const int4 *__restrict__ stab = suntab + sp->sun_off;
+    const int scount = sp->sun_count;
+    const bool masks = (scount == 0 || __int_as_float(stab[scount - 1].z) >= 0.f) &&
+                       __builtin_amdgcn_ballot_w64(valid && !(at >= 0.f && vegt >= 0.f)) == 0;
+    SunOut o;
+    if (masks) {
+        const __amdgpu_buffer_rsrc_t hrs = __builtin_amdgcn_make_buffer_rsrc((void *)hp, (short)0,
+                                                                             (int)(n * 16), 0x00020000);
+        o = march_sun_masks<PLANES>(hrs, valid ? (uint32_t)t * 16u : 0u, r, c, rows, cols, at, vegt, valid,
+                                    w > 0.f, stab, scount, amax, hmax);
+    } else {
+        o = march_sun_pixel<true, PLANES>(hp, rows, cols, r, c, at, vegt, valid, w > 0.f, stab, scount, amax,
+                                          hmax);
+    }
     if (!valid) return;
     // finalisation (solweig.py:1198-1219)
     const int shv = 1 - o.sh;
@@ -569,13 +635,15 @@ __device__ __forceinline__ void walk_azimuth(const uint8_t *__restrict__ s_src, 
         acc.fs += f;                                                                         \
     }
 #define SW_WALK_GROUP(S)                                                                     \
-    {                                                                                        \
+    if (f != 0.f) {                                                                          \
         const uint32_t e0 = s_src[lbase + wdx[S] * lw + wdy[S]];                             \
         const uint32_t e1 = s_src[lbase + wdx[S + 1] * lw + wdy[S + 1]];                     \
         const uint32_t e2 = s_src[lbase + wdx[S + 2] * lw + wdy[S + 2]];                     \
         const uint32_t e3 = s_src[lbase + wdx[S + 3] * lw + wdy[S + 3]];                     \
         const float4 d0 = s_dec[e0], d1 = s_dec[e1], d2 = s_dec[e2], d3 = s_dec[e3];          \
         SW_WALK_ACC(e0, d0) SW_WALK_ACC(e1, d1) SW_WALK_ACC(e2, d2) SW_WALK_ACC(e3, d3)      \
+    } else {                                                                                 \
+        acc.kb = __builtin_fmaf(bub, 4.f, acc.kb);                                           \
     }
         for (; s + 4 <= nfirst; s += 4) SW_WALK_GROUP(s)
         for (; s < nfirst; ++s) SW_WALK_STEP(s)
