@@ -571,7 +571,8 @@ __device__ __forceinline__ void patch_pass_a(const SW_CONST sw_step_t *__restric
 // Ground-view walk of one azimuth (sunonsurface_2018a, solweig.py:136-288).
 // Every boolean of the walk is a float 0 or 1, as in the reference: the
 // running minimum of building flags is a product f, the "wall seen" flag is
-// a maximum, and "wall or building seen" (bubw) equals 1 - f because f never
+// a maximum (of 0/1 values: max(bub, tw * f) = min(1, bub + tw * f), one fma
+// whose result the median clamps to [0, 1]), and "wall or building seen" (bubw) equals 1 - f because f never
 // rises again, so its count is the step count minus the sum of f. Sums that add a constant (walls seen, their longwave and albedo
 // terms) are kept as counts k and read from the sequential-sum tables
 // s_tlw[k] / s_tal[k] afterwards, which is the same float32 value the
@@ -612,7 +613,7 @@ __device__ __forceinline__ void walk_azimuth(const uint8_t *__restrict__ s_src, 
         acc.wlupsh = __builtin_fmaf(f, tl, acc.wlupsh);                                      \
         acc.walbsh = __builtin_fmaf(f, ta, acc.walbsh);                                      \
         acc.walbnosh = __builtin_fmaf(f, tn, acc.walbnosh);                                  \
-        bub = fmaxf(bub, tw * f);                                                            \
+        bub = __builtin_amdgcn_fmed3f(__builtin_fmaf(tw, f, bub), 0.f, 1.f);                 \
         acc.kb += bub;                                                                       \
         acc.fs += f;                                                                         \
     }
@@ -630,7 +631,7 @@ __device__ __forceinline__ void walk_azimuth(const uint8_t *__restrict__ s_src, 
         acc.wlupsh = __builtin_fmaf(f, D.x, acc.wlupsh);                                     \
         acc.walbsh = __builtin_fmaf(f, D.y, acc.walbsh);                                     \
         acc.walbnosh = __builtin_fmaf(f, fabsf(D.z), acc.walbnosh);                          \
-        bub = fmaxf(bub, D.w * f);                                                           \
+        bub = __builtin_amdgcn_fmed3f(__builtin_fmaf(D.w, f, bub), 0.f, 1.f);                \
         acc.kb += bub;                                                                       \
         acc.fs += f;                                                                         \
     }
