@@ -494,6 +494,30 @@ __device__ __forceinline__ void diffuse_table(const SW_CONST sw_step_t *__restri
     }
 }
 
+// The 153 patches are 8 altitude bands of consecutive patches (31, 30, 28, 24, 19, 13, 7 and 1
+// patches at 6, 18, ..., 90 degrees; the reference's patch_option 2 vault). Steradian, cos_alt and
+// sin_alt are the same for every patch of a band, so every per-patch constant of a flag-only sum
+// (lsky_*, lv_*, k_sh, lh_*, refl * ster * cos | sin) is the same within a band too: a sum
+// sum_p flag_p * c_p is sum_b n_b * c_b, where n_b counts the set flags of band b (an exact small
+// integer). That is 8 fma per sum instead of 153. The reference adds the patches one at a time
+// (solweig.py:1514-1611 and 1615-1625), so the two differ by float32 rounding of the sum only.
+#define SW_NBAND 8
+__device__ constexpr int kBandStart[SW_NBAND + 1] = {0, 31, 61, 89, 113, 132, 145, 152, SW_NPATCH};
+
+// set flags of band b in a 153-bit patch flag (bit p % 32 of word p / 32; bits past patch 152 ignored)
+__device__ __forceinline__ float band_count(const uint32_t *f, int b) {
+    uint32_t c = 0u;
+#pragma unroll
+    for (int w = 0; w < SW_MASK_WORDS; ++w) {
+        const int lo = max(kBandStart[b] - 32 * w, 0), hi = min(kBandStart[b + 1] - 32 * w, 32);
+        if (lo < hi) {
+            const uint32_t m = (hi - lo == 32) ? 0xffffffffu : (((1u << (hi - lo)) - 1u) << lo);
+            c += (uint32_t)__builtin_popcount(f[w] & m);
+        }
+    }
+    return (float)c;
+}
+
 template <bool DAY, bool ROLLED>
 __device__ __forceinline__ void patch_pass_a(const SW_CONST sw_step_t *__restrict__ sp, const uint32_t *msh,
                                              const uint32_t *mvs, const uint32_t *mv1, const uint32_t *mv2,
@@ -501,66 +525,81 @@ __device__ __forceinline__ void patch_pass_a(const SW_CONST sw_step_t *__restric
     float aniLum = 0.f, KsideD = 0.f, Kref_sun = 0.f, Kref_sh = 0.f, Kref_veg = 0.f;
     float Ldown_sky = 0.f, Ldown_veg = 0.f, Ldown_sun = 0.f, Ldown_sh = 0.f;
     float Lside_sky = 0.f, Lside_veg = 0.f, Lside_sun = 0.f, Lside_sh = 0.f;
+    // per-patch booleans as 153-bit flags: tvs = vegsh == 0 or vbsh == 0; tsky = sh and vegsh;
+    // tsh = (1 - sh) * vbsh == 1
+    uint32_t ftvs[SW_MASK_WORDS], ftsky[SW_MASK_WORDS], ftsh[SW_MASK_WORDS];
 #pragma unroll
     for (int w = 0; w < SW_MASK_WORDS; ++w) {
-        const uint32_t ash = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
-        const uint32_t avs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
-        const uint32_t av1 = w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4];
-        const uint32_t av2 = w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4];
-        // per-word patch booleans, bit k = patch w * 32 + k: tvs = vegsh == 0 or vbsh == 0;
-        // tsh = (1 - sh) * vbsh == 1; tsky = sh and vegsh
-        const uint32_t btvs = ~avs | ~(av1 | av2), btsh = ~ash & av1, btsky = ash & avs;
-        const int pend = min(w * 32 + 32, SW_NPATCH);
-        auto patch = [&](int p) {
-            const uint32_t k = (uint32_t)(p - w * 32);
-            const float tvs = (float)__builtin_amdgcn_ubfe(btvs, k, 1u);
-            const float tsky = (float)__builtin_amdgcn_ubfe(btsky, k, 1u);
-            const bool tsh = __builtin_amdgcn_ubfe(btsh, k, 1u) != 0u;
-            const sw_patch_k_t q = sp->pk[p];
-            const sw_patch_l_t u = sp->pl[p];
-            // sum + c * b with b in {0, 1}: fma(b, c, sum) rounds once, like the reference
-            if (DAY) {
+        ftvs[w] = ~mvs[w] | ~(mv1[w] | mv2[w]);
+        ftsky[w] = msh[w] & mvs[w];
+        ftsh[w] = ~msh[w] & mv1[w];
+    }
+    // flag-only sums by band counts, in band order
+#pragma unroll
+    for (int b = 0; b < SW_NBAND; ++b) {
+        const SW_CONST sw_patch_k_t &q = sp->pk[kBandStart[b]];
+        const float nsky = band_count(ftsky, b), nvs = band_count(ftvs, b);
+        Ldown_sky = __builtin_fmaf(nsky, q.lsky_down, Ldown_sky);
+        Lside_sky = __builtin_fmaf(nsky, q.lsky_side, Lside_sky);
+        Lside_veg = __builtin_fmaf(nvs, q.lv_side, Lside_veg);
+        Ldown_veg = __builtin_fmaf(nvs, q.lv_down, Ldown_veg);
+        if (DAY) Kref_veg = __builtin_fmaf(nvs, q.k_sh, Kref_veg);
+        if (!DAY) {
+            // night steps have no sun-facing window (l_sun == 0 and sunlit_l == 0 for every patch,
+            // solweig.py:1514-1611 with altitude <= 0): every shaded wall patch counts as shaded
+            const SW_CONST sw_patch_l_t &u = sp->pl[kBandStart[b]];
+            const float nsh = band_count(ftsh, b);
+            Lside_sh = __builtin_fmaf(nsh, u.lh_side, Lside_sh);
+            Ldown_sh = __builtin_fmaf(nsh, u.lh_down, Ldown_sh);
+        }
+    }
+    if (DAY) {
+#pragma unroll
+        for (int w = 0; w < SW_MASK_WORDS; ++w) {
+            const uint32_t ash = w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4];
+            const uint32_t avs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
+            const uint32_t btsh = w == 0 ? ftsh[0] : w == 1 ? ftsh[1] : w == 2 ? ftsh[2] : w == 3 ? ftsh[3] : ftsh[4];
+            const int pend = min(w * 32 + 32, SW_NPATCH);
+            // the diffuse sky (per-patch Perez luminance) and the wall patches (per-patch sun
+            // thresholds) stay per patch, each accumulator in the reference's patch order
+            auto patch = [&](int p) {
+                const uint32_t k = (uint32_t)(p - w * 32);
+                const bool tsh = __builtin_amdgcn_ubfe(btsh, k, 1u) != 0u;
                 const uint32_t st = __builtin_amdgcn_ubfe(ash, k, 1u) | (__builtin_amdgcn_ubfe(avs, k, 1u) << 1);
                 const float2 dv = s_diff[4 * p + (int)st];
                 aniLum = aniLum + dv.x;
                 KsideD = KsideD + dv.y;
-                Kref_veg = __builtin_fmaf(tvs, q.k_sh, Kref_veg);
-            }
-            Ldown_sky = __builtin_fmaf(tsky, q.lsky_down, Ldown_sky);
-            Lside_sky = __builtin_fmaf(tsky, q.lsky_side, Lside_sky);
-            Lside_veg = __builtin_fmaf(tvs, q.lv_side, Lside_veg);
-            Ldown_veg = __builtin_fmaf(tvs, q.lv_down, Ldown_veg);
-            // wall-patch terms need tsh; a wave whose lanes all have tsh == 0 adds
-            // only zeros there and skips them
-            if (wave_any(tsh)) {
-                if (DAY) {
+                // wall-patch terms need tsh; a wave whose lanes all have tsh == 0 adds
+                // only zeros there and skips them
+                if (wave_any(tsh)) {
+                    const sw_patch_k_t q = sp->pk[p];
+                    const sw_patch_l_t u = sp->pl[p];
                     const float x = hsvf + q.yi_k;
+                    // sum + c * b with b in {0, 1}: fma(b, c, sum) rounds once, like the reference
                     Kref_sun = __builtin_fmaf((tsh && x < q.sun_lt) ? 1.f : 0.f, q.k_sun, Kref_sun);
                     Kref_sh = __builtin_fmaf((tsh && x > q.sun_gt) ? 1.f : 0.f, q.k_sh, Kref_sh);
+                    // sun-facing window: sunlit / shaded by the threshold test; elsewhere every
+                    // shaded wall patch counts as shaded and none as sunlit
+                    const float xl = hsvf + u.yi_l;
+                    const bool ls = q.l_sun != 0;
+                    const float sunlit = (tsh && ls && (xl < q.sun_lt)) ? 1.f : 0.f;
+                    const float shaded = (tsh && (!ls || (xl > q.sun_gt))) ? 1.f : 0.f;
+                    Lside_sun = __builtin_fmaf(sunlit, u.ls_side, Lside_sun);
+                    Lside_sh = __builtin_fmaf(shaded, u.lh_side, Lside_sh);
+                    Ldown_sun = __builtin_fmaf(sunlit, u.ls_down, Ldown_sun);
+                    Ldown_sh = __builtin_fmaf(shaded, u.lh_down, Ldown_sh);
                 }
-                // sun-facing window: sunlit / shaded by the threshold test; elsewhere every
-                // shaded wall patch counts as shaded and none as sunlit
-                const float xl = hsvf + u.yi_l;
-                const bool ls = q.l_sun != 0;
-                const float sunlit = (tsh && ls && (xl < q.sun_lt)) ? 1.f : 0.f;
-                const float shaded = (tsh && (!ls || (xl > q.sun_gt))) ? 1.f : 0.f;
-                Lside_sun = __builtin_fmaf(sunlit, u.ls_side, Lside_sun);
-                Lside_sh = __builtin_fmaf(shaded, u.lh_side, Lside_sh);
-                Ldown_sun = __builtin_fmaf(sunlit, u.ls_down, Ldown_sun);
-                Ldown_sh = __builtin_fmaf(shaded, u.lh_down, Ldown_sh);
-            }
-        };
-        // ROLLED (the walking kernel, which holds both passes): no unrolling. Unrolled, its
-        // iterations keep several patches' scalar constants live at once; the register
-        // allocator then spills SGPRs to VGPR lanes and holds 114 VGPRs (4 waves per SIMD).
-        // Rolled, it needs 68 VGPRs (7 waves per SIMD), spills nothing and runs 7 % faster.
-        // The night-only kernel keeps 4 (its rolled form is slower).
-        if (ROLLED) {
+            };
+            // ROLLED (the walking kernel, which holds both passes): no unrolling. Unrolled, its
+            // iterations keep several patches' scalar constants live at once and the register
+            // allocator spills.
+            if (ROLLED) {
 #pragma unroll 1
-            for (int p = w * 32; p < pend; ++p) patch(p);
-        } else {
+                for (int p = w * 32; p < pend; ++p) patch(p);
+            } else {
 #pragma unroll 4
-            for (int p = w * 32; p < pend; ++p) patch(p);
+                for (int p = w * 32; p < pend; ++p) patch(p);
+            }
         }
     }
     o.aniLum = aniLum; o.KsideD = KsideD; o.Kref_sun = Kref_sun; o.Kref_sh = Kref_sh; o.Kref_veg = Kref_veg;
@@ -884,8 +923,8 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     // step and patch constants come precomputed (k_sh, k_sun, lv_*, ls_*,
     // lh_*) and are added where the patch's boolean factors are 1, which is
     // the value the full product takes (adding +0 elsewhere changes no sum
-    // that starts at +0 and only grows); each accumulator keeps the
-    // reference's patch order ----
+    // that starts at +0 and only grows); the flag-only sums go by band
+    // counts, the diffuse-sky and wall-patch sums per patch ----
     if (staged) {
         // every wave of the block is done with the walk tile: the step's diffuse-sky table replaces it
         __syncthreads();
@@ -920,24 +959,18 @@ __device__ __forceinline__ void flux_body(const uint32_t *__restrict__ masks, co
     // ---- pass B: reflected longwave ----
     const float pi_f = 3.14159265358979323846f;
     const float refl = ((Ldown_sky + Lup) * (1.f - md->ewall) * 0.5f) / pi_f;
+    // reflected term uses patches where any of sh, vegsh, vbsh is 0; per band, the reference's
+    // per-patch product (refl * ster) * cos | sin, added once per flagged patch (band counts)
+    uint32_t ftr[SW_MASK_WORDS];
 #pragma unroll
-    for (int w = 0; w < SW_MASK_WORDS; ++w) {
-        // reflected term uses patches where any of sh, vegsh, vbsh is 0
-        const uint32_t tr = ~((w == 0 ? msh[0] : w == 1 ? msh[1] : w == 2 ? msh[2] : w == 3 ? msh[3] : msh[4]) &
-                        (w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4]) &
-                        ((w == 0 ? mv1[0] : w == 1 ? mv1[1] : w == 2 ? mv1[2] : w == 3 ? mv1[3] : mv1[4]) |
-                         (w == 0 ? mv2[0] : w == 1 ? mv2[1] : w == 2 ? mv2[2] : w == 3 ? mv2[3] : mv2[4])));
-        const int pend = min(w * 32 + 32, SW_NPATCH);
-#pragma unroll 4
-        for (int p = w * 32; p < pend; ++p) {
-            const float trf = (float)__builtin_amdgcn_ubfe(tr, (uint32_t)(p - w * 32), 1u);
-            const sw_patch_k_t q = sp->pk[p];
-            // sum + ((refl * ster) * c) * trf with trf in {0, 1}: the product rounded in the
-            // reference's order, then added by an fma whose flag product is exact
-            const float rs = refl * q.ster;
-            Lside_ref = __builtin_fmaf(trf, rs * q.cos_alt, Lside_ref);
-            Ldown_ref = __builtin_fmaf(trf, rs * q.sin_alt, Ldown_ref);
-        }
+    for (int w = 0; w < SW_MASK_WORDS; ++w) ftr[w] = ~(msh[w] & mvs[w] & (mv1[w] | mv2[w]));
+#pragma unroll
+    for (int b = 0; b < SW_NBAND; ++b) {
+        const SW_CONST sw_patch_k_t &q = sp->pk[kBandStart[b]];
+        const float ntr = band_count(ftr, b);
+        const float rs = refl * q.ster;
+        Lside_ref = __builtin_fmaf(ntr, rs * q.cos_alt, Lside_ref);
+        Ldown_ref = __builtin_fmaf(ntr, rs * q.sin_alt, Ldown_ref);
     }
     const float Lside = Lside_sky + Lside_veg + Lside_sh + Lside_sun + Lside_ref;
     const float Ldown = Ldown_sky + Ldown_veg + Ldown_sh + Ldown_sun + Ldown_ref;

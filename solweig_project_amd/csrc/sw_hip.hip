@@ -453,7 +453,23 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
     const float hmax = funkey(stats[1]);
     const bool checked = stats[3] != 0u;
     const int tiles_x = (cols + SVF_TILE - 1) / SVF_TILE;
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    // Block -> (tile block bx, word w, part sub). With split > 1 the parts of one tile's words are
+    // the blocks L, L + 8, ..., L + 8 (split - 1) of the linear dispatch order L = x + gx y: the same
+    // XCD (workgroups go round-robin over the 8 XCDs) and dispatched together, so the partial writes
+    // of each mask line merge in that XCD's L2 before it is written back once. gx is a multiple of 8,
+    // so the map is a bijection and keeps bx % 8 == L % 8 (the XCD xcd_tile assumes).
+    int bx = blockIdx.x, w, sub;
+    if (split == 1) {
+        w = blockIdx.y;
+        sub = 0;
+    } else {
+        const int L = blockIdx.x + gridDim.x * blockIdx.y, grp = L >> 3;
+        const int q = (grp / split) * 8 + (L & 7);
+        sub = grp - (grp / split) * split;
+        w = q / (int)gridDim.x;
+        bx = q - w * (int)gridDim.x;
+    }
+    int tile = xcd_tile(bx, gridDim.x);
     if (tiles) {                               // listed tiles only (uniform per block)
         if (tile >= ntiles) return;
         tile = tiles[tile];
@@ -483,7 +499,6 @@ k_svf_march(const float *__restrict__ hpad, const int32_t *__restrict__ offs, co
     }
     // split 1: one block per mask word (32 patches); split 4: one block per byte of a word (8 patches,
     // short tile lists: four times the blocks, a quarter of the latency)
-    const int w = blockIdx.y / split, sub = blockIdx.y - w * split;
     const int p0 = w * 32;
     const int pb = p0 + sub * (32 / split);
     const int p1 = min(pb + 32 / split, SW_NPATCH);
