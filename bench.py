@@ -541,7 +541,10 @@ def run_surrogate(args):
             "network_ms": round(rep.ms_network, 3)}
     line = {
         "metric": "surrogate vs exact SVF (speedup of the gated surrogate path, 2000² city, 10 SVF fields)",
-        "value": round(exact_ms / wall, 3),
+        # the speed-up at the accuracy bar (max |dSVF| <= 0.02 and derived max |dTmrt| <= 0.5 K) of the
+        # fastest gate that meets it; the mean gate's speed-up, which misses the bar, stays beside it
+        "value": at_bar["speedup"] if at_bar else None,
+        "value_mean_gate": round(exact_ms / wall, 3),
         "unit": "x (exact ms / gated surrogate ms)",
         "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": round(wall, 3),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "bf16 (MFMA, f32 accumulate)",
@@ -1018,7 +1021,8 @@ def run_extras(args, extras, serve_pipeline):
             r = run_surrogate(args)
             bar = {"svf_max_abs": 0.02, "tmrt_max_abs_K": 0.5}
             ge, te = r["gated_error"]["max"], r["derived_tmrt_error"]["max_K"]
-            out["surrogate"] = {"speedup": r["value"], "exact_svf_ms": r["exact_svf_ms"], "gated_ms": r["gated_ms"],
+            out["surrogate"] = {"speedup_at_bar": r["value"], "speedup_mean_gate": r["value_mean_gate"],
+                                "exact_svf_ms": r["exact_svf_ms"], "gated_ms": r["gated_ms"],
                                 "gated_svf_error_max": ge, "derived_tmrt_error_max_K": te, "bar": bar,
                                 "meets_bar": bool(ge <= bar["svf_max_abs"] and te <= bar["tmrt_max_abs_K"]),
                                 "accepted_windows": f"{r['accepted']}/{r['windows']}", "gate": "mean, 4 tiles per core",
@@ -1026,7 +1030,8 @@ def run_extras(args, extras, serve_pipeline):
                                 "gates": [{k: g[k] for k in ("criterion", "samples_per_core", "speedup", "svf_error_max",
                                                              "tmrt_error_max_K", "meets_bar")} for g in r["gates"]],
                                 "params": r["config"]["params"], "site": "2000x2000",
-                                "fast_network": {"params": rf["config"]["params"], "speedup": rf["value"],
+                                "fast_network": {"params": rf["config"]["params"], "speedup_at_bar": rf["value"],
+                                                 "speedup_mean_gate": rf["value_mean_gate"],
                                                  "gated_svf_error_max": rf["gated_error"]["max"],
                                                  "derived_tmrt_error_max_K": rf["derived_tmrt_error"]["max_K"],
                                                  "at_bar": rf["at_bar"]}}
