@@ -223,15 +223,16 @@ __device__ __forceinline__ float pow4(float x) {
 }
 
 // Walk-source tile in LDS: a 16 x 16 pixel tile plus the walk halo. Once every wave of the block has
-// walked, the step's diffuse-sky patch terms (4 float2 per patch, one per (sh, vegsh) state) take
-// its place.
+// walked, the step's diffuse-sky terms of each pair of patches (16 float2 per pair, one per (sh, vegsh)
+// state of both) take its place.
+#define SW_NPAIR ((SW_NPATCH + 1) / 2)
 #define FLUX_TX 16
 #define FLUX_TY 16
 
 static inline size_t flux_lds_bytes(int walk_halo) {
     if (walk_halo < 0) return 0;
     const size_t tile = (size_t)(FLUX_TX + 2 * walk_halo) * (FLUX_TY + 2 * walk_halo);   // one flag byte per cell
-    const size_t table = (size_t)SW_NPATCH * 4 * sizeof(float2);   // overlays the tile after the walk
+    const size_t table = (size_t)SW_NPAIR * 16 * sizeof(float2);   // overlays the tile after the walk
     return tile > table ? tile : table;
 }
 
@@ -481,16 +482,27 @@ struct PassA {
 
 // The diffuse-sky terms of a day step per patch and (sh, vegsh) state st = sh | vegsh << 1:
 // {diffsh * lv, diffsh * lumchi * angl * ster}, diffsh = sh - (1 - vegsh) * 0.97 (solweig.py:2069-2073,
-// Kside_veg_v2022a), each product rounded in the reference's order. One LDS read per patch
-// replaces the state select and four products.
+// Kside_veg_v2022a), each product rounded in the reference's order, summed over the patch pair
+// (2j, 2j + 1) for each of the 16 states of the pair: entry 16 j + i, bits of i {sh_2j, sh_2j+1,
+// vegsh_2j, vegsh_2j+1}. One LDS read per pair replaces two; the reference adds the patches one at
+// a time, so the pair sum reassociates its sum (float32 rounding only).
+__device__ __forceinline__ float2 diffuse_term(const SW_CONST sw_step_t *__restrict__ sp, int p, int st) {
+    const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
+    const float diffsh = (st & 2) ? ((st & 1) ? 1.f : 0.f) : ((st & 1) ? d10 : d00);
+    const sw_patch_k_t q = sp->pk[p];
+    return make_float2(diffsh * q.lv, diffsh * q.lumchi * q.angl * q.ster);
+}
+
 __device__ __forceinline__ void diffuse_table(const SW_CONST sw_step_t *__restrict__ sp, float2 *__restrict__ tab) {
     const int tid = sw_tid();
-    const float d00 = 0.f - (1.f - 0.f) * 0.97f, d10 = 1.f - (1.f - 0.f) * 0.97f;  // diffsh for vegsh == 0
-    for (int i = tid; i < SW_NPATCH * 4; i += blockDim.x) {
-        const int p = i >> 2, st = i & 3;
-        const float diffsh = (st & 2) ? ((st & 1) ? 1.f : 0.f) : ((st & 1) ? d10 : d00);
-        const sw_patch_k_t q = sp->pk[p];
-        tab[i] = make_float2(diffsh * q.lv, diffsh * q.lumchi * q.angl * q.ster);
+    for (int i = tid; i < SW_NPAIR * 16; i += blockDim.x) {
+        const int j = i >> 4, b = i & 15, p = 2 * j;
+        float2 v = diffuse_term(sp, p, (b & 1) | ((b >> 1) & 2));
+        if (p + 1 < SW_NPATCH) {   // the last pair has one patch: its entries ignore the second's bits
+            const float2 c = diffuse_term(sp, p + 1, ((b >> 1) & 1) | ((b >> 2) & 2));
+            v = make_float2(v.x + c.x, v.y + c.y);
+        }
+        tab[i] = v;
     }
 }
 
@@ -560,15 +572,10 @@ __device__ __forceinline__ void patch_pass_a(const SW_CONST sw_step_t *__restric
             const uint32_t avs = w == 0 ? mvs[0] : w == 1 ? mvs[1] : w == 2 ? mvs[2] : w == 3 ? mvs[3] : mvs[4];
             const uint32_t btsh = w == 0 ? ftsh[0] : w == 1 ? ftsh[1] : w == 2 ? ftsh[2] : w == 3 ? ftsh[3] : ftsh[4];
             const int pend = min(w * 32 + 32, SW_NPATCH);
-            // the diffuse sky (per-patch Perez luminance) and the wall patches (per-patch sun
-            // thresholds) stay per patch, each accumulator in the reference's patch order
-            auto patch = [&](int p) {
-                const uint32_t k = (uint32_t)(p - w * 32);
+            // the diffuse sky (per-patch Perez luminance) by patch pairs, the wall patches
+            // (per-patch sun thresholds) per patch in the reference's patch order
+            auto wall = [&](int p, uint32_t k) {
                 const bool tsh = __builtin_amdgcn_ubfe(btsh, k, 1u) != 0u;
-                const uint32_t st = __builtin_amdgcn_ubfe(ash, k, 1u) | (__builtin_amdgcn_ubfe(avs, k, 1u) << 1);
-                const float2 dv = s_diff[4 * p + (int)st];
-                aniLum = aniLum + dv.x;
-                KsideD = KsideD + dv.y;
                 // wall-patch terms need tsh; a wave whose lanes all have tsh == 0 adds
                 // only zeros there and skips them
                 if (wave_any(tsh)) {
@@ -590,15 +597,24 @@ __device__ __forceinline__ void patch_pass_a(const SW_CONST sw_step_t *__restric
                     Ldown_sh = __builtin_fmaf(shaded, u.lh_down, Ldown_sh);
                 }
             };
+            auto pair = [&](int p) {   // p even: patches p and p + 1 share a word
+                const uint32_t k = (uint32_t)(p - w * 32);
+                const uint32_t st = __builtin_amdgcn_ubfe(ash, k, 2u) | (__builtin_amdgcn_ubfe(avs, k, 2u) << 2);
+                const float2 dv = s_diff[8 * p + (int)st];   // 16 (p / 2) + st
+                aniLum = aniLum + dv.x;
+                KsideD = KsideD + dv.y;
+                wall(p, k);
+                if (p + 1 < pend) wall(p + 1, k + 1u);
+            };
             // ROLLED (the walking kernel, which holds both passes): no unrolling. Unrolled, its
             // iterations keep several patches' scalar constants live at once and the register
             // allocator spills.
             if (ROLLED) {
 #pragma unroll 1
-                for (int p = w * 32; p < pend; ++p) patch(p);
+                for (int p = w * 32; p < pend; p += 2) pair(p);
             } else {
-#pragma unroll 4
-                for (int p = w * 32; p < pend; ++p) patch(p);
+#pragma unroll 2
+                for (int p = w * 32; p < pend; p += 2) pair(p);
             }
         }
     }
